@@ -1,0 +1,23 @@
+"""MI355X-native drop-in for DiffEqGPU's EnsembleGPUKernel path (GPUTsit5, GPUVern7,
+GPURosenbrock23), over the C ABI of libdeghip.so (include/deghip.h).
+
+Import under any module name (the directory name is not a Python identifier); the test
+suite, the benchmark and the build entry point load it as ``diffeqgpu_amd``.
+"""
+from .algorithms import (EnsembleGPUKernel, EnsembleKernelAlgorithm, GPUODEAlgorithm, GPURosenbrock23, GPUTsit5,
+                         GPUVern7, HIPDevice, alg_order)
+from .ensemble import EnsembleContext, EnsembleSolution, ODESolution, solve
+from .lowerlevel import SolveResult, vectorized_asolve, vectorized_solve
+from .problems import (EnsembleProblem, HIPProbVector, ImmutableODEProblem, ODEFunction, ODEProblem, adapt,
+                       get_backend, make_prob_compatible, remake)
+from .ranges import colon, linrange
+from .tracer import SVector
+from . import tracer as math  # sin, cos, exp, log, sqrt, tanh, fabs, pow, ifelse on traced values
+
+__all__ = [
+    "EnsembleGPUKernel", "EnsembleKernelAlgorithm", "GPUODEAlgorithm", "GPURosenbrock23", "GPUTsit5", "GPUVern7",
+    "HIPDevice", "alg_order", "EnsembleContext", "EnsembleSolution", "ODESolution", "solve", "SolveResult",
+    "vectorized_asolve", "vectorized_solve", "EnsembleProblem", "HIPProbVector", "ImmutableODEProblem",
+    "ODEFunction", "ODEProblem", "adapt", "get_backend", "make_prob_compatible", "remake", "colon", "linrange",
+    "SVector", "math",
+]

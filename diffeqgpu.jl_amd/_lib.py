@@ -1,0 +1,210 @@
+"""ctypes binding of libdeghip.so (C ABI declared in include/deghip.h).
+
+The library is built in-tree by ``csrc/build.py``; nothing here falls back to another
+implementation: if the library is missing, loading fails with an error that says how to
+build it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "_build", "libdeghip.so")
+CACHE_DIR = os.path.join(PKG_DIR, "_jitcache")
+
+DEG_OK = 0
+DEG_TSIT5, DEG_VERN7, DEG_RB23 = 0, 1, 2
+F_ADAPTIVE = 1 << 0
+F_PARITY = 1 << 1
+F_DETPOW = 1 << 2
+F_KAHAN = 1 << 3
+F_PERSISTENT = 1 << 4
+F_LEARNED = 1 << 5
+F_FASTPOW = 1 << 6
+F_STEPLOG = 1 << 7
+
+RET_SUCCESS, RET_DTMIN, RET_MAXITERS, RET_NONFINITE = 0, 1, 2, 3
+RETCODE_NAMES = {RET_SUCCESS: "Success", RET_DTMIN: "DtLessThanMin", RET_MAXITERS: "MaxIters",
+                 RET_NONFINITE: "Unstable"}
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_up = ctypes.POINTER(ctypes.c_uint32)
+
+
+class SolveArgs(ctypes.Structure):
+    """Mirror of ``deg_solve_args``."""
+
+    _fields_ = [
+        ("struct_size", ctypes.c_uint32),
+        ("n", ctypes.c_int32),
+        ("n_traj", ctypes.c_int64),
+        ("stride", ctypes.c_int64),
+        ("id_offset", ctypes.c_int64),
+        ("u0", ctypes.c_void_p),
+        ("p", ctypes.c_void_p),
+        ("t0s", ctypes.c_void_p),
+        ("tfs", ctypes.c_void_p),
+        ("t0", ctypes.c_float),
+        ("tf", ctypes.c_float),
+        ("dt", ctypes.c_float),
+        ("abstol", ctypes.c_float),
+        ("reltol", ctypes.c_float),
+        ("saveat", ctypes.c_void_p),
+        ("nsaveat", ctypes.c_int32),
+        ("nsave", ctypes.c_int32),
+        ("save_everystep", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+        ("ts", ctypes.c_void_p),
+        ("us", ctypes.c_void_p),
+        ("retcode", ctypes.c_void_p),
+        ("naccept", ctypes.c_void_p),
+        ("nreject", ctypes.c_void_p),
+        ("max_attempts", ctypes.c_int64),
+        ("stream", ctypes.c_void_p),
+        ("grid_blocks", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
+        ("elapsed_ms", ctypes.c_void_p),
+        ("mlp", ctypes.c_void_p),
+        ("steplog", ctypes.c_void_p),
+        ("steplog_count", ctypes.c_void_p),
+        ("steplog_cap", ctypes.c_int64),
+    ]
+
+
+EXPORTS = [
+    "deg_ctx_create", "deg_ctx_destroy", "deg_ctx_num_cus", "deg_set_cache_dir", "deg_kernel_compile",
+    "deg_kernel_destroy", "deg_kernel_attr", "deg_jit_precompile", "deg_jit_source", "deg_solve", "deg_sync",
+    "deg_last_error", "deg_version", "deg_solve_args_size",
+]
+
+_lock = threading.Lock()
+_lib = None
+
+
+class DegError(RuntimeError):
+    pass
+
+
+def _declare(lib):
+    vp = ctypes.c_void_p
+    lib.deg_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    lib.deg_ctx_destroy.argtypes = [vp]
+    lib.deg_ctx_num_cus.argtypes = [vp]
+    lib.deg_set_cache_dir.argtypes = [ctypes.c_char_p]
+    lib.deg_kernel_compile.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_uint32, ctypes.POINTER(vp)]
+    lib.deg_kernel_destroy.argtypes = [vp]
+    lib.deg_kernel_attr.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.deg_jit_precompile.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                       ctypes.c_char_p]
+    lib.deg_jit_source.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                   ctypes.c_char_p, ctypes.c_int64]
+    lib.deg_jit_source.restype = ctypes.c_int64
+    lib.deg_solve.argtypes = [vp, ctypes.POINTER(SolveArgs)]
+    lib.deg_sync.argtypes = [vp]
+    lib.deg_last_error.restype = ctypes.c_char_p
+    lib.deg_version.restype = ctypes.c_char_p
+    lib.deg_solve_args_size.restype = ctypes.c_uint32
+    for name in ("deg_ctx_create", "deg_ctx_destroy", "deg_ctx_num_cus", "deg_set_cache_dir", "deg_kernel_compile",
+                 "deg_kernel_destroy", "deg_kernel_attr", "deg_jit_precompile", "deg_solve", "deg_sync"):
+        getattr(lib, name).restype = ctypes.c_int
+
+
+def lib():
+    """Load libdeghip.so once (raises DegError when it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise DegError(f"{LIB_PATH} is missing: build it with `python __graft_entry__.py` "
+                               "(or diffeqgpu.jl_amd/csrc/build.py); the HIP path has no fallback")
+            L = ctypes.CDLL(LIB_PATH)
+            _declare(L)
+            if L.deg_solve_args_size() != ctypes.sizeof(SolveArgs):
+                raise DegError("deg_solve_args layout mismatch between libdeghip.so and the Python binding")
+            os.makedirs(CACHE_DIR, exist_ok=True)
+            L.deg_set_cache_dir(CACHE_DIR.encode())
+            _lib = L
+        return _lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != DEG_OK:
+        msg = lib().deg_last_error().decode(errors="replace")
+        raise DegError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def precompile(rhs_src: str, n: int, np_: int, method: int, flags: int) -> str:
+    key = ctypes.create_string_buffer(32)
+    check(lib().deg_jit_precompile(rhs_src.encode(), n, np_, method, flags, key), "deg_jit_precompile")
+    return key.value.decode()
+
+
+def jit_source(rhs_src: str, n: int, np_: int, method: int, flags: int) -> str:
+    L = lib()
+    size = L.deg_jit_source(rhs_src.encode(), n, np_, method, flags, None, 0)
+    buf = ctypes.create_string_buffer(int(size) + 1)
+    L.deg_jit_source(rhs_src.encode(), n, np_, method, flags, buf, size + 1)
+    return buf.value.decode()
+
+
+class Context:
+    """A device context of libdeghip (one per HIP device)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = ctypes.c_void_p()
+        check(L.deg_ctx_create(device, ctypes.byref(h)), "deg_ctx_create")
+        self.handle = h
+        self.device = device
+        self.num_cus = L.deg_ctx_num_cus(h)
+        self._kernels = {}
+
+    def kernel(self, rhs_src: str, n: int, np_: int, method: int, flags: int):
+        key = (rhs_src, n, np_, method, flags)
+        k = self._kernels.get(key)
+        if k is None:
+            h = ctypes.c_void_p()
+            check(lib().deg_kernel_compile(self.handle, rhs_src.encode(), n, np_, method, flags, ctypes.byref(h)),
+                  "deg_kernel_compile")
+            k = Kernel(h, n, np_, method, flags)
+            self._kernels[key] = k
+        return k
+
+    def sync(self):
+        check(lib().deg_sync(self.handle), "deg_sync")
+
+
+class Kernel:
+    def __init__(self, handle, n, np_, method, flags):
+        self.handle = handle
+        self.n = n
+        self.np = np_
+        self.method = method
+        self.flags = flags
+
+    def attr(self, which: int) -> int:
+        v = ctypes.c_int()
+        check(lib().deg_kernel_attr(self.handle, which, ctypes.byref(v)), "deg_kernel_attr")
+        return v.value
+
+    def solve(self, args: SolveArgs):
+        args.struct_size = ctypes.sizeof(SolveArgs)
+        check(lib().deg_solve(self.handle, ctypes.byref(args)), "deg_solve")
+
+
+_contexts = {}
+
+
+def context(device: int = 0) -> Context:
+    with _lock:
+        c = _contexts.get(device)
+    if c is None:
+        c = Context(device)
+        with _lock:
+            _contexts[device] = c
+    return c

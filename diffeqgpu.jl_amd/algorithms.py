@@ -1,0 +1,131 @@
+"""Algorithm selectors and the ensemble algorithm.
+
+Mirrors src/ensemblegpukernel/gpukernel_algorithms.jl (GPUTsit5 :30, GPUVern7 :57,
+GPURosenbrock23 :112 with the ``autodiff`` keyword :214-220), alg_utils.jl:1-13
+(``alg_order``) and src/algorithms.jl:192-211 (``EnsembleGPUKernel(dev, cpu_offload)``).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+from . import _lib
+
+
+class GPUODEAlgorithm:
+    method: int = -1
+    order: int = 0
+
+    def __repr__(self):
+        return f"{type(self).__name__}()"
+
+    def __eq__(self, other):
+        return type(self) is type(other) and vars(self) == vars(other)
+
+    def __hash__(self):
+        return hash((type(self).__name__, tuple(sorted(vars(self).items()))))
+
+
+class GPUTsit5(GPUODEAlgorithm):
+    """Tsitouras 5(4) with FSAL (gpukernel_algorithms.jl:30)."""
+
+    method = _lib.DEG_TSIT5
+    order = 5
+
+
+class GPUVern7(GPUODEAlgorithm):
+    """Verner 7(6), not FSAL (gpukernel_algorithms.jl:57)."""
+
+    method = _lib.DEG_VERN7
+    order = 7
+
+
+class GPURosenbrock23(GPUODEAlgorithm):
+    """Rosenbrock 2(3) (gpukernel_algorithms.jl:112, :214-220). ``autodiff=True`` derives the
+    Jacobian by forward-mode differentiation of the traced RHS (unless the problem's
+    ODEFunction supplies ``jac``/``tgrad``); ``autodiff=False`` is not provided (the reference
+    then uses finite differences, alg_utils.jl:17-27)."""
+
+    method = _lib.DEG_RB23
+    order = 2
+
+    def __init__(self, autodiff: bool = True):
+        if not autodiff:
+            raise NotImplementedError("GPURosenbrock23(autodiff=False) (finite-difference Jacobian) is not provided")
+        self.autodiff = True
+
+
+def alg_order(alg: GPUODEAlgorithm) -> int:
+    """alg_utils.jl:1-13."""
+    if not isinstance(alg, GPUODEAlgorithm) or alg.order == 0:
+        raise ValueError("Order is not defined for this algorithm")
+    return alg.order
+
+
+@dataclass(frozen=True)
+class HIPDevice:
+    """The backend object handed to ``EnsembleGPUKernel`` (the role of ``ROCBackend()``).
+
+    Knobs (all optional):
+      device      HIP device ordinal.
+      contract    "off": no FP contraction (the reference's unfused Float32 sequence; bitwise
+                  comparable with the CPU restatement). "fast": FMA contraction. None: "off" for
+                  fixed-step solves, "fast" for adaptive solves.
+      powmode     controller powers: "ocml" (device powf, the same library call AMDGPU.jl makes
+                  for Float32 ^), "fast" (hardware log2/exp2) or "det" (deterministic polynomial).
+      persistent  persistent grid with a device work counter (lanes refill as trajectories end);
+                  None picks it for adaptive solves.
+      kahan       compensated accumulation of t.
+      block_size  threads per block (64..256); None: 256 or the autotuned value.
+      max_attempts per-trajectory attempt cap (0: 10^7). The reference has none and loops forever
+                  when t + dt rounds to t; here such a lane ends with retcode MaxIters.
+    """
+
+    device: int = 0
+    contract: str | None = None
+    powmode: str = "ocml"
+    persistent: bool | None = None
+    kahan: bool = False
+    block_size: int | None = None
+    max_attempts: int = 0
+
+    def flags(self, adaptive: bool) -> int:
+        f = 0
+        if adaptive:
+            f |= _lib.F_ADAPTIVE
+        contract = self.contract or ("fast" if adaptive else "off")
+        if contract not in ("off", "fast"):
+            raise ValueError("contract must be 'off' or 'fast'")
+        if contract == "off":
+            f |= _lib.F_PARITY
+        if self.powmode == "det":
+            f |= _lib.F_DETPOW
+        elif self.powmode == "fast":
+            f |= _lib.F_FASTPOW
+        elif self.powmode != "ocml":
+            raise ValueError("powmode must be 'ocml', 'fast' or 'det'")
+        if self.kahan:
+            f |= _lib.F_KAHAN
+        persistent = self.persistent if self.persistent is not None else adaptive
+        if persistent:
+            f |= _lib.F_PERSISTENT
+        return f
+
+
+class EnsembleKernelAlgorithm:
+    """Abstract type of the kernel ensemble algorithms (src/DiffEqGPU.jl:107)."""
+
+
+class EnsembleGPUKernel(EnsembleKernelAlgorithm):
+    """``EnsembleGPUKernel(dev, cpu_offload = 0.0)`` (src/algorithms.jl:192-211).
+
+    ``cpu_offload`` must be 0.0: this build has no CPU solver path (the reference offloads a
+    fraction of trajectories to SimpleDiffEq on host threads, solve.jl:37-81)."""
+
+    def __init__(self, dev: HIPDevice | None = None, cpu_offload: float = 0.0):
+        self.dev = dev if dev is not None else HIPDevice()
+        if cpu_offload != 0.0:
+            raise NotImplementedError("cpu_offload > 0 is not provided: every trajectory runs on the GPU")
+        self.cpu_offload = 0.0
+
+    def __repr__(self):
+        return f"EnsembleGPUKernel({self.dev!r}, {self.cpu_offload})"

@@ -1,0 +1,653 @@
+// Per-lane ensemble ODE solver for gfx950, compiled at run time by hiprtc together with
+// a traced right-hand side. One trajectory per lane; state, stages and controller in
+// VGPRs; tableau coefficients are compile-time literals.
+//
+// Reference behaviour restated here (DiffEqGPU src/ensemblegpukernel/):
+//   kernels.jl:1-72 / :74-152          per-trajectory loop, initial save, endpoint rules
+//   perform_step/gpu_tsit5_perform_step.jl:1-175, gpu_vern7_perform_step.jl:1-210,
+//   perform_step/gpu_rosenbrock23_perform_step.jl:1-201   one step / one attempt
+//   integrators/integrator_utils.jl:1-50   PI constants, savevalues!
+//   integrators/nonstiff/interpolants.jl:372-387, stiff/interpolants.jl   dense output
+//
+// Configuration macros supplied by the host before this file:
+//   DEG_N, DEG_NP                state and parameter counts
+//   DEG_METHOD                   0 Tsit5, 1 Vern7, 2 Rosenbrock23
+//   DEG_ADAPTIVE                 0 fixed step, 1 adaptive (PI controller)
+//   DEG_DETPOW                   controller powers via det_pow (bitwise-reproducible on CPU)
+//   DEG_FASTPOW                  controller powers via hardware log2/exp2
+//   DEG_KAHAN                    compensated accumulation of t
+//   DEG_PERSISTENT               lanes refill from a device work counter
+//   DEG_LEARNED                  MLP step-size proposal (see deg_mlp.h)
+//   DEG_KERNEL                   kernel symbol name
+//   DEG_BLOCK_MAX                launch-bounds block size
+// and the functions deg_rhs(u, p, t, du) (+ deg_jac, deg_tgrad for Rosenbrock23).
+#pragma once
+
+#define DEG_RET_SUCCESS 0
+#define DEG_RET_DTMIN 1
+#define DEG_RET_MAXITERS 2
+#define DEG_RET_NONFINITE 3
+
+#define DEG_FOR(c) _Pragma("unroll") for (int c = 0; c < DEG_N; ++c)
+
+// ------------------------------------------------------------------------------------
+// Julia Float32 semantics
+// ------------------------------------------------------------------------------------
+// Base.min / Base.max on IEEE floats propagate NaN.
+__device__ __forceinline__ float jl_min(float a, float b) {
+    return (a != a || b != b) ? a + b : ((b < a) ? b : a);
+}
+__device__ __forceinline__ float jl_max(float a, float b) {
+    return (a != a || b != b) ? a + b : ((b > a) ? b : a);
+}
+
+// Deterministic x^y (x >= 0) from IEEE +,-,*,/ and exponent bit manipulation: log2 by the
+// atanh series on a mantissa in [sqrt(1/2), sqrt(2)], exp2 by a degree-7 Taylor
+// polynomial on [-ln2/2, ln2/2]. Relative error about 1e-6. Evaluated without contraction
+// so that a CPU build of the same expression gives the same bits.
+__device__ __forceinline__ float det_exp2i(int k) {
+    return __builtin_bit_cast(float, (unsigned int)(k + 127) << 23);
+}
+__device__ __noinline__ float det_pow(float x, float y) {
+#pragma clang fp contract(off)
+    if (x != x || y != y) return x + y;
+    if (x == 0.0f) return 0.0f;
+    if (x == __builtin_inff()) return __builtin_inff();
+    unsigned int b = __builtin_bit_cast(unsigned int, x);
+    int e = (int)((b >> 23) & 0xffu);
+    if (e == 0) {
+        x = x * 16777216.0f;
+        b = __builtin_bit_cast(unsigned int, x);
+        e = (int)((b >> 23) & 0xffu) - 24;
+    }
+    e -= 127;
+    float m = __builtin_bit_cast(float, (b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float lnm = (2.0f * s) * (1.0f + s2 * (0.333333343f + s2 * (0.2f + s2 * (0.142857149f + s2 * 0.111111112f))));
+    float l2 = (float)e + lnm * 1.44269502f;
+    float z = y * l2;
+    if (z > 128.0f) return __builtin_inff();
+    if (z < -149.0f) return 0.0f;
+    float n = __builtin_rintf(z);
+    float g = (z - n) * 0.693147182f;
+    float r = 1.0f + g * (1.0f + g * (0.5f + g * (0.166666672f + g * (0.0416666679f + g * (0.00833333377f + g * (0.00138888892f + g * 0.000198412701f))))));
+    int ni = (int)n;
+    int h1 = ni / 2;
+    int h2 = ni - h1;
+    return (r * det_exp2i(h1)) * det_exp2i(h2);
+}
+
+__device__ __forceinline__ float ctl_pow(float x, float y) {
+#if DEG_DETPOW
+    return det_pow(x, y);
+#elif DEG_FASTPOW
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+#else
+    return powf(x, y);
+#endif
+}
+
+// DiffEqBase.ODE_DEFAULT_NORM for an SVector: sqrt(((x1^2 + x2^2) + ...) / n)
+__device__ __forceinline__ float rms_norm(const float (&x)[DEG_N]) {
+    float acc = x[0] * x[0];
+    _Pragma("unroll") for (int c = 1; c < DEG_N; ++c) acc = acc + x[c] * x[c];
+    return __builtin_sqrtf(acc / (float)DEG_N);
+}
+
+// error scaling: tmp ./ (abstol .+ max.(abs.(uprev), abs.(u)) * reltol)
+__device__ __forceinline__ float scaled_error(float (&e)[DEG_N], const float (&uprev)[DEG_N],
+                                              const float (&u)[DEG_N], float abstol, float reltol) {
+    DEG_FOR(c) {
+        float sc = abstol + jl_max(__builtin_fabsf(uprev[c]), __builtin_fabsf(u[c])) * reltol;
+        e[c] = e[c] / sc;
+    }
+    return rms_norm(e);
+}
+
+// ------------------------------------------------------------------------------------
+// PI controller (integrator_utils.jl:1-11; gpu_tsit5_perform_step.jl:124-137)
+// ------------------------------------------------------------------------------------
+#if DEG_METHOD == 0
+#define DEG_ORDER 5
+#elif DEG_METHOD == 1
+#define DEG_ORDER 7
+#else
+#define DEG_ORDER 2
+#endif
+#define DEG_BETA1 ((float)(7.0 / (10.0 * DEG_ORDER)))
+#define DEG_BETA2 ((float)(2.0 / (5.0 * DEG_ORDER)))
+#define DEG_QMAX 10.0f
+#define DEG_QMIN ((float)(1.0 / 5.0))
+#define DEG_GAMMA ((float)(9.0 / 10.0))
+#define DEG_QOLDINIT ((float)1.0e-4)
+#define DEG_DTMIN 1.0e-14f
+
+// ------------------------------------------------------------------------------------
+// Tsit5 (gpu_tsit5_perform_step.jl:36-49, :104-122; interpolants.jl:372-387)
+// ------------------------------------------------------------------------------------
+#if DEG_METHOD == 0
+#define DEG_NK 7
+__device__ __forceinline__ void deg_attempt(const float (&uprev)[DEG_N], const float* p, float t, float dt,
+                                            float (&k)[DEG_NK + 1][DEG_N], float (&unew)[DEG_N]) {
+    float tmp[DEG_N];
+    const float a = dt * T5_a21;
+    DEG_FOR(c) tmp[c] = uprev[c] + a * k[1][c];
+    deg_rhs(tmp, p, t + T5_c1 * dt, k[2]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (T5_a31 * k[1][c] + T5_a32 * k[2][c]);
+    deg_rhs(tmp, p, t + T5_c2 * dt, k[3]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (T5_a41 * k[1][c] + T5_a42 * k[2][c] + T5_a43 * k[3][c]);
+    deg_rhs(tmp, p, t + T5_c3 * dt, k[4]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (T5_a51 * k[1][c] + T5_a52 * k[2][c] + T5_a53 * k[3][c] + T5_a54 * k[4][c]);
+    deg_rhs(tmp, p, t + T5_c4 * dt, k[5]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (T5_a61 * k[1][c] + T5_a62 * k[2][c] + T5_a63 * k[3][c] + T5_a64 * k[4][c] +
+                                         T5_a65 * k[5][c]);
+    deg_rhs(tmp, p, t + dt, k[6]);
+    DEG_FOR(c) unew[c] = uprev[c] + dt * (T5_a71 * k[1][c] + T5_a72 * k[2][c] + T5_a73 * k[3][c] + T5_a74 * k[4][c] +
+                                          T5_a75 * k[5][c] + T5_a76 * k[6][c]);
+    deg_rhs(unew, p, t + dt, k[7]);
+}
+__device__ __forceinline__ float deg_error(float dt, const float (&k)[DEG_NK + 1][DEG_N], const float (&uprev)[DEG_N],
+                                           const float (&unew)[DEG_N], float abstol, float reltol) {
+    float e[DEG_N];
+    DEG_FOR(c) e[c] = dt * (T5_bt1 * k[1][c] + T5_bt2 * k[2][c] + T5_bt3 * k[3][c] + T5_bt4 * k[4][c] +
+                            T5_bt5 * k[5][c] + T5_bt6 * k[6][c] + T5_bt7 * k[7][c]);
+    return scaled_error(e, uprev, unew, abstol, reltol);
+}
+// y0 + dt * (b1(th) k1 + ... + b7(th) k7) with b_i evaluated by fused Horner steps
+__device__ __forceinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                           const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
+    (void)tprev;
+    (void)p;
+    const float b1 = th * __builtin_fmaf(th, __builtin_fmaf(th, __builtin_fmaf(th, T5_r14, T5_r13), T5_r12), T5_r11);
+    const float b2 = th * (th * __builtin_fmaf(th, __builtin_fmaf(th, T5_r24, T5_r23), T5_r22));
+    const float b3 = th * (th * __builtin_fmaf(th, __builtin_fmaf(th, T5_r34, T5_r33), T5_r32));
+    const float b4 = th * (th * __builtin_fmaf(th, __builtin_fmaf(th, T5_r44, T5_r43), T5_r42));
+    const float b5 = th * (th * __builtin_fmaf(th, __builtin_fmaf(th, T5_r54, T5_r53), T5_r52));
+    const float b6 = th * (th * __builtin_fmaf(th, __builtin_fmaf(th, T5_r64, T5_r63), T5_r62));
+    const float b7 = th * (th * __builtin_fmaf(th, __builtin_fmaf(th, T5_r74, T5_r73), T5_r72));
+    DEG_FOR(c) out[c] = y0[c] + dt * (b1 * k[1][c] + b2 * k[2][c] + b3 * k[3][c] + b4 * k[4][c] + b5 * k[5][c] +
+                                      b6 * k[6][c] + b7 * k[7][c]);
+}
+#endif
+
+// ------------------------------------------------------------------------------------
+// Vern7 (gpu_vern7_perform_step.jl:30-56, :117-155); not FSAL: k1 recomputed per attempt
+// ------------------------------------------------------------------------------------
+#if DEG_METHOD == 1
+#define DEG_NK 10
+__device__ __forceinline__ void deg_attempt(const float (&uprev)[DEG_N], const float* p, float t, float dt,
+                                            float (&k)[DEG_NK + 1][DEG_N], float (&unew)[DEG_N]) {
+    float tmp[DEG_N];
+    const float a = dt * V7_a021;
+    DEG_FOR(c) tmp[c] = uprev[c] + a * k[1][c];
+    deg_rhs(tmp, p, t + V7_c2 * dt, k[2]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V7_a031 * k[1][c] + V7_a032 * k[2][c]);
+    deg_rhs(tmp, p, t + V7_c3 * dt, k[3]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V7_a041 * k[1][c] + V7_a043 * k[3][c]);
+    deg_rhs(tmp, p, t + V7_c4 * dt, k[4]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V7_a051 * k[1][c] + V7_a053 * k[3][c] + V7_a054 * k[4][c]);
+    deg_rhs(tmp, p, t + V7_c5 * dt, k[5]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V7_a061 * k[1][c] + V7_a063 * k[3][c] + V7_a064 * k[4][c] + V7_a065 * k[5][c]);
+    deg_rhs(tmp, p, t + V7_c6 * dt, k[6]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V7_a071 * k[1][c] + V7_a073 * k[3][c] + V7_a074 * k[4][c] + V7_a075 * k[5][c] +
+                                         V7_a076 * k[6][c]);
+    deg_rhs(tmp, p, t + V7_c7 * dt, k[7]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V7_a081 * k[1][c] + V7_a083 * k[3][c] + V7_a084 * k[4][c] + V7_a085 * k[5][c] +
+                                         V7_a086 * k[6][c] + V7_a087 * k[7][c]);
+    deg_rhs(tmp, p, t + V7_c8 * dt, k[8]);
+    float g10[DEG_N];
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V7_a091 * k[1][c] + V7_a093 * k[3][c] + V7_a094 * k[4][c] + V7_a095 * k[5][c] +
+                                         V7_a096 * k[6][c] + V7_a097 * k[7][c] + V7_a098 * k[8][c]);
+    DEG_FOR(c) g10[c] = uprev[c] + dt * (V7_a101 * k[1][c] + V7_a103 * k[3][c] + V7_a104 * k[4][c] + V7_a105 * k[5][c] +
+                                         V7_a106 * k[6][c] + V7_a107 * k[7][c]);
+    deg_rhs(tmp, p, t + dt, k[9]);
+    deg_rhs(g10, p, t + dt, k[10]);
+    DEG_FOR(c) unew[c] = uprev[c] + dt * (V7_b1 * k[1][c] + V7_b4 * k[4][c] + V7_b5 * k[5][c] + V7_b6 * k[6][c] +
+                                          V7_b7 * k[7][c] + V7_b8 * k[8][c] + V7_b9 * k[9][c]);
+}
+__device__ __forceinline__ float deg_error(float dt, const float (&k)[DEG_NK + 1][DEG_N], const float (&uprev)[DEG_N],
+                                           const float (&unew)[DEG_N], float abstol, float reltol) {
+    float e[DEG_N];
+    DEG_FOR(c) e[c] = dt * (V7_btilde1 * k[1][c] + V7_btilde4 * k[4][c] + V7_btilde5 * k[5][c] + V7_btilde6 * k[6][c] +
+                            V7_btilde7 * k[7][c] + V7_btilde8 * k[8][c] + V7_btilde9 * k[9][c] +
+                            V7_btilde10 * k[10][c]);
+    return scaled_error(e, uprev, unew, abstol, reltol);
+}
+// Vern7 dense output (interpolants.jl:28-190): extra stages k11..k16 from uprev, then
+// y0 + th*step_sum + th*(th-1)*correction with q_i(th) from the synthetic division of the
+// r coefficients. Scalar muladds (stage times, Horner steps) are fused.
+__device__ __forceinline__ float v7q(float th, float r2, float r3, float r4, float r5, float r6, float r7) {
+    const float s5 = r7;
+    const float s4 = r6 + s5;
+    const float s3 = r5 + s4;
+    const float s2 = r4 + s3;
+    const float s1 = r3 + s2;
+    const float c0 = r2 + s1;
+    return __builtin_fmaf(th, __builtin_fmaf(th, __builtin_fmaf(th, __builtin_fmaf(th, __builtin_fmaf(th, s5, s4), s3), s2), s1), c0);
+}
+#define V7Q(i) v7q(th, V7_r##i##2, V7_r##i##3, V7_r##i##4, V7_r##i##5, V7_r##i##6, V7_r##i##7)
+__device__ __noinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                        const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
+    float tmp[DEG_N], k11[DEG_N], k12[DEG_N], k13[DEG_N], k14[DEG_N], k15[DEG_N], k16[DEG_N];
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V7_a1101 * k[1][c] + V7_a1104 * k[4][c] + V7_a1105 * k[5][c] + V7_a1106 * k[6][c] +
+                                      V7_a1107 * k[7][c] + V7_a1108 * k[8][c] + V7_a1109 * k[9][c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V7_c11, dt, tprev), k11);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V7_a1201 * k[1][c] + V7_a1204 * k[4][c] + V7_a1205 * k[5][c] + V7_a1206 * k[6][c] +
+                                      V7_a1207 * k[7][c] + V7_a1208 * k[8][c] + V7_a1209 * k[9][c] + V7_a1211 * k11[c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V7_c12, dt, tprev), k12);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V7_a1301 * k[1][c] + V7_a1304 * k[4][c] + V7_a1305 * k[5][c] + V7_a1306 * k[6][c] +
+                                      V7_a1307 * k[7][c] + V7_a1308 * k[8][c] + V7_a1309 * k[9][c] + V7_a1311 * k11[c] +
+                                      V7_a1312 * k12[c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V7_c13, dt, tprev), k13);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V7_a1401 * k[1][c] + V7_a1404 * k[4][c] + V7_a1405 * k[5][c] + V7_a1406 * k[6][c] +
+                                      V7_a1407 * k[7][c] + V7_a1408 * k[8][c] + V7_a1409 * k[9][c] + V7_a1411 * k11[c] +
+                                      V7_a1412 * k12[c] + V7_a1413 * k13[c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V7_c14, dt, tprev), k14);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V7_a1501 * k[1][c] + V7_a1504 * k[4][c] + V7_a1505 * k[5][c] + V7_a1506 * k[6][c] +
+                                      V7_a1507 * k[7][c] + V7_a1508 * k[8][c] + V7_a1509 * k[9][c] + V7_a1511 * k11[c] +
+                                      V7_a1512 * k12[c] + V7_a1513 * k13[c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V7_c15, dt, tprev), k15);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V7_a1601 * k[1][c] + V7_a1604 * k[4][c] + V7_a1605 * k[5][c] + V7_a1606 * k[6][c] +
+                                      V7_a1607 * k[7][c] + V7_a1608 * k[8][c] + V7_a1609 * k[9][c] + V7_a1611 * k11[c] +
+                                      V7_a1612 * k12[c] + V7_a1613 * k13[c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V7_c16, dt, tprev), k16);
+    const float q1 = V7Q(01), q4 = V7Q(04), q5 = V7Q(05), q6 = V7Q(06), q7 = V7Q(07), q8 = V7Q(08), q9 = V7Q(09);
+    const float q11 = V7Q(11), q12 = V7Q(12), q13 = V7Q(13), q14 = V7Q(14), q15 = V7Q(15), q16 = V7Q(16);
+    const float thm = th * (th - 1.0f);
+    DEG_FOR(c) {
+        const float step_sum = dt * (V7_b1 * k[1][c] + V7_b4 * k[4][c] + V7_b5 * k[5][c] + V7_b6 * k[6][c] +
+                                     V7_b7 * k[7][c] + V7_b8 * k[8][c] + V7_b9 * k[9][c]);
+        const float corr = dt * (k[1][c] * q1 + k[4][c] * q4 + k[5][c] * q5 + k[6][c] * q6 + k[7][c] * q7 +
+                                 k[8][c] * q8 + k[9][c] * q9 + k11[c] * q11 + k12[c] * q12 + k13[c] * q13 +
+                                 k14[c] * q14 + k15[c] * q15 + k16[c] * q16);
+        out[c] = (y0[c] + th * step_sum) + thm * corr;
+    }
+}
+#endif
+
+// ------------------------------------------------------------------------------------
+// Rosenbrock23 (gpu_rosenbrock23_perform_step.jl:33-60, :109-153) with an FP64 W = I - gamma J
+// ------------------------------------------------------------------------------------
+#if DEG_METHOD == 2
+#define DEG_NK 2
+// LU with partial pivoting; row swaps are applied with compile-time indices so W stays
+// in VGPRs.
+__device__ __forceinline__ void w_factor(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N], float gam,
+                                         const float (&J)[DEG_N * DEG_N]) {
+    const double g = (double)gam;
+    DEG_FOR(r) {
+        _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (r == c ? 1.0 : 0.0) - g * (double)J[r * DEG_N + c];
+    }
+    _Pragma("unroll") for (int kk = 0; kk < DEG_N; ++kk) {
+        int pk = kk;
+        double mx = __builtin_fabs(W[kk][kk]);
+        _Pragma("unroll") for (int i = kk + 1; i < DEG_N; ++i) {
+            double v = __builtin_fabs(W[i][kk]);
+            if (v > mx) {
+                mx = v;
+                pk = i;
+            }
+        }
+        piv[kk] = pk;
+        _Pragma("unroll") for (int i = kk + 1; i < DEG_N; ++i) {
+            if (pk == i) {
+                _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) {
+                    double tv = W[kk][c];
+                    W[kk][c] = W[i][c];
+                    W[i][c] = tv;
+                }
+            }
+        }
+        _Pragma("unroll") for (int i = kk + 1; i < DEG_N; ++i) {
+            double l = W[i][kk] / W[kk][kk];
+            W[i][kk] = l;
+            _Pragma("unroll") for (int c = kk + 1; c < DEG_N; ++c) W[i][c] = W[i][c] - l * W[kk][c];
+        }
+    }
+}
+__device__ __forceinline__ void w_solve(const double (&W)[DEG_N][DEG_N], const int (&piv)[DEG_N],
+                                        const float (&b)[DEG_N], float (&x)[DEG_N]) {
+    double y[DEG_N];
+    DEG_FOR(i) y[i] = (double)b[i];
+    _Pragma("unroll") for (int kk = 0; kk < DEG_N; ++kk) {
+        _Pragma("unroll") for (int i = kk + 1; i < DEG_N; ++i) {
+            if (piv[kk] == i) {
+                double tv = y[kk];
+                y[kk] = y[i];
+                y[i] = tv;
+            }
+        }
+    }
+    DEG_FOR(i) {
+        double v = y[i];
+        _Pragma("unroll") for (int j = 0; j < i; ++j) v = v - W[i][j] * y[j];
+        y[i] = v;
+    }
+    _Pragma("unroll") for (int i = DEG_N - 1; i >= 0; --i) {
+        double v = y[i];
+        _Pragma("unroll") for (int j = i + 1; j < DEG_N; ++j) v = v - W[i][j] * y[j];
+        y[i] = v / W[i][i];
+    }
+    DEG_FOR(i) x[i] = (float)y[i];
+}
+#define RB23_D (1.0f / (2.0f + __builtin_sqrtf(2.0f)))
+#define RB23_E32 (6.0f + __builtin_sqrtf(2.0f))
+// One Rosenbrock23 step from (uprev, t) with step dt; k[1], k[2] receive the slopes used by
+// the dense output. Returns EEst when want_error is set.
+template <bool want_error>
+__device__ __forceinline__ float deg_rb23(const float (&uprev)[DEG_N], const float* p, float t, float dt,
+                                          float (&k)[DEG_NK + 1][DEG_N], float (&unew)[DEG_N], float abstol,
+                                          float reltol) {
+    const float d = RB23_D;
+    const float gam = dt * d;
+    const float dto2 = dt / 2.0f;
+    float Jm[DEG_N * DEG_N], dT[DEG_N], F0[DEG_N], F1[DEG_N], rhs[DEG_N], tmp[DEG_N], x[DEG_N];
+    deg_jac(uprev, p, t, Jm);
+    deg_tgrad(uprev, p, t, dT);
+    double W[DEG_N][DEG_N];
+    int piv[DEG_N];
+    w_factor(W, piv, gam, Jm);
+    deg_rhs(uprev, p, t, F0);
+    DEG_FOR(c) rhs[c] = F0[c] + gam * dT[c];
+    w_solve(W, piv, rhs, k[1]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dto2 * k[1][c];
+    deg_rhs(tmp, p, t + dto2, F1);
+    DEG_FOR(c) rhs[c] = F1[c] - k[1][c];
+    w_solve(W, piv, rhs, x);
+    DEG_FOR(c) k[2][c] = x[c] + k[1][c];
+    DEG_FOR(c) unew[c] = uprev[c] + dt * k[2][c];
+    if (!want_error) return 0.0f;
+    const float dto6 = dt / 6.0f;
+    const float e32 = RB23_E32;
+    float F2[DEG_N], k3[DEG_N], e[DEG_N];
+    deg_rhs(unew, p, t + dt, F2);
+    DEG_FOR(c) rhs[c] = ((F2[c] - e32 * (k[2][c] - F1[c])) - 2.0f * (k[1][c] - F0[c])) + dt * dT[c];
+    w_solve(W, piv, rhs, k3);
+    DEG_FOR(c) e[c] = dto6 * ((k[1][c] - 2.0f * k[2][c]) + k3[c]);
+    return scaled_error(e, uprev, unew, abstol, reltol);
+}
+// y0 + dt * (c1 k1 + c2 k2), c1 = th(1-th)/(1-2d), c2 = th(th-2d)/(1-2d)
+__device__ __forceinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                           const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
+    (void)tprev;
+    (void)p;
+    const float d = RB23_D;
+    const float den = __builtin_fmaf(-2.0f, d, 1.0f);
+    const float c1 = (th * (1.0f - th)) / den;
+    const float c2 = (th * __builtin_fmaf(-2.0f, d, th)) / den;
+    DEG_FOR(c) out[c] = y0[c] + dt * (c1 * k[1][c] + c2 * k[2][c]);
+}
+#endif
+
+#define DEG_HAS_INTERP 1
+
+// ------------------------------------------------------------------------------------
+// Persistent work queue: lanes that need a trajectory take ids from one device counter,
+// one atomic per wave (ballot + popcount + prefix rank).
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ long long deg_queue_next(unsigned int* q) {
+    const unsigned long long mask = __ballot(1);
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(q, (unsigned int)__popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned int rank = (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+    return (long long)(base + rank);
+}
+
+// t <- t + dt, optionally compensated
+__device__ __forceinline__ float deg_advance(float t, float dt, float& comp) {
+#if DEG_KAHAN
+    float y = dt - comp;
+    float tn = t + y;
+    comp = (tn - t) - y;
+    return tn;
+#else
+    (void)comp;
+    return t + dt;
+#endif
+}
+
+#ifndef DEG_BLOCK_MAX
+#define DEG_BLOCK_MAX 256
+#endif
+
+// ------------------------------------------------------------------------------------
+// The ensemble kernel (kernels.jl:1-152), one trajectory per lane, one attempt per
+// loop iteration so that lanes stay in step and a finished lane can be refilled.
+// ------------------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg_kargs a) {
+    const long long S = a.stride;
+#if DEG_PERSISTENT
+    long long gid = deg_queue_next(a.queue);
+#else
+    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+#endif
+    if (gid >= a.n_traj) return;
+    const bool use_saveat = a.saveat != nullptr;
+    const float finf = __builtin_inff();
+
+    long long i = 0;
+    float p[DEG_NP > 0 ? DEG_NP : 1];
+    float u[DEG_N], uprev[DEG_N], unew[DEG_N];
+    float k[DEG_NK + 1][DEG_N];
+    float t = 0.0f, tf = 0.0f, tprev = 0.0f, dt = 0.0f, dtacc = 0.0f, tc = 0.0f, tnext = finf;
+    float qold = DEG_QOLDINIT, qpow = 1.0f;
+    int cur = 0, step_idx = 1, nacc = 0, nrej = 0, status = DEG_RET_SUCCESS;
+    long long attempts = 0;
+    bool have_k = false;
+    bool fsal_pending = false;
+    bool need_init = true;
+
+    for (;;) {
+        if (need_init) {
+            // ---- init (nonstiff/types.jl:305-402, stiff/types.jl:32-177; kernels.jl:35-45, :116-126)
+            need_init = false;
+            i = a.id_offset + gid;
+            DEG_FOR(c) u[c] = a.u0[(long long)c * S + i];
+            _Pragma("unroll") for (int j = 0; j < DEG_NP; ++j) p[j] = a.p[(long long)j * S + i];
+            t = a.t0s ? a.t0s[i] : a.t0;
+            tf = a.tfs ? a.tfs[i] : a.tf;
+            tprev = t;
+            dt = a.dt;
+            dtacc = dt;
+            tc = 0.0f;
+            qold = DEG_QOLDINIT;
+#if DEG_ADAPTIVE
+            qpow = ctl_pow(qold, DEG_BETA2);
+#endif
+            nacc = 0;
+            nrej = 0;
+            attempts = 0;
+            status = DEG_RET_SUCCESS;
+            have_k = false;
+            fsal_pending = false;
+            DEG_FOR(c) uprev[c] = u[c];
+            if (use_saveat) {
+                cur = 0;
+                if (a.nsaveat > 0 && t == a.saveat[0]) {
+                    if (a.nsave > 0) {
+                        DEG_FOR(c) a.us[(long long)c * S + i] = u[c];
+                        a.ts[i] = a.t0;
+                    }
+                    cur = 1;
+                }
+                tnext = (cur < a.nsaveat) ? a.saveat[cur] : finf;
+            } else {
+                a.ts[i] = t;
+                DEG_FOR(c) a.us[(long long)c * S + i] = u[c];
+                step_idx = 1;
+            }
+        }
+
+        bool done = !(t < tf);
+        if (!done) {
+            bool accepted = false;
+            if (attempts >= a.max_attempts) {
+                status = DEG_RET_MAXITERS;
+            }
+#if DEG_ADAPTIVE
+            else if (dt < DEG_DTMIN) {
+                status = DEG_RET_DTMIN;
+            }
+#endif
+            else {
+                ++attempts;
+#if DEG_ADAPTIVE
+                // ---- one attempt of the `while EEst > 1` loop
+#if DEG_METHOD == 0
+                if (!have_k) {
+                    deg_rhs(u, p, t, k[1]);
+                    have_k = true;
+                } else if (fsal_pending) {
+                    DEG_FOR(c) k[1][c] = k[7][c];
+                }
+                fsal_pending = false;
+                deg_attempt(u, p, t, dt, k, unew);
+                float EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
+#elif DEG_METHOD == 1
+                deg_rhs(u, p, t, k[1]);
+                deg_attempt(u, p, t, dt, k, unew);
+                float EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
+#else
+                float EEst = deg_rb23<true>(u, p, t, dt, k, unew, a.abstol, a.reltol);
+#endif
+                float q11 = 0.0f, q;
+                if (EEst == 0.0f) {
+                    q = 1.0f / DEG_QMAX;
+                } else {
+                    q11 = ctl_pow(EEst, DEG_BETA1);
+                    q = q11 / qpow;
+                }
+                if (EEst > 1.0f) {
+                    dt = dt / jl_min(1.0f / DEG_QMIN, q11 / DEG_GAMMA);
+                    ++nrej;
+                } else {
+                    q = jl_max(1.0f / DEG_QMAX, jl_min(1.0f / DEG_QMIN, q / DEG_GAMMA));
+                    qold = jl_max(EEst, DEG_QOLDINIT);
+                    qpow = ctl_pow(qold, DEG_BETA2);
+                    float dtnew = dt / q;
+                    dtnew = jl_min(__builtin_fabsf(dtnew), __builtin_fabsf((tf - t) - dt));
+                    dtacc = dt;
+                    tprev = t;
+                    DEG_FOR(c) {
+                        uprev[c] = u[c];
+                        u[c] = unew[c];
+                    }
+                    if ((tf - t) - dt < DEG_DTMIN) {
+                        t = tf;
+                        tc = 0.0f;
+                    } else {
+                        t = deg_advance(t, dt, tc);
+                    }
+                    dt = dtnew;
+                    ++nacc;
+                    accepted = true;
+                }
+#else
+                // ---- one fixed step (gpu_*_perform_step.jl fixed versions)
+                const float told = t;
+                DEG_FOR(c) uprev[c] = u[c];
+                tprev = told;
+                dtacc = dt;
+                t = deg_advance(t, dt, tc);
+#if DEG_METHOD == 0
+                if (!have_k) {
+                    deg_rhs(uprev, p, told, k[1]);
+                    have_k = true;
+                } else {
+                    DEG_FOR(c) k[1][c] = k[7][c];
+                }
+                deg_attempt(uprev, p, told, dt, k, unew);
+#elif DEG_METHOD == 1
+                deg_rhs(uprev, p, told, k[1]);
+                deg_attempt(uprev, p, told, dt, k, unew);
+#else
+                deg_rb23<false>(uprev, p, told, dt, k, unew, 0.0f, 0.0f);
+#endif
+                DEG_FOR(c) u[c] = unew[c];
+                ++nacc;
+                accepted = true;
+#endif
+            }
+            if (accepted) {
+                // ---- savevalues! (integrator_utils.jl:13-50)
+                if (use_saveat) {
+#if DEG_HAS_INTERP
+                    while (tnext <= t) {
+                        float out[DEG_N];
+                        const float th = (tnext - tprev) / dtacc;
+                        deg_interp(th, dtacc, tprev, p, uprev, k, out);
+                        if (cur < a.nsave) {
+                            DEG_FOR(c) a.us[((long long)cur * DEG_N + c) * S + i] = out[c];
+                            a.ts[(long long)cur * S + i] = tnext;
+                        }
+                        ++cur;
+                        tnext = (cur < a.nsaveat) ? a.saveat[cur] : finf;
+                    }
+#endif
+                } else if (a.save_everystep) {
+#if !DEG_ADAPTIVE
+                    if (step_idx < a.nsave) {
+                        DEG_FOR(c) a.us[((long long)step_idx * DEG_N + c) * S + i] = u[c];
+                        a.ts[(long long)step_idx * S + i] = t;
+                    }
+                    ++step_idx;
+#endif
+                }
+                fsal_pending = true;
+            }
+            done = (status != DEG_RET_SUCCESS) || !(t < tf);
+        }
+
+        if (done) {
+            // ---- endpoint rules (kernels.jl:53-62 / :133-142) and unsaved rows
+            if (use_saveat) {
+                for (int r = cur; r < a.nsave; ++r) a.ts[(long long)r * S + i] = a.t0;
+            } else if (a.save_everystep) {
+                for (int r = step_idx; r < a.nsave; ++r) a.ts[(long long)r * S + i] = a.t0;
+#if !DEG_ADAPTIVE && DEG_HAS_INTERP
+                if (t > tf && status == DEG_RET_SUCCESS && a.nsave > 1) {
+                    // k[1] holds k7 after the FSAL shift for Tsit5; undo it for the dense output
+                    float out[DEG_N];
+                    const float th = (tf - tprev) / dtacc;
+                    deg_interp(th, dtacc, tprev, p, uprev, k, out);
+                    const long long r = a.nsave - 1;
+                    DEG_FOR(c) a.us[(r * DEG_N + c) * S + i] = out[c];
+                    a.ts[r * S + i] = tf;
+                }
+#endif
+            } else if (a.nsave > 1) {
+                a.ts[S + i] = t;
+                DEG_FOR(c) a.us[((long long)DEG_N + c) * S + i] = u[c];
+            }
+            if (status == DEG_RET_SUCCESS) {
+                bool fin = __builtin_isfinite(t);
+                DEG_FOR(c) fin = fin && __builtin_isfinite(u[c]);
+                if (!fin) status = DEG_RET_NONFINITE;
+            }
+            if (a.retcode) a.retcode[i] = status;
+            if (a.naccept) a.naccept[i] = nacc;
+            if (a.nreject) a.nreject[i] = nrej;
+#if DEG_PERSISTENT
+            gid = deg_queue_next(a.queue);
+            if (gid >= a.n_traj) break;
+            need_init = true;
+#else
+            break;
+#endif
+        }
+    }
+}
+
+// Clears the persistent work counter before a launch (a kernel, not a memset, so the
+// sequence can be captured in a graph).
+extern "C" __global__ void DEG_QUEUE_RESET(unsigned int* q) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) q[0] = 0u;
+}

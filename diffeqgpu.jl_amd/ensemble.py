@@ -1,0 +1,139 @@
+"""High-level ``solve(EnsembleProblem, alg, EnsembleGPUKernel(dev); kwargs...)``.
+
+Mirrors the kernel branch of DiffEqGPU's ensemble front end: src/solve.jl:1-95
+(``__solve``: trajectories, batch_size, adaptive, seeding), :174-287 (``batch_solve``:
+prob_func expansion, per-trajectory tspan/saveat rules, build_solution with trimming and
+return codes) and :382-419 (``batch_solve_up_kernel``: adapt, dispatch to
+vectorized_solve / vectorized_asolve, copy back).
+"""
+from __future__ import annotations
+
+import copy
+import time
+from dataclasses import dataclass, field
+from typing import Any, List, Optional
+
+import numpy as np
+
+from . import _lib
+from .algorithms import EnsembleGPUKernel, GPUODEAlgorithm
+from .lowerlevel import vectorized_asolve, vectorized_solve
+from .problems import EnsembleProblem, adapt, make_prob_compatible
+
+
+@dataclass
+class EnsembleContext:
+    """What prob_func / output_func receive as ``ctx``: 1-based ``sim_id`` and an ``rng``."""
+
+    sim_id: int
+    rng: Optional[np.random.Generator] = None
+    seed: Optional[int] = None
+
+
+@dataclass
+class ODESolution:
+    t: np.ndarray
+    u: np.ndarray          # (len(t), n)
+    retcode: str
+    prob: Any = None
+    alg: Any = None
+    stats: Any = None
+
+    def __getitem__(self, i):
+        return self.u[i]
+
+
+@dataclass
+class EnsembleSolution:
+    u: List[Any]
+    elapsedTime: float
+    converged: bool = True
+    stats: dict = field(default_factory=dict)
+
+    def __len__(self):
+        return len(self.u)
+
+    def __getitem__(self, i):
+        return self.u[i]
+
+
+def _ctx(i: int, seed: Optional[int]) -> EnsembleContext:
+    if seed is None:
+        return EnsembleContext(sim_id=i)
+    s = (int(seed) * 1000003 + i) % (2 ** 63)
+    return EnsembleContext(sim_id=i, rng=np.random.default_rng(s), seed=s)
+
+
+def solve(ensembleprob: EnsembleProblem, alg: GPUODEAlgorithm, ensemblealg: EnsembleGPUKernel, *,
+          trajectories: int, batch_size: Optional[int] = None, adaptive: bool = True, seed=None,
+          unstable_check=None, **kwargs) -> EnsembleSolution:
+    """solve.jl:1-95. ``adaptive`` selects vectorized_asolve (default) or vectorized_solve."""
+    if not isinstance(ensemblealg, EnsembleGPUKernel):
+        raise TypeError("only EnsembleGPUKernel is provided")
+    if not isinstance(alg, GPUODEAlgorithm):
+        raise TypeError("alg must be GPUTsit5(), GPUVern7() or GPURosenbrock23()")
+    if trajectories < 1:
+        raise ValueError("trajectories must be >= 1")
+    batch_size = trajectories if batch_size is None else int(batch_size)
+    t_start = time.perf_counter()
+    sols: List[Any] = []
+    for lo in range(0, trajectories, batch_size):
+        I = range(lo + 1, min(lo + batch_size, trajectories) + 1)  # 1-based sim ids like the reference
+        sols.extend(batch_solve(ensembleprob, alg, ensemblealg, I, adaptive, seed=seed, **kwargs))
+    return EnsembleSolution(sols, time.perf_counter() - t_start, True)
+
+
+def batch_solve(ensembleprob, alg, ensemblealg, I, adaptive, seed=None, **kwargs):
+    """solve.jl:174-287 (kernel branch)."""
+    if len(I) == 0:
+        raise ValueError("empty batch")
+    probs = []
+    for i in I:
+        base = copy.deepcopy(ensembleprob.prob) if ensembleprob.safetycopy else ensembleprob.prob
+        probs.append(make_prob_compatible(ensembleprob.prob_func(base, _ctx(i, seed))))
+    # per-trajectory time spans (solve.jl:206-245)
+    same_tspan = all(p.tspan == probs[0].tspan for p in probs)
+    inner_saveat = probs[0].kwargs.get("saveat")
+    if not same_tspan:
+        if inner_saveat is None and not adaptive and kwargs.get("save_everystep", True):
+            raise ValueError("Using different time-spans require either turning off save_everystep or using "
+                             "saveat. If using saveat, it should be of same length across the ensemble.")
+        lens = {np.size(p.kwargs.get("saveat")) if p.kwargs.get("saveat") is not None else None for p in probs}
+        if len(lens) > 1:
+            raise ValueError("Using different saveat in EnsembleGPUKernel requires all of them to be of same "
+                             "length. Use saveats of same size only.")
+    saveat = inner_saveat if inner_saveat is not None else kwargs.pop("saveat", None)
+    kwargs.pop("saveat", None)
+    solts, solus, rc = batch_solve_up_kernel(ensembleprob, probs, alg, ensemblealg, I, adaptive,
+                                             saveat=saveat, **kwargs)
+    out = []
+    for k, i in enumerate(I):
+        ts = solts[:, k]
+        us = solus[:, k, :]
+        nz = np.nonzero(ts != probs[k].tspan[0])[0]
+        if len(nz) == 0:
+            raise RuntimeError(f"Batch solve failed: no solution found for trajectory {i} (tspan[1] = "
+                               f"{probs[k].tspan[0]})")
+        sol_idx = int(nz[-1])
+        retcode = "Terminated" if sol_idx != len(ts) - 1 else "Success"
+        if rc[k] != _lib.RET_SUCCESS:
+            retcode = _lib.RETCODE_NAMES.get(int(rc[k]), "Failure")
+        sol = ODESolution(ts[: sol_idx + 1].copy(), us[: sol_idx + 1].copy(), retcode, probs[k], alg)
+        out.append(ensembleprob.output_func(sol, _ctx(i, seed))[0])
+    return out
+
+
+def batch_solve_up_kernel(ensembleprob, probs, alg, ensemblealg, I, adaptive, **kwargs):
+    """solve.jl:382-419: adapt to the device, run the lower-level solve, copy back."""
+    if kwargs.get("callback") is not None:
+        raise NotImplementedError("callbacks are not provided by the HIP path")
+    dev = ensemblealg.dev
+    gprobs = adapt(dev, probs)
+    if adaptive:
+        res = vectorized_asolve(gprobs, ensembleprob.prob, alg, **kwargs)
+    else:
+        if "dt" not in kwargs:
+            raise ValueError("a fixed-step solve needs dt")
+        res = vectorized_solve(gprobs, ensembleprob.prob, alg, **kwargs)
+    ts, us = res
+    return ts.cpu().numpy(), us.cpu().numpy(), res.retcode.cpu().numpy()

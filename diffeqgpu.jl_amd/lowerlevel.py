@@ -1,0 +1,223 @@
+"""Lower-level API: ``vectorized_solve`` / ``vectorized_asolve``.
+
+Host half of src/ensemblegpukernel/lowerlevel_solve.jl:53-131 (fixed step) and :253-346
+(adaptive): saveat normalisation, output sizing and the t0 prefill of ``ts``. The device
+half is one call of ``deg_solve`` in libdeghip.so.
+
+Layout. The device writes structure-of-arrays: ``ts`` is a (nsave, N) tensor and the state
+output is stored as (nsave, n, N). ``us`` is returned as the (nsave, N, n) view of that
+storage, so ``us[s, i]`` is the n-vector of trajectory i at save point s, the same indexing as
+the reference's ``us[s, i]::SVector`` of its (nsave, N) matrix.
+"""
+from __future__ import annotations
+
+import math
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .algorithms import GPUODEAlgorithm, GPURosenbrock23, HIPDevice
+from .problems import HIPProbVector, ODEProblem, get_backend
+from .ranges import JlRange, colon, linrange
+
+MAX_SAVEAT_LENGTH = 100_000  # lowerlevel_solve.jl:284-290
+
+
+class SolveResult(tuple):
+    """``(ts, us)`` with per-trajectory statistics attached.
+
+    Attributes: ``retcode`` (int32, 0 Success / 1 DtLessThanMin / 2 MaxIters / 3 Unstable),
+    ``naccept``, ``nreject`` (int32 per trajectory), ``us_soa`` ((nsave, n, N) storage),
+    ``kernel_ms`` (float or None), ``saveat`` (normalised device tensor or None)."""
+
+    def __new__(cls, ts, us, **attrs):
+        obj = super().__new__(cls, (ts, us))
+        for k, v in attrs.items():
+            setattr(obj, k, v)
+        return obj
+
+    @property
+    def ts(self):
+        return self[0]
+
+    @property
+    def us(self):
+        return self[1]
+
+
+def _is_range(x) -> bool:
+    return isinstance(x, (range, JlRange))
+
+
+def _to_T(x):
+    return np.asarray(x, dtype=np.float64).astype(np.float32)
+
+
+def normalize_saveat_solve(saveat, tspan) -> Optional[np.ndarray]:
+    """lowerlevel_solve.jl:89-102 (fixed-step entry point)."""
+    if saveat is None:
+        return None
+    t0, tf = np.float32(tspan[0]), np.float32(tspan[1])
+    if _is_range(saveat):
+        vals = np.asarray(list(saveat) if isinstance(saveat, range) else saveat.collect(), dtype=np.float64)
+        if len(vals) == 0:
+            return np.zeros(0, np.float32)
+        return linrange(np.float32(vals[0]), np.float32(vals[-1]), len(vals)).collect()
+    if np.ndim(saveat) >= 1:
+        return _to_T(saveat).reshape(-1)
+    s = np.float32(saveat)
+    if s == np.float32(0.0):
+        return np.array([t0, tf], np.float32)
+    num = int(math.ceil(float(np.float32(abs(np.float32(tf - t0)) / abs(s))))) + 1
+    return linrange(t0, tf, num).collect()
+
+
+def normalize_saveat_asolve(saveat, tspan) -> Optional[np.ndarray]:
+    """lowerlevel_solve.jl:271-306 (adaptive entry point)."""
+    if saveat is None:
+        return None
+    t0, tf = np.float32(tspan[0]), np.float32(tspan[1])
+    if np.ndim(saveat) == 0 and not _is_range(saveat):
+        s = np.float32(saveat)
+        if s == np.float32(0.0):
+            return np.array([t0, tf], np.float32)
+        num = int(math.ceil(float(np.float32(abs(np.float32(tf - t0)) / abs(s))))) + 1
+        if num > MAX_SAVEAT_LENGTH:
+            raise ValueError(f"saveat would create too many save points ({num}). "
+                             "Consider using a larger saveat value.")
+        return linrange(t0, tf, num).collect()
+    if isinstance(saveat, range):
+        return _to_T(list(saveat))
+    if isinstance(saveat, JlRange):
+        return saveat.collect().astype(np.float32)
+    return _to_T(saveat).reshape(-1)
+
+
+def _rep_tspan(prob) -> tuple:
+    if isinstance(prob, ODEProblem) or hasattr(prob, "tspan"):
+        return (np.float32(prob.tspan[0]), np.float32(prob.tspan[1]))
+    raise TypeError("prob must be an ODEProblem")
+
+
+def _check_unsupported(callback, tstops):
+    if callback is not None:
+        raise NotImplementedError("callbacks are not provided by the HIP path")
+    if tstops is not None:
+        raise NotImplementedError("tstops are not provided by the HIP path")
+
+
+_AUTOTUNE_BLOCK = {}
+
+
+def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, abstol, reltol, saveat_np,
+            nsave, save_everystep, t0_fill, timing=False, stream=None, mlp=None, steplog=None):
+    import torch
+
+    if not isinstance(alg, GPUODEAlgorithm):
+        raise TypeError(f"unsupported algorithm {alg!r}")
+    dev: HIPDevice = probs.dev
+    flags = dev.flags(adaptive)
+    if mlp is not None:
+        flags |= _lib.F_LEARNED
+    if steplog is not None:
+        flags |= _lib.F_STEPLOG
+    with_jac = isinstance(alg, GPURosenbrock23)
+    src, _flops = probs.f.hip_source(probs.n, probs.np, with_jac)
+    ctx = _lib.context(dev.device)
+    kern = ctx.kernel(src, probs.n, probs.np, alg.method, flags)
+    device = probs.u0.device
+    N, n = probs.N, probs.n
+    ts = torch.empty((nsave, N), dtype=torch.float32, device=device)
+    us_soa = torch.empty((nsave, n, N), dtype=torch.float32, device=device)
+    retcode = torch.empty(N, dtype=torch.int32, device=device)
+    nacc = torch.empty(N, dtype=torch.int32, device=device)
+    nrej = torch.empty(N, dtype=torch.int32, device=device)
+    sv = None
+    if saveat_np is not None:
+        sv = torch.as_tensor(np.ascontiguousarray(saveat_np, dtype=np.float32)).to(device)
+    a = _lib.SolveArgs()
+    a.n = n
+    a.n_traj = N
+    a.stride = N
+    a.id_offset = 0
+    a.u0 = probs.u0.data_ptr()
+    a.p = probs.p.data_ptr() if probs.np > 0 else None
+    a.t0s = probs.t0s.data_ptr() if probs.t0s is not None else None
+    a.tfs = probs.tfs.data_ptr() if probs.tfs is not None else None
+    a.t0 = float(t0_fill)
+    a.tf = float(probs.tspan[1])
+    a.dt = float(np.float32(dt))
+    a.abstol = float(np.float32(abstol))
+    a.reltol = float(np.float32(reltol))
+    a.saveat = sv.data_ptr() if sv is not None else None
+    a.nsaveat = int(sv.numel()) if sv is not None else 0
+    a.nsave = int(nsave)
+    a.save_everystep = int(bool(save_everystep))
+    a.block_size = int(dev.block_size or _AUTOTUNE_BLOCK.get((alg.method, flags, n), 0))
+    a.ts = ts.data_ptr()
+    a.us = us_soa.data_ptr()
+    a.retcode = retcode.data_ptr()
+    a.naccept = nacc.data_ptr()
+    a.nreject = nrej.data_ptr()
+    a.max_attempts = int(dev.max_attempts)
+    st = stream if stream is not None else torch.cuda.current_stream(device)
+    a.stream = st.cuda_stream
+    ms = None
+    if timing:
+        import ctypes
+
+        box = ctypes.c_float(0.0)
+        a.elapsed_ms = ctypes.addressof(box)
+    if mlp is not None:
+        a.mlp = mlp.data_ptr()
+    if steplog is not None:
+        buf, counter = steplog
+        a.steplog = buf.data_ptr()
+        a.steplog_count = counter.data_ptr()
+        a.steplog_cap = int(buf.shape[0])
+    # keep the host-side tensors alive across the asynchronous launch
+    kern.solve(a)
+    if timing:
+        ms = float(box.value)
+    return SolveResult(ts, us_soa.permute(0, 2, 1), retcode=retcode, naccept=nacc, nreject=nrej, us_soa=us_soa,
+                       kernel_ms=ms, saveat=sv, kernel=kern, flags=flags)
+
+
+def vectorized_solve(probs, prob, alg, *, dt, saveat=None, save_everystep=True, debug=False, callback=None,
+                     tstops=None, timing=False, **kwargs):
+    """Fixed-step ensemble solve (lowerlevel_solve.jl:53-131). Returns ``(ts, us)``."""
+    _check_unsupported(callback, tstops)
+    get_backend(probs)
+    t0, tf = _rep_tspan(prob)
+    dt32 = np.float32(dt)
+    if not dt32 > 0:
+        raise ValueError("dt must be positive for a fixed-step solve")
+    sv = normalize_saveat_solve(saveat, (t0, tf))
+    if sv is None:
+        nsave = len(colon(t0, dt32, tf)) if save_everystep else 2
+    else:
+        nsave = len(sv)
+    return _launch(probs, alg, False, dt=dt32, abstol=0.0, reltol=0.0, saveat_np=sv, nsave=nsave,
+                   save_everystep=save_everystep and sv is None, t0_fill=t0, timing=timing, **kwargs)
+
+
+def vectorized_asolve(probs, prob, alg, *, dt=0.1, saveat=None, save_everystep=False, abstol=1.0e-6,
+                      reltol=1.0e-3, debug=False, callback=None, tstops=None, timing=False, **kwargs):
+    """Adaptive ensemble solve (lowerlevel_solve.jl:253-346). Returns ``(ts, us)``."""
+    _check_unsupported(callback, tstops)
+    get_backend(probs)
+    t0, tf = _rep_tspan(prob)
+    sv = normalize_saveat_asolve(saveat, (t0, tf))
+    dt32 = np.float32(dt)
+    if sv is None:
+        if save_everystep:
+            nsave = int(math.ceil(float(np.float32(np.float32(tf - t0) / dt32)))) + 1
+        else:
+            nsave = 2
+    else:
+        nsave = len(sv)
+    # The adaptive integrator never saves every step (nonstiff/types.jl:340-402 passes
+    # save_everystep = false); the kernel only skips the endpoint write in that case.
+    return _launch(probs, alg, True, dt=dt32, abstol=abstol, reltol=reltol, saveat_np=sv, nsave=nsave,
+                   save_everystep=save_everystep and sv is None, t0_fill=t0, timing=timing, **kwargs)

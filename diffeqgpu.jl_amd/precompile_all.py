@@ -1,0 +1,52 @@
+"""Warm the code-object cache (diffeqgpu.jl_amd/_jitcache) with the solver variants the test
+suite and the benchmark launch, so a GPU run does not have to JIT them. hiprtc compiles
+gfx950 code objects without a GPU."""
+from __future__ import annotations
+
+import multiprocessing as mp
+
+from . import _lib
+from .algorithms import HIPDevice
+from .systems import SYSTEMS
+
+_METHODS = (_lib.DEG_TSIT5, _lib.DEG_VERN7, _lib.DEG_RB23)
+
+
+def variants():
+    devs = [
+        (HIPDevice(contract="off"), False),                                   # fixed step, parity
+        (HIPDevice(contract="off", powmode="det", persistent=False), True),   # adaptive, bitwise vs oracle
+        (HIPDevice(contract="off", powmode="det", persistent=True), True),
+        (HIPDevice(contract="off", persistent=True), True),                   # adaptive, exact ops, device powf
+        (HIPDevice(), True),                                                  # adaptive defaults (persistent)
+        (HIPDevice(persistent=False), True),
+        (HIPDevice(powmode="fast"), True),
+        (HIPDevice(contract="off", powmode="det", persistent=True, kahan=True), True),
+        (HIPDevice(kahan=True), True),
+    ]
+    out = []
+    for name, (f, n, np_, _sid) in SYSTEMS.items():
+        for m in _METHODS:
+            src, _ = f.hip_source(n, np_, m == _lib.DEG_RB23)
+            for dev, adaptive in devs:
+                out.append((name, src, n, np_, m, dev.flags(adaptive)))
+    return out
+
+
+def _one(j):
+    name, src, n, np_, m, flags = j
+    return _lib.precompile(src, n, np_, m, flags)
+
+
+def run(workers: int = 8, verbose: bool = False):
+    """hiprtc serialises compiles inside one process, so the warm-up forks workers."""
+    jobs = variants()
+    with mp.get_context("fork").Pool(workers) as pool:
+        keys = pool.map(_one, jobs, chunksize=4)
+    if verbose:
+        print(f"{len(keys)} solver variants in the code-object cache")
+    return keys
+
+
+if __name__ == "__main__":
+    run(verbose=True)

@@ -1,0 +1,89 @@
+"""Right-hand sides of the benchmark and test problems, written for the tracer.
+
+Each is the out-of-place f(u, p, t) of a reference test or benchmark problem with the
+same association order (u and p are 0-based here, 1-based in the Julia sources):
+  lorenz     test/gpu_kernel_de/gpu_ode_regression.jl:4-12 (jac: test/lower_level_api.jl:18-35)
+  decay      test/gpu_kernel_de/stiff_ode/gpu_ode_regression.jl:5-8
+  growth     test/public_interface.jl:9-11
+  quad       test/gpu_kernel_de/finite_diff.jl:6-9
+  robertson  Robertson kinetics (test/gpu_kernel_de/stiff_ode/gpu_ode_mass_matrix.jl, ODE form)
+  vdp        Van der Pol
+  tdep       u' = p t u
+  rot        u1' = -p u2, u2' = p u1
+"""
+from .problems import ODEFunction
+from .tracer import SVector
+
+
+def lorenz(u, p, t):
+    s, r, b = p[0], p[1], p[2]
+    du1 = s * (u[1] - u[0])
+    du2 = u[0] * (r - u[2]) - u[1]
+    du3 = u[0] * u[1] - b * u[2]
+    return SVector(du1, du2, du3)
+
+
+def lorenz_jac(u, p, t):
+    s, r, b = p[0], p[1], p[2]
+    return [[-s, s, 0.0], [r - u[2], -1.0, -u[0]], [u[1], u[0], -b]]
+
+
+def zero_tgrad3(u, p, t):
+    return SVector(0.0, 0.0, 0.0)
+
+
+def decay(u, p, t):
+    return SVector(-p[0] * u[0])
+
+
+def growth(u, p, t):
+    return SVector(u[0])
+
+
+def quad(u, p, t):
+    return SVector(-p[0] * u[0] * u[0])
+
+
+def robertson(u, p, t):
+    k1, k2, k3 = p[0], p[1], p[2]
+    return SVector(-k1 * u[0] + k3 * u[1] * u[2],
+                   k1 * u[0] - k2 * u[1] * u[1] - k3 * u[1] * u[2],
+                   k2 * u[1] * u[1])
+
+
+def robertson_jac(u, p, t):
+    k1, k2, k3 = p[0], p[1], p[2]
+    return [[-k1, k3 * u[2], k3 * u[1]],
+            [k1, -2.0 * k2 * u[1] - k3 * u[2], -k3 * u[1]],
+            [0.0, 2.0 * k2 * u[1], 0.0]]
+
+
+def vdp(u, p, t):
+    mu = p[0]
+    return SVector(u[1], mu * (1.0 - u[0] * u[0]) * u[1] - u[0])
+
+
+def vdp_jac(u, p, t):
+    mu = p[0]
+    return [[0.0, 1.0], [-2.0 * mu * u[0] * u[1] - 1.0, mu * (1.0 - u[0] * u[0])]]
+
+
+def tdep(u, p, t):
+    return SVector(p[0] * t * u[0])
+
+
+def rot(u, p, t):
+    return SVector(-p[0] * u[1], p[0] * u[0])
+
+
+# (ODEFunction, n, np, oracle system id)
+SYSTEMS = {
+    "lorenz": (ODEFunction(lorenz, jac=lorenz_jac, tgrad=zero_tgrad3), 3, 3, 0),
+    "decay": (ODEFunction(decay), 1, 1, 1),
+    "growth": (ODEFunction(growth), 1, 1, 2),
+    "quad": (ODEFunction(quad), 1, 1, 3),
+    "robertson": (ODEFunction(robertson, jac=robertson_jac, tgrad=zero_tgrad3), 3, 3, 4),
+    "vdp": (ODEFunction(vdp, jac=vdp_jac), 2, 1, 5),
+    "tdep": (ODEFunction(tdep), 1, 1, 6),
+    "rot": (ODEFunction(rot), 2, 1, 7),
+}

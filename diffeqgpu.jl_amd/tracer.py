@@ -1,0 +1,491 @@
+"""Trace an out-of-place right-hand side ``f(u, p, t)`` into HIP C.
+
+The reference compiles the user's ``f(u::SVector, p, t)::SVector`` straight into its
+GPU kernel (src/ensemblegpukernel/kernels.jl:1-152, the RHS is ``prob.f``). Here the
+Python function is called once with symbolic arguments; every arithmetic operation it
+performs is recorded in evaluation order and emitted as Float32 HIP C, so the kernel
+performs the same IEEE operations with the same association as the source expression.
+
+Python float constants are Float32 constants (``(float)<double literal>``, i.e.
+``convert(Float32, lit)``); integer constants are exact. Elementary functions come from
+:mod:`diffeqgpu_amd.math` (``sin``, ``cos``, ``exp``, ``log``, ``sqrt``, ``tanh``, ``abs``,
+``pow``), which accept both numbers and traced values.
+
+For Rosenbrock23 the Jacobian ``d f / d u`` and ``d f / d t`` are produced by symbolic
+forward-mode differentiation of the recorded expression graph (the role ForwardDiff plays
+in nlsolve/type.jl:129-157) unless the problem supplies ``jac``/``tgrad`` functions,
+which are traced like ``f``.
+"""
+from __future__ import annotations
+
+import math as _pymath
+import struct
+from typing import Callable, Dict, List, Sequence, Tuple
+
+
+class TraceError(TypeError):
+    pass
+
+
+def _f32(x: float) -> float:
+    return struct.unpack("f", struct.pack("f", float(x)))[0]
+
+
+class Node:
+    """A traced scalar. Nodes are hash-consed per :class:`Graph`."""
+
+    __slots__ = ("g", "op", "args", "val", "idx", "_key")
+
+    def __init__(self, g: "Graph", op: str, args: Tuple["Node", ...], val=None):
+        self.g = g
+        self.op = op
+        self.args = args
+        self.val = val
+        self.idx = -1
+        self._key = None
+
+    # -- arithmetic (left operand first, exactly as written) --
+    def _bin(self, op, other, rev=False):
+        o = self.g.lift(other)
+        return self.g.mk(op, (o, self) if rev else (self, o))
+
+    def __add__(self, o): return self._bin("+", o)
+    def __radd__(self, o): return self._bin("+", o, True)
+    def __sub__(self, o): return self._bin("-", o)
+    def __rsub__(self, o): return self._bin("-", o, True)
+    def __mul__(self, o): return self._bin("*", o)
+    def __rmul__(self, o): return self._bin("*", o, True)
+    def __truediv__(self, o): return self._bin("/", o)
+    def __rtruediv__(self, o): return self._bin("/", o, True)
+    def __neg__(self): return self.g.mk("neg", (self,))
+    def __pos__(self): return self
+
+    def __pow__(self, e):
+        # Julia lowers x^2, x^3 with literal integer exponents to repeated multiplication
+        # (Base.literal_pow); other exponents call pow.
+        if isinstance(e, int) and not isinstance(e, bool):
+            if e == 0:
+                return self.g.const(1.0)
+            if e == 1:
+                return self
+            if e == 2:
+                return self * self
+            if e == 3:
+                return (self * self) * self
+            if e == -1:
+                return self.g.const(1.0) / self
+        return self.g.mk("pow", (self, self.g.lift(e)))
+
+    def __rpow__(self, b):
+        return self.g.mk("pow", (self.g.lift(b), self))
+
+    def __bool__(self):
+        raise TraceError("a traced value has no truth value; use diffeqgpu_amd.math.ifelse for branches")
+
+    def _no_compare(self, other):
+        raise TraceError("comparisons of traced values are not recorded; use diffeqgpu_amd.math.ifelse")
+
+    __lt__ = __le__ = __gt__ = __ge__ = _no_compare
+
+    def __float__(self):
+        raise TraceError("cannot convert a traced value to float inside a traced RHS")
+
+    def __repr__(self):
+        return f"Node({self.op}, #{self.idx})"
+
+
+_UNARY = {"neg", "sin", "cos", "exp", "log", "sqrt", "tanh", "abs"}
+
+
+class Graph:
+    def __init__(self):
+        self.nodes: List[Node] = []
+        self.table: Dict[tuple, Node] = {}
+
+    def _key(self, op, args, val):
+        return (op, tuple(a.idx for a in args), val)
+
+    def mk(self, op, args, val=None):
+        for a in args:
+            if not isinstance(a, Node) or a.g is not self:
+                raise TraceError("mixing values from different traces")
+        key = self._key(op, args, val)
+        n = self.table.get(key)
+        if n is None:
+            n = Node(self, op, tuple(args), val)
+            n.idx = len(self.nodes)
+            self.nodes.append(n)
+            self.table[key] = n
+        return n
+
+    def const(self, x: float) -> Node:
+        return self.mk("const", (), float(x))
+
+    def lift(self, x) -> Node:
+        if isinstance(x, Node):
+            return x
+        if isinstance(x, bool):
+            raise TraceError("booleans are not arithmetic values")
+        if isinstance(x, (int, float)):
+            return self.const(x)
+        try:
+            import numpy as np
+
+            if isinstance(x, np.generic):
+                return self.const(float(x))
+        except ImportError:  # pragma: no cover
+            pass
+        raise TraceError(f"unsupported value in traced RHS: {type(x).__name__}")
+
+
+class SymVec:
+    """Read-only indexable vector of traced inputs (u or p), 0-based like a Python list."""
+
+    def __init__(self, items: Sequence[Node], name: str):
+        self._items = list(items)
+        self._name = name
+
+    def __len__(self):
+        return len(self._items)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return list(self._items[i])
+        return self._items[i]
+
+    def __iter__(self):
+        return iter(self._items)
+
+
+class SVector(tuple):
+    """Out-of-place RHS return type, the counterpart of StaticArrays' ``SVector``."""
+
+    def __new__(cls, *xs):
+        if len(xs) == 1 and isinstance(xs[0], (list, tuple)):
+            xs = tuple(xs[0])
+        return super().__new__(cls, xs)
+
+
+# ------------------------------------------------------------------------------------------
+# elementary functions usable on numbers and traced values
+# ------------------------------------------------------------------------------------------
+def _unary(name, pyfn):
+    def fn(x):
+        if isinstance(x, Node):
+            return x.g.mk(name, (x,))
+        return pyfn(x)
+
+    fn.__name__ = name
+    return fn
+
+
+sin = _unary("sin", _pymath.sin)
+cos = _unary("cos", _pymath.cos)
+exp = _unary("exp", _pymath.exp)
+log = _unary("log", _pymath.log)
+sqrt = _unary("sqrt", _pymath.sqrt)
+tanh = _unary("tanh", _pymath.tanh)
+fabs = _unary("abs", abs)
+
+
+def pow(x, y):  # noqa: A001 - mirrors the math function name
+    if isinstance(x, Node) or isinstance(y, Node):
+        g = x.g if isinstance(x, Node) else y.g
+        return g.mk("pow", (g.lift(x), g.lift(y)))
+    return _pymath.pow(x, y)
+
+
+def ifelse(cond_lhs, cmp: str, cond_rhs, a, b):
+    """``cond_lhs <cmp> cond_rhs ? a : b`` with cmp in <, <=, >, >=, ==, != (both branches evaluated)."""
+    if cmp not in ("<", "<=", ">", ">=", "==", "!="):
+        raise TraceError(f"bad comparison {cmp!r}")
+    vals = [cond_lhs, cond_rhs, a, b]
+    g = next((v.g for v in vals if isinstance(v, Node)), None)
+    if g is None:
+        ok = {"<": cond_lhs < cond_rhs, "<=": cond_lhs <= cond_rhs, ">": cond_lhs > cond_rhs,
+              ">=": cond_lhs >= cond_rhs, "==": cond_lhs == cond_rhs, "!=": cond_lhs != cond_rhs}[cmp]
+        return a if ok else b
+    return g.mk("select", tuple(g.lift(v) for v in vals), cmp)
+
+
+# ------------------------------------------------------------------------------------------
+# tracing
+# ------------------------------------------------------------------------------------------
+class Trace:
+    """Result of tracing one function: inputs, the graph and the output nodes."""
+
+    def __init__(self, g: Graph, u: List[Node], p: List[Node], t: Node, outs: List[Node]):
+        self.g = g
+        self.u = u
+        self.p = p
+        self.t = t
+        self.outs = outs
+
+
+def _new_inputs(n: int, np_: int):
+    g = Graph()
+    u = [g.mk("u", (), i) for i in range(n)]
+    p = [g.mk("p", (), j) for j in range(np_)]
+    t = g.mk("t", (), None)
+    return g, u, p, t
+
+
+def trace(f: Callable, n: int, np_: int, nout: int | None = None, g: Graph | None = None,
+          inputs=None) -> Trace:
+    if g is None:
+        g, u, p, t = _new_inputs(n, np_)
+    else:
+        u, p, t = inputs
+    out = f(SymVec(u, "u"), SymVec(p, "p"), t)
+    if isinstance(out, (int, float, Node)):
+        out = (out,)
+    try:
+        outs = [g.lift(x) for x in out]
+    except TypeError as e:
+        raise TraceError(f"RHS must return a sequence of scalars: {e}") from e
+    if nout is not None and len(outs) != nout:
+        raise TraceError(f"expected {nout} outputs, got {len(outs)}")
+    return Trace(g, u, p, t, outs)
+
+
+# ------------------------------------------------------------------------------------------
+# symbolic forward-mode differentiation
+# ------------------------------------------------------------------------------------------
+class _Diff:
+    """d(node)/d(var) over the graph, with zero/one folding."""
+
+    def __init__(self, g: Graph, var: Node):
+        self.g = g
+        self.var = var
+        self.memo: Dict[int, Node | None] = {}
+
+    def zero(self, x):
+        return x is None
+
+    def add(self, a, b):
+        if a is None:
+            return b
+        if b is None:
+            return a
+        return a + b
+
+    def sub(self, a, b):
+        if b is None:
+            return a
+        if a is None:
+            return -b
+        return a - b
+
+    def mul(self, a, b):  # a, b may be None (zero)
+        if a is None or b is None:
+            return None
+        if a.op == "const" and a.val == 1.0:
+            return b
+        if b.op == "const" and b.val == 1.0:
+            return a
+        return a * b
+
+    def d(self, x: Node):
+        if x.idx in self.memo:
+            return self.memo[x.idx]
+        r = self._d(x)
+        self.memo[x.idx] = r
+        return r
+
+    def _d(self, x: Node):
+        g = self.g
+        if x is self.var:
+            return g.const(1.0)
+        op = x.op
+        if op in ("const", "u", "p", "t"):
+            return None
+        a = x.args
+        if op == "+":
+            return self.add(self.d(a[0]), self.d(a[1]))
+        if op == "-":
+            return self.sub(self.d(a[0]), self.d(a[1]))
+        if op == "*":
+            return self.add(self.mul(self.d(a[0]), a[1]), self.mul(a[0], self.d(a[1])))
+        if op == "/":
+            da, db = self.d(a[0]), self.d(a[1])
+            t1 = None if da is None else da / a[1]
+            t2 = None if db is None else (x * db) / a[1]
+            return self.sub(t1, t2)
+        if op == "neg":
+            da = self.d(a[0])
+            return None if da is None else -da
+        if op == "sin":
+            return self.mul(g.mk("cos", (a[0],)), self.d(a[0]))
+        if op == "cos":
+            da = self.d(a[0])
+            return None if da is None else -(g.mk("sin", (a[0],)) * da)
+        if op == "exp":
+            return self.mul(x, self.d(a[0]))
+        if op == "log":
+            da = self.d(a[0])
+            return None if da is None else da / a[0]
+        if op == "sqrt":
+            da = self.d(a[0])
+            return None if da is None else da / (g.const(2.0) * x)
+        if op == "tanh":
+            da = self.d(a[0])
+            return None if da is None else (g.const(1.0) - x * x) * da
+        if op == "abs":
+            da = self.d(a[0])
+            if da is None:
+                return None
+            return g.mk("select", (a[0], g.const(0.0), da, -da), ">=")
+        if op == "pow":
+            base, ex = a
+            db, de = self.d(base), self.d(ex)
+            r = None
+            if db is not None:
+                r = (ex * g.mk("pow", (base, ex - g.const(1.0)))) * db
+            if de is not None:
+                r = self.add(r, (x * g.mk("log", (base,))) * de)
+            return r
+        if op == "select":
+            da, db = self.d(a[2]), self.d(a[3])
+            if da is None and db is None:
+                return None
+            return g.mk("select", (a[0], a[1], da if da is not None else g.const(0.0),
+                                   db if db is not None else g.const(0.0)), x.val)
+        raise TraceError(f"cannot differentiate op {op}")
+
+
+def jacobian_nodes(tr: Trace) -> Tuple[List[List[Node]], List[Node]]:
+    """Rows J[r][c] = d out_r / d u_c and tgrad[r] = d out_r / d t (zeros as const 0)."""
+    g = tr.g
+    zero = g.const(0.0)
+    J = []
+    for r, o in enumerate(tr.outs):
+        row = []
+        for c, uc in enumerate(tr.u):
+            dd = _Diff(g, uc).d(o)
+            row.append(zero if dd is None else dd)
+        J.append(row)
+    T = []
+    for o in tr.outs:
+        dd = _Diff(g, tr.t).d(o)
+        T.append(zero if dd is None else dd)
+    return J, T
+
+
+# ------------------------------------------------------------------------------------------
+# HIP C emission
+# ------------------------------------------------------------------------------------------
+_FN = {"sin": "sinf", "cos": "cosf", "exp": "expf", "log": "logf", "sqrt": "__builtin_sqrtf",
+       "tanh": "tanhf", "abs": "__builtin_fabsf"}
+
+
+def _lit(v: float) -> str:
+    if v == int(v) and abs(v) < 2 ** 24:
+        return f"{int(v)}.0f"
+    return f"(float){v!r}"
+
+
+def _emit_body(g: Graph, outs: List[Node], out_name: str, indent="    ") -> Tuple[List[str], int]:
+    """Emit statements computing `outs` into out_name[i]; returns lines and the number of flops."""
+    need = set()
+    stack = list(outs)
+    while stack:
+        x = stack.pop()
+        if x.idx in need:
+            continue
+        need.add(x.idx)
+        stack.extend(x.args)
+    names = {}
+    lines = []
+    flops = 0
+    for x in g.nodes:
+        if x.idx not in need:
+            continue
+        op = x.op
+        if op == "const":
+            names[x.idx] = _lit(x.val)
+            continue
+        if op == "u":
+            names[x.idx] = f"u[{x.val}]"
+            continue
+        if op == "p":
+            names[x.idx] = f"p[{x.val}]"
+            continue
+        if op == "t":
+            names[x.idx] = "t"
+            continue
+        a = [names[y.idx] for y in x.args]
+        if op in ("+", "-", "*", "/"):
+            expr = f"{a[0]} {op} {a[1]}"
+            flops += 1
+        elif op == "neg":
+            expr = f"-({a[0]})"
+        elif op in _FN:
+            expr = f"{_FN[op]}({a[0]})"
+            if op == "sqrt":
+                flops += 1
+        elif op == "pow":
+            expr = f"powf({a[0]}, {a[1]})"
+        elif op == "select":
+            expr = f"({a[0]} {x.val} {a[1]}) ? {a[2]} : {a[3]}"
+        else:  # pragma: no cover
+            raise TraceError(f"unknown op {op}")
+        v = f"v{x.idx}"
+        lines.append(f"{indent}const float {v} = {expr};")
+        names[x.idx] = v
+    for i, o in enumerate(outs):
+        lines.append(f"{indent}{out_name}[{i}] = {names[o.idx]};")
+    return lines, flops
+
+
+def count_flops(tr: Trace) -> int:
+    return _emit_body(tr.g, tr.outs, "du")[1]
+
+
+def emit_rhs(f: Callable, n: int, np_: int, jac: Callable | None = None, tgrad: Callable | None = None,
+             with_jac: bool = False) -> Tuple[str, int]:
+    """HIP C source defining deg_rhs (and deg_jac/deg_tgrad when with_jac) plus the RHS flop count."""
+    tr = trace(f, n, np_, nout=n)
+    body, flops = _emit_body(tr.g, tr.outs, "du")
+    src = ["__device__ __forceinline__ void deg_rhs(const float* __restrict__ u, const float* __restrict__ p, float t,",
+           "                                        float* __restrict__ du) {",
+           "    (void)u; (void)p; (void)t;", *body, "}"]
+    if with_jac:
+        if jac is not None:
+            g, u, p, t = _new_inputs(n, np_)
+            out = jac(SymVec(u, "u"), SymVec(p, "p"), t)
+            rows = _flatten_matrix(out, n)
+            jt = Trace(g, u, p, t, [g.lift(x) for x in rows])
+            jbody, _ = _emit_body(jt.g, jt.outs, "J")
+        else:
+            J, _T = jacobian_nodes(tr)
+            jbody, _ = _emit_body(tr.g, [x for row in J for x in row], "J")
+        if tgrad is not None:
+            tt = trace(tgrad, n, np_, nout=n)
+            tbody, _ = _emit_body(tt.g, tt.outs, "dT")
+        else:
+            _J, T = jacobian_nodes(tr)
+            tbody, _ = _emit_body(tr.g, T, "dT")
+        src += ["__device__ __forceinline__ void deg_jac(const float* __restrict__ u, const float* __restrict__ p, float t,",
+                "                                        float* __restrict__ J) {",
+                "    (void)u; (void)p; (void)t;", *jbody, "}",
+                "__device__ __forceinline__ void deg_tgrad(const float* __restrict__ u, const float* __restrict__ p, float t,",
+                "                                          float* __restrict__ dT) {",
+                "    (void)u; (void)p; (void)t;", *tbody, "}"]
+    return "\n".join(src) + "\n", flops
+
+
+def _flatten_matrix(out, n):
+    """Accept a nested row list [[J11, J12, ...], ...] or a flat row-major sequence."""
+    if len(out) == n and all(isinstance(r, (list, tuple)) for r in out):
+        rows = [x for r in out for x in r]
+    else:
+        rows = list(out)
+    if len(rows) != n * n:
+        raise TraceError(f"jac must return {n}x{n} entries, got {len(rows)}")
+    return rows
+
+
+def evaluate(f: Callable, u, p, t):
+    """Evaluate an RHS written for the tracer on plain numbers (float64), for host checks."""
+    out = f(list(u), list(p), t)
+    return [float(x) for x in out]

@@ -1,0 +1,146 @@
+/* deghip.h -- C ABI of the MI355X (gfx950) EnsembleGPUKernel solver library (libdeghip.so).
+ *
+ * This is the boundary a host language binds (Julia `ccall`, Python `ctypes`). It replaces
+ * the device half of DiffEqGPU's EnsembleGPUKernel path:
+ *   vectorized_solve  (fixed step)   src/ensemblegpukernel/lowerlevel_solve.jl:53-131
+ *   vectorized_asolve (adaptive)     src/ensemblegpukernel/lowerlevel_solve.jl:253-346
+ *   ode_solve_kernel / ode_asolve_kernel   src/ensemblegpukernel/kernels.jl:1-152
+ *   the KernelAbstractions ROCm backend    ext/AMDGPUExt.jl:1-17
+ * The host side keeps output sizing and saveat normalisation (lowerlevel_solve.jl:64-109,
+ * :271-324); this library receives the normalised arrays and fills them.
+ *
+ * Conventions: plain pointers and sizes only. Every array is structure-of-arrays with a
+ * leading dimension `stride` (>= id_offset + n_traj):
+ *   u0[c*stride + i], p[j*stride + i], ts[row*stride + i], us[(row*n + c)*stride + i].
+ * Input and output arrays are device pointers owned by the caller. Functions return
+ * DEG_OK or a negative DEG_E_* code; deg_last_error() describes the last failure of the
+ * calling thread. A context is not thread-safe; the JIT cache is.
+ */
+#ifndef DEGHIP_H
+#define DEGHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define DEG_OK 0
+#define DEG_E_INVALID (-1)
+#define DEG_E_HIP (-2)
+#define DEG_E_COMPILE (-3)
+#define DEG_E_UNSUPPORTED (-4)
+#define DEG_E_NOMEM (-5)
+
+/* methods: GPUTsit5, GPUVern7, GPURosenbrock23 (src/ensemblegpukernel/gpukernel_algorithms.jl:30,57,112) */
+#define DEG_TSIT5 0
+#define DEG_VERN7 1
+#define DEG_RB23 2
+
+/* compile flags (one code object per (rhs, n, np, method, flags)) */
+#define DEG_F_ADAPTIVE (1u << 0)   /* PI-controlled adaptive stepping (vectorized_asolve) */
+#define DEG_F_PARITY (1u << 1)     /* no FP contraction: the reference's unfused Float32 sequence */
+#define DEG_F_DETPOW (1u << 2)     /* controller powers by a deterministic polynomial pow */
+#define DEG_F_KAHAN (1u << 3)      /* compensated accumulation of t */
+#define DEG_F_PERSISTENT (1u << 4) /* persistent grid; lanes refill from a device work counter */
+#define DEG_F_LEARNED (1u << 5)    /* MLP step-size proposal, accepted or rejected by the error estimate */
+#define DEG_F_FASTPOW (1u << 6)    /* controller powers by hardware log2/exp2 */
+#define DEG_F_STEPLOG (1u << 7)    /* write one log row per attempt (training data) */
+
+/* per-trajectory return codes written to retcode[] */
+#define DEG_RET_SUCCESS 0
+#define DEG_RET_DTMIN 1     /* the reference raises error("dt<dtmin") (gpu_tsit5_perform_step.jl:102) */
+#define DEG_RET_MAXITERS 2  /* attempt cap reached (no reference counterpart: the reference loops on) */
+#define DEG_RET_NONFINITE 3 /* final t or u not finite */
+
+typedef struct deg_ctx deg_ctx;
+typedef struct deg_kernel deg_kernel;
+
+/* Arguments of one ensemble solve. Mirrors the keywords of vectorized_solve /
+ * vectorized_asolve (lowerlevel_solve.jl:53-59, :253-260) after host normalisation. */
+typedef struct deg_solve_args {
+    uint32_t struct_size;    /* sizeof(deg_solve_args) */
+    int32_t n;               /* state dimension; must equal the kernel's */
+    int64_t n_traj;          /* trajectories in this call */
+    int64_t stride;          /* leading dimension of the SoA arrays; 0 means n_traj */
+    int64_t id_offset;       /* first trajectory (column) of this call */
+    const float *u0;         /* [n][stride] */
+    const float *p;          /* [np][stride]; may be NULL when np == 0 */
+    const float *t0s;        /* [stride] per-trajectory tspan[1], or NULL for t0 */
+    const float *tfs;        /* [stride] per-trajectory tspan[2], or NULL for tf */
+    float t0, tf;            /* shared tspan; t0 is also the prefill value of unsaved ts rows */
+    float dt;                /* fixed step, or initial step of the adaptive method */
+    float abstol, reltol;    /* adaptive tolerances */
+    const float *saveat;     /* [nsaveat] normalised saveat times, or NULL */
+    int32_t nsaveat;
+    int32_t nsave;           /* rows of ts/us (2 for endpoint-only output) */
+    int32_t save_everystep;  /* fixed step: write every step (rows 1..nsave-1) */
+    int32_t block_size;      /* 0: 256 */
+    float *ts;               /* [nsave][stride] */
+    float *us;               /* [nsave][n][stride] */
+    int32_t *retcode;        /* [stride] or NULL */
+    int32_t *naccept;        /* [stride] or NULL: accepted steps */
+    int32_t *nreject;        /* [stride] or NULL: rejected attempts */
+    int64_t max_attempts;    /* per-trajectory attempt cap; 0: 10^7 (ends lanes whose t stops advancing) */
+    void *stream;            /* hipStream_t; NULL: the context's stream */
+    int32_t grid_blocks;     /* persistent kernels: 0 fills the device */
+    int32_t pad0;
+    float *elapsed_ms;       /* optional: kernel time from HIP events around the launch (synchronises) */
+    const void *mlp;         /* DEG_F_LEARNED: packed proposer weights (device), see deg_mlp_* */
+    float *steplog;          /* DEG_F_STEPLOG: [steplog_cap][DEG_STEPLOG_COLS] rows (device) */
+    uint32_t *steplog_count; /* DEG_F_STEPLOG: device row counter (zeroed by the caller) */
+    int64_t steplog_cap;
+} deg_solve_args;
+
+/* Context: one device, one default stream, the JIT cache directory. */
+int deg_ctx_create(int device, deg_ctx **out);
+int deg_ctx_destroy(deg_ctx *ctx);
+/* Number of compute units of the context's device. */
+int deg_ctx_num_cus(deg_ctx *ctx);
+
+/* Directory for compiled code objects (shared by all contexts; NULL resets to none). */
+int deg_set_cache_dir(const char *dir);
+
+/* Compile (or fetch from the cache) the solver for one traced right-hand side.
+ * rhs_src defines, in HIP C:
+ *   __device__ void deg_rhs(const float* u, const float* p, float t, float* du);
+ * and for DEG_RB23 also deg_jac (row-major J[r*n + c] = d du_r / d u_c) and deg_tgrad.
+ * Replaces the compile of ode_solve_kernel / ode_asolve_kernel for one problem type. */
+int deg_kernel_compile(deg_ctx *ctx, const char *rhs_src, int n, int np, int method, uint32_t flags,
+                       deg_kernel **out);
+int deg_kernel_destroy(deg_kernel *k);
+/* Resource figures of the compiled kernel: which = 0 VGPRs, 1 SGPRs, 2 scratch bytes/lane,
+ * 3 max block size, 4 resident blocks per CU at block_size 256. */
+int deg_kernel_attr(deg_kernel *k, int which, int *value);
+
+/* Compile into the cache without a GPU (build-time warm-up). Writes the 16-hex-digit cache
+ * key to key_out (>= 17 bytes) when non-NULL. */
+int deg_jit_precompile(const char *rhs_src, int n, int np, int method, uint32_t flags, char *key_out);
+
+/* The full HIP C++ source the JIT compiles for this configuration (for inspection).
+ * Returns the length; copies at most cap-1 bytes plus a terminator into buf when non-NULL. */
+int64_t deg_jit_source(const char *rhs_src, int n, int np, int method, uint32_t flags, char *buf, int64_t cap);
+
+/* Launch one ensemble solve (asynchronous on the stream unless elapsed_ms is set).
+ * Replaces the kernel launch of vectorized_solve / vectorized_asolve
+ * (lowerlevel_solve.jl:119-123, :339-343). */
+int deg_solve(deg_kernel *k, const deg_solve_args *args);
+
+/* Wait for the context's stream. */
+int deg_sync(deg_ctx *ctx);
+
+/* Last error message of the calling thread ("" when none). */
+const char *deg_last_error(void);
+
+/* Library version string. */
+const char *deg_version(void);
+
+/* sizeof(deg_solve_args) as compiled into the library, for bindings to check their layout. */
+uint32_t deg_solve_args_size(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DEGHIP_H */

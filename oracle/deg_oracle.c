@@ -1,0 +1,848 @@
+/* deg_oracle.c -- CPU restatement of DiffEqGPU's EnsembleGPUKernel ODE path.
+ *
+ * TEST INFRASTRUCTURE. This library is the checker for the HIP solver: only the test
+ * suite, the smoke check and the benchmark's CPU-baseline leg load it. The product
+ * never calls it and has no CPU fallback through it.
+ *
+ * It restates, operation for operation in IEEE Float32 (build with -ffp-contract=off):
+ *   fixed-step loop and endpoint rules      src/ensemblegpukernel/kernels.jl:1-72
+ *   adaptive loop and endpoint rules        src/ensemblegpukernel/kernels.jl:74-152
+ *   Tsit5 fixed / adaptive step             perform_step/gpu_tsit5_perform_step.jl:1-64 / :68-175
+ *   Vern7 fixed / adaptive step             perform_step/gpu_vern7_perform_step.jl:1-74 / :78-210
+ *   Rosenbrock23 fixed / adaptive step      perform_step/gpu_rosenbrock23_perform_step.jl:1-70 / :74-201
+ *   PI controller constants                 integrators/integrator_utils.jl:1-11, alg_utils.jl:5-8
+ *   saveat / save_everystep writes          integrators/integrator_utils.jl:13-50
+ *   dense output                            integrators/nonstiff/interpolants.jl:28-190 (Vern7), :372-387
+ *                                           (Tsit5), integrators/stiff/interpolants.jl (Rosenbrock23)
+ *   error norm                              DiffEqBase.ODE_DEFAULT_NORM = sqrt(sum(abs2, x) / n)
+ * with these stated deviations, shared with the GPU solver:
+ *   - a lane that would hit the reference's in-kernel error("dt<dtmin") stops with a
+ *     per-trajectory return code instead of aborting the launch;
+ *   - a per-trajectory attempt cap (max_attempts) ends a lane that would loop forever;
+ *   - Rosenbrock23 forms W = I - gamma J and solves it in FP64 (LU with partial
+ *     pivoting) instead of Float32 Cramer's rule (linalg/linsolve.jl:31-57);
+ *   - optional compensated (Kahan) accumulation of t (flag DEG_O_KAHAN).
+ * Dense-output polynomials use fused multiply-add Horner steps (the reference's
+ * @muladd/@evalpoly), vector combinations are left folds without fusion.
+ *
+ * An FP64 Vern9 integrator (tableaus/verner_tableaus.jl:1198-1332, same PI controller
+ * in double) provides the high-accuracy reference solutions.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "tableaus.h"
+#include "systems.h"
+
+#define OMAXN 16
+#define OMAXS 17
+
+enum { OM_TSIT5 = 0, OM_VERN7 = 1, OM_RB23 = 2 };
+enum { DEG_O_KAHAN = 1, DEG_O_DETPOW = 2 };
+/* per-trajectory return codes (same values as the C ABI of the HIP library) */
+enum { ORET_SUCCESS = 0, ORET_DTMIN = 1, ORET_MAXITERS = 2, ORET_NONFINITE = 3 };
+
+/* ------------------------------------------------------------------------------------ */
+/* Float32 tableau in the shape every explicit method here uses                         */
+/* ------------------------------------------------------------------------------------ */
+typedef struct {
+    int s;
+    int fsal;                 /* Tsit5: the last stage is evaluated at the new solution */
+    float c[OMAXS + 1];
+    float a[OMAXS + 1][OMAXS + 1];
+    unsigned char am[OMAXS + 1][OMAXS + 1];
+    float b[OMAXS + 1];
+    unsigned char bm[OMAXS + 1];
+    float bt[OMAXS + 1];
+    unsigned char btm[OMAXS + 1];
+} erk32;
+
+static void erk32_load(erk32 *T, int s, int fsal, const double *c, const double *a, const unsigned char *am,
+                       const double *b, const unsigned char *bm, const double *bt, const unsigned char *btm) {
+    memset(T, 0, sizeof(*T));
+    T->s = s;
+    T->fsal = fsal;
+    for (int i = 0; i <= s; ++i) {
+        T->c[i] = (float)c[i];
+        T->b[i] = (float)b[i];
+        T->bm[i] = bm[i];
+        T->bt[i] = (float)bt[i];
+        T->btm[i] = btm[i];
+        for (int j = 0; j <= s; ++j) {
+            T->a[i][j] = (float)a[i * (s + 1) + j];
+            T->am[i][j] = am[i * (s + 1) + j];
+        }
+    }
+}
+
+static erk32 TAB_T5, TAB_V7;
+static void init_v7x(void);
+static float T5R[8][5];
+static int tabs_ready = 0;
+
+static void init_tabs(void) {
+    if (tabs_ready) return;
+    erk32_load(&TAB_T5, OT5_S, 1, OT5_C, &OT5_A[0][0], &OT5_AM[0][0], OT5_B, OT5_BM, OT5_BT, OT5_BTM);
+    erk32_load(&TAB_V7, OV7_S, 0, OV7_C, &OV7_A[0][0], &OV7_AM[0][0], OV7_B, OV7_BM, OV7_BT, OV7_BTM);
+    for (int i = 0; i < 8; ++i)
+        for (int k = 0; k < 5; ++k) T5R[i][k] = (float)OT5_R[i][k];
+    init_v7x();
+    tabs_ready = 1;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Julia Float32 semantics helpers                                                       */
+/* ------------------------------------------------------------------------------------ */
+/* Julia's min/max propagate NaN (Base.min for IEEE floats). */
+static inline float jl_min(float a, float b) {
+    if (a != a || b != b) return a + b;
+    return (b < a) ? b : a;
+}
+static inline float jl_max(float a, float b) {
+    if (a != a || b != b) return a + b;
+    return (b > a) ? b : a;
+}
+
+/* Deterministic power x^y for x >= 0 built only from IEEE +,-,*,/ and exponent bit
+ * manipulation: log2 by the atanh series on a mantissa in [sqrt(1/2), sqrt(2)], exp2 by a
+ * degree-7 Taylor polynomial on [-1/2, 1/2] ln 2. Relative error about 1e-6, which is
+ * irrelevant for a step-size controller. The HIP solver carries a bit-identical copy so
+ * that adaptive runs can be compared bit for bit (flag DEG_O_DETPOW); the default path
+ * uses the platform powf. */
+static inline float det_exp2i(int k) {
+    uint32_t b = (uint32_t)(k + 127) << 23;
+    float r;
+    memcpy(&r, &b, 4);
+    return r;
+}
+static float det_pow(float x, float y) {
+    if (x != x || y != y) return x + y;
+    if (x == 0.0f) return 0.0f;
+    if (x == INFINITY) return INFINITY;
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    int e = (int)((b >> 23) & 0xffu);
+    if (e == 0) {
+        x = x * 16777216.0f;
+        memcpy(&b, &x, 4);
+        e = (int)((b >> 23) & 0xffu) - 24;
+    }
+    e -= 127;
+    uint32_t mb = (b & 0x007fffffu) | 0x3f800000u;
+    float m;
+    memcpy(&m, &mb, 4);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float lnm = (2.0f * s) * (1.0f + s2 * (0.333333343f + s2 * (0.2f + s2 * (0.142857149f + s2 * 0.111111112f))));
+    float l2 = (float)e + lnm * 1.44269502f;
+    float z = y * l2;
+    if (z > 128.0f) return INFINITY;
+    if (z < -149.0f) return 0.0f;
+    float n = rintf(z);
+    float g = (z - n) * 0.693147182f;
+    float r = 1.0f + g * (1.0f + g * (0.5f + g * (0.166666672f + g * (0.0416666679f + g * (0.00833333377f + g * (0.00138888892f + g * 0.000198412701f))))));
+    int ni = (int)n;
+    int h1 = ni / 2;
+    int h2 = ni - h1;
+    return (r * det_exp2i(h1)) * det_exp2i(h2);
+}
+
+static inline float ctl_pow(float x, float y, int detpow) { return detpow ? det_pow(x, y) : powf(x, y); }
+
+/* ODE_DEFAULT_NORM for an SVector: sqrt(((x1^2 + x2^2) + ...) / n) */
+static inline float rms_norm(const float *x, int n) {
+    float acc = x[0] * x[0];
+    for (int c = 1; c < n; ++c) acc = acc + x[c] * x[c];
+    return sqrtf(acc / (float)n);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Explicit RK stage combinations in the reference's association order                  */
+/* ------------------------------------------------------------------------------------ */
+/* stage i input: uprev + dt * (sum_j a_ij k_j), except stage 2: uprev + (dt*a21) * k1 */
+static void stage_input(const erk32 *T, int i, int n, const float *uprev, float dt,
+                        float k[][OMAXN], float *out) {
+    if (i == 2) {
+        float a = dt * T->a[2][1];
+        for (int c = 0; c < n; ++c) out[c] = uprev[c] + a * k[1][c];
+        return;
+    }
+    for (int c = 0; c < n; ++c) {
+        float acc = 0.0f;
+        int first = 1;
+        for (int j = 1; j < i; ++j) {
+            if (!T->am[i][j]) continue;
+            float term = T->a[i][j] * k[j][c];
+            acc = first ? term : acc + term;
+            first = 0;
+        }
+        out[c] = uprev[c] + dt * acc;
+    }
+}
+
+static void weighted_sum(const float *w, const unsigned char *wm, int s, int n, float dt,
+                         float k[][OMAXN], float *out) {
+    for (int c = 0; c < n; ++c) {
+        float acc = 0.0f;
+        int first = 1;
+        for (int j = 1; j <= s; ++j) {
+            if (!wm[j]) continue;
+            float term = w[j] * k[j][c];
+            acc = first ? term : acc + term;
+            first = 0;
+        }
+        out[c] = dt * acc;
+    }
+}
+
+/* One explicit RK attempt from (uprev, t) with step dt. k[1] must hold f(uprev, t).
+ * Fills k[2..s], unew. */
+static void erk_attempt(const erk32 *T, const osys *S, const float *p, const float *uprev, float t,
+                        float dt, float k[][OMAXN], float *unew) {
+    int n = S->n;
+    float tmp[OMAXN];
+    int s = T->s;
+    for (int i = 2; i <= s; ++i) {
+        stage_input(T, i, n, uprev, dt, k, tmp);
+        if (T->fsal && i == s) {
+            for (int c = 0; c < n; ++c) unew[c] = tmp[c];
+        }
+        S->f(tmp, p, t + T->c[i] * dt, k[i]);
+    }
+    if (!T->fsal) {
+        float inc[OMAXN];
+        weighted_sum(T->b, T->bm, s, n, dt, k, inc);
+        for (int c = 0; c < n; ++c) unew[c] = uprev[c] + inc[c];
+    }
+}
+
+/* scaled error  EEst = norm( dt*(sum btilde_j k_j) ./ (abstol .+ max.(|uprev|,|u|) * reltol) ) */
+static float erk_eest(const erk32 *T, int n, float dt, float k[][OMAXN], const float *uprev,
+                      const float *u, float abstol, float reltol) {
+    float e[OMAXN];
+    weighted_sum(T->bt, T->btm, T->s, n, dt, k, e);
+    for (int c = 0; c < n; ++c) {
+        float sc = abstol + jl_max(fabsf(uprev[c]), fabsf(u[c])) * reltol;
+        e[c] = e[c] / sc;
+    }
+    return rms_norm(e, n);
+}
+
+/* Tsit5 dense output: y0 + dt*(b1(th) k1 + ... + b7(th) k7) */
+static void tsit5_interp(float th, float dt, int n, const float *y0, float k[][OMAXN], float *out) {
+    float bth[8];
+    bth[1] = th * fmaf(th, fmaf(th, fmaf(th, T5R[1][4], T5R[1][3]), T5R[1][2]), T5R[1][1]);
+    for (int i = 2; i <= 7; ++i) bth[i] = th * (th * fmaf(th, fmaf(th, T5R[i][4], T5R[i][3]), T5R[i][2]));
+    for (int c = 0; c < n; ++c) {
+        float acc = bth[1] * k[1][c];
+        for (int i = 2; i <= 7; ++i) acc = acc + bth[i] * k[i][c];
+        out[c] = y0[c] + dt * acc;
+    }
+}
+
+/* Rosenbrock23 dense output: y0 + dt*(c1 k1 + c2 k2), c1 = th(1-th)/(1-2d), c2 = th(th-2d)/(1-2d) */
+static void rb23_interp(float th, float dt, int n, float d, const float *y0, const float *k1,
+                        const float *k2, float *out) {
+    float den = fmaf(-2.0f, d, 1.0f);
+    float c1 = (th * (1.0f - th)) / den;
+    float c2 = (th * fmaf(-2.0f, d, th)) / den;
+    for (int c = 0; c < n; ++c) out[c] = y0[c] + dt * (c1 * k1[c] + c2 * k2[c]);
+}
+
+/* Vern7 dense output (interpolants.jl:28-190): six extra stages k11..k16 from uprev, then
+ * y0 + th*step_sum + th*(th-1)*correction with q_i(th) from the synthetic division of the
+ * reference's r coefficients. Scalar muladds (stage times, polynomial Horner steps) are fused;
+ * vector combinations are left folds without fusion. */
+static float V7X_A[17][17], V7X_C[17], V7R[17][8];
+static unsigned char V7X_AM[17][17];
+static const int V7_QSTAGES[13] = {1, 4, 5, 6, 7, 8, 9, 11, 12, 13, 14, 15, 16};
+
+static void init_v7x(void) {
+    for (int i = 0; i < 17; ++i) {
+        V7X_C[i] = (float)OV7X_C[i];
+        for (int j = 0; j < 17; ++j) {
+            V7X_A[i][j] = (float)OV7X_A[i][j];
+            V7X_AM[i][j] = OV7X_AM[i][j];
+        }
+        for (int k = 0; k < 8; ++k) V7R[i][k] = (float)OV7R[i][k];
+    }
+}
+
+static float vern7_q(int i, float th) {
+    const float *r = V7R[i];
+    float s5 = r[7];
+    float s4 = r[6] + s5;
+    float s3 = r[5] + s4;
+    float s2 = r[4] + s3;
+    float s1 = r[3] + s2;
+    float c0 = r[2] + s1;
+    return fmaf(th, fmaf(th, fmaf(th, fmaf(th, fmaf(th, s5, s4), s3), s2), s1), c0);
+}
+
+static void vern7_interp(const osys *S, const float *p, float th, float dt, float tprev, int n, const float *y0,
+                         float k[][OMAXN], float *out) {
+    float kx[17][OMAXN];
+    for (int s = 1; s <= 10; ++s)
+        for (int c = 0; c < n; ++c) kx[s][c] = k[s][c];
+    float tmp[OMAXN];
+    for (int i = 11; i <= 16; ++i) {
+        for (int c = 0; c < n; ++c) {
+            float acc = 0.0f;
+            int first = 1;
+            for (int j = 1; j < i; ++j) {
+                if (!V7X_AM[i][j]) continue;
+                float term = V7X_A[i][j] * kx[j][c];
+                acc = first ? term : acc + term;
+                first = 0;
+            }
+            tmp[c] = y0[c] + dt * acc;
+        }
+        S->f(tmp, p, fmaf(V7X_C[i], dt, tprev), kx[i]);
+    }
+    float q[17];
+    for (int m = 0; m < 13; ++m) q[V7_QSTAGES[m]] = vern7_q(V7_QSTAGES[m], th);
+    const erk32 *T = &TAB_V7;
+    for (int c = 0; c < n; ++c) {
+        float ss = T->b[1] * kx[1][c];
+        const int bi[6] = {4, 5, 6, 7, 8, 9};
+        for (int m = 0; m < 6; ++m) ss = ss + T->b[bi[m]] * kx[bi[m]][c];
+        float step_sum = dt * ss;
+        float cs = kx[1][c] * q[1];
+        for (int m = 1; m < 13; ++m) cs = cs + kx[V7_QSTAGES[m]][c] * q[V7_QSTAGES[m]];
+        float corr = dt * cs;
+        out[c] = (y0[c] + th * step_sum) + (th * (th - 1.0f)) * corr;
+    }
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* FP64 LU with partial pivoting for W = I - gamma*J (Rosenbrock23)                     */
+/* ------------------------------------------------------------------------------------ */
+typedef struct {
+    double lu[OMAXN][OMAXN];
+    int perm[OMAXN];
+} olu;
+
+static void w_factor(olu *F, int n, float gam, const float *J) {
+    double g = (double)gam;
+    for (int r = 0; r < n; ++r) {
+        for (int c = 0; c < n; ++c) F->lu[r][c] = (r == c ? 1.0 : 0.0) - g * (double)J[r * n + c];
+        F->perm[r] = r;
+    }
+    for (int k = 0; k < n; ++k) {
+        int piv = k;
+        double mx = fabs(F->lu[k][k]);
+        for (int i = k + 1; i < n; ++i) {
+            double v = fabs(F->lu[i][k]);
+            if (v > mx) {
+                mx = v;
+                piv = i;
+            }
+        }
+        if (piv != k) {
+            for (int c = 0; c < n; ++c) {
+                double tv = F->lu[k][c];
+                F->lu[k][c] = F->lu[piv][c];
+                F->lu[piv][c] = tv;
+            }
+            int tp = F->perm[k];
+            F->perm[k] = F->perm[piv];
+            F->perm[piv] = tp;
+        }
+        for (int i = k + 1; i < n; ++i) {
+            double l = F->lu[i][k] / F->lu[k][k];
+            F->lu[i][k] = l;
+            for (int c = k + 1; c < n; ++c) F->lu[i][c] = F->lu[i][c] - l * F->lu[k][c];
+        }
+    }
+}
+
+static void w_solve(const olu *F, int n, const float *b, float *x) {
+    double y[OMAXN];
+    for (int i = 0; i < n; ++i) {
+        double v = (double)b[F->perm[i]];
+        for (int j = 0; j < i; ++j) v = v - F->lu[i][j] * y[j];
+        y[i] = v;
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double v = y[i];
+        for (int j = i + 1; j < n; ++j) v = v - F->lu[i][j] * y[j];
+        y[i] = v / F->lu[i][i];
+    }
+    for (int i = 0; i < n; ++i) x[i] = (float)y[i];
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Per-trajectory solver                                                                 */
+/* ------------------------------------------------------------------------------------ */
+typedef struct {
+    const osys *S;
+    int method;
+    int adaptive;
+    int64_t N;
+    float dt, abstol, reltol;
+    const float *saveat;
+    int nsaveat, nsave, save_everystep;
+    int kahan, detpow;
+    int64_t max_attempts;
+    float *ts, *us;
+} ojob;
+
+static inline void put_u(const ojob *J, int row, int64_t i, const float *u) {
+    int n = J->S->n;
+    for (int c = 0; c < n; ++c) J->us[((int64_t)row * n + c) * J->N + i] = u[c];
+}
+
+/* t <- t + dt, optionally compensated */
+static inline float advance_t(float t, float dt, float *comp, int kahan) {
+    if (!kahan) return t + dt;
+    float y = dt - *comp;
+    float tn = t + y;
+    *comp = (tn - t) - y;
+    return tn;
+}
+
+static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p, float t0, float tf,
+                      int32_t *nacc_out, int32_t *nrej_out) {
+    const osys *S = J->S;
+    const int n = S->n;
+    const erk32 *T = (J->method == OM_TSIT5) ? &TAB_T5 : &TAB_V7;
+    float u[OMAXN], uprev[OMAXN], unew[OMAXN];
+    float k[OMAXS + 1][OMAXN];
+    float rk1[OMAXN], rk2[OMAXN]; /* Rosenbrock23 interpolation slopes */
+    memcpy(u, u0, sizeof(float) * n);
+    memcpy(uprev, u0, sizeof(float) * n);
+    int retcode = ORET_SUCCESS;
+    int32_t nacc = 0, nrej = 0;
+    int64_t attempts = 0;
+    const int use_saveat = (J->saveat != NULL);
+    int cur = 0; /* 0-based saveat cursor */
+    int step_idx = 1; /* 0-based row of the next save_everystep write */
+
+    /* kernels.jl:35-45 / :116-126: initial save */
+    if (use_saveat) {
+        if (J->nsaveat > 0 && t0 == J->saveat[0]) {
+            put_u(J, 0, i, u0);
+            cur = 1;
+        }
+    } else {
+        J->ts[i] = t0;
+        put_u(J, 0, i, u0);
+    }
+
+    float t = t0, tprev = t0, tcomp = 0.0f;
+    const float d = 1.0f / (2.0f + sqrtf(2.0f));
+    float dtstep = J->dt; /* fixed dt, or the accepted dt of the last adaptive step */
+
+    if (!J->adaptive) {
+        /* ---------------- fixed step: kernels.jl:49-52 -------------------------------- */
+        const float dt = J->dt;
+        int first = 1;
+        while (t < tf) {
+            if (attempts >= J->max_attempts) { retcode = ORET_MAXITERS; break; }
+            ++attempts;
+            memcpy(uprev, u, sizeof(float) * n);
+            tprev = t;
+            float told = t;
+            t = advance_t(t, dt, &tcomp, J->kahan);
+            if (J->method == OM_RB23) {
+                /* gpu_rosenbrock23_perform_step.jl:1-70 */
+                float Jm[OMAXN * OMAXN], dT[OMAXN], F0[OMAXN], rhs[OMAXN], tmp[OMAXN], F1[OMAXN], x[OMAXN];
+                float gam = dt * d;
+                float dto2 = dt / 2.0f;
+                S->jac(uprev, p, told, Jm);
+                S->tgrad(uprev, p, told, dT);
+                olu F;
+                w_factor(&F, n, gam, Jm);
+                S->f(uprev, p, told, F0);
+                for (int c = 0; c < n; ++c) rhs[c] = F0[c] + gam * dT[c];
+                w_solve(&F, n, rhs, rk1);
+                for (int c = 0; c < n; ++c) tmp[c] = uprev[c] + dto2 * rk1[c];
+                S->f(tmp, p, told + dto2, F1);
+                for (int c = 0; c < n; ++c) rhs[c] = F1[c] - rk1[c];
+                w_solve(&F, n, rhs, x);
+                for (int c = 0; c < n; ++c) rk2[c] = x[c] + rk1[c];
+                for (int c = 0; c < n; ++c) u[c] = uprev[c] + dt * rk2[c];
+            } else {
+                /* gpu_tsit5_perform_step.jl:1-64, gpu_vern7_perform_step.jl:1-74 */
+                if (T->fsal && !first) {
+                    memcpy(k[1], k[T->s], sizeof(float) * n);
+                } else {
+                    S->f(uprev, p, told, k[1]);
+                }
+                erk_attempt(T, S, p, uprev, told, dt, k, unew);
+                memcpy(u, unew, sizeof(float) * n);
+            }
+            first = 0;
+            ++nacc;
+            /* savevalues!: integrator_utils.jl:13-50 */
+            if (use_saveat) {
+                while (cur < J->nsaveat && J->saveat[cur] <= t) {
+                    float st = J->saveat[cur];
+                    float th = (st - tprev) / dt;
+                    float out[OMAXN];
+                    if (J->method == OM_TSIT5) tsit5_interp(th, dt, n, uprev, k, out);
+                    else if (J->method == OM_RB23) rb23_interp(th, dt, n, d, uprev, rk1, rk2, out);
+                    else vern7_interp(S, p, th, dt, tprev, n, uprev, k, out);
+                    if (cur < J->nsave) {
+                        put_u(J, cur, i, out);
+                        J->ts[(int64_t)cur * J->N + i] = st;
+                    }
+                    ++cur;
+                }
+            } else if (J->save_everystep) {
+                if (step_idx < J->nsave) {
+                    put_u(J, step_idx, i, u);
+                    J->ts[(int64_t)step_idx * J->N + i] = t;
+                }
+                ++step_idx;
+            }
+        }
+        /* kernels.jl:53-62 */
+        if (t > tf && !use_saveat && J->save_everystep && retcode == ORET_SUCCESS) {
+            float out[OMAXN];
+            float th = (tf - tprev) / dt;
+            int row = J->nsave - 1;
+            if (J->method == OM_TSIT5) tsit5_interp(th, dt, n, uprev, k, out);
+            else if (J->method == OM_RB23) rb23_interp(th, dt, n, d, uprev, rk1, rk2, out);
+            else vern7_interp(S, p, th, dt, tprev, n, uprev, k, out);
+            put_u(J, row, i, out);
+            J->ts[(int64_t)row * J->N + i] = tf;
+        }
+    } else {
+        /* ---------------- adaptive: kernels.jl:128-131 -------------------------------- */
+        const int order = (J->method == OM_TSIT5) ? 5 : (J->method == OM_VERN7 ? 7 : 2);
+        /* integrator_utils.jl:1-11 */
+        const float beta1 = (float)(7.0 / (10.0 * order));
+        const float beta2 = (float)(2.0 / (5.0 * order));
+        const float qmax = 10.0f, qmin = (float)(1.0 / 5.0), gamma = (float)(9.0 / 10.0);
+        const float qoldinit = (float)1.0e-4;
+        float qold = qoldinit;
+        float dt = J->dt; /* integ.dtnew */
+        int have_k7 = 0;
+        const float e32 = 6.0f + sqrtf(2.0f);
+        while (t < tf) {
+            memcpy(uprev, u, sizeof(float) * n);
+            float EEst = INFINITY;
+            /* FSAL: gpu_tsit5_perform_step.jl:92-97 */
+            if (J->method == OM_TSIT5) {
+                if (have_k7) memcpy(k[1], k[7], sizeof(float) * n);
+                else S->f(uprev, p, t, k[1]);
+            }
+            int stop = 0;
+            while (EEst > 1.0f) {
+                if (dt < 1.0e-14f) { retcode = ORET_DTMIN; stop = 1; break; }
+                if (attempts >= J->max_attempts) { retcode = ORET_MAXITERS; stop = 1; break; }
+                ++attempts;
+                float q11 = 0.0f, q;
+                if (J->method == OM_RB23) {
+                    /* gpu_rosenbrock23_perform_step.jl:106-153 */
+                    float Jm[OMAXN * OMAXN], dT[OMAXN], F0[OMAXN], F1[OMAXN], F2[OMAXN], rhs[OMAXN], tmp[OMAXN], x[OMAXN];
+                    float gam = dt * d;
+                    float dto2 = dt / 2.0f;
+                    float dto6 = dt / 6.0f;
+                    S->jac(uprev, p, t, Jm);
+                    S->tgrad(uprev, p, t, dT);
+                    olu F;
+                    w_factor(&F, n, gam, Jm);
+                    S->f(uprev, p, t, F0);
+                    for (int c = 0; c < n; ++c) rhs[c] = F0[c] + gam * dT[c];
+                    w_solve(&F, n, rhs, rk1);
+                    for (int c = 0; c < n; ++c) tmp[c] = uprev[c] + dto2 * rk1[c];
+                    S->f(tmp, p, t + dto2, F1);
+                    for (int c = 0; c < n; ++c) rhs[c] = F1[c] - rk1[c];
+                    w_solve(&F, n, rhs, x);
+                    for (int c = 0; c < n; ++c) rk2[c] = x[c] + rk1[c];
+                    for (int c = 0; c < n; ++c) unew[c] = uprev[c] + dt * rk2[c];
+                    S->f(unew, p, t + dt, F2);
+                    for (int c = 0; c < n; ++c)
+                        rhs[c] = ((F2[c] - e32 * (rk2[c] - F1[c])) - 2.0f * (rk1[c] - F0[c])) + dt * dT[c];
+                    float k3[OMAXN];
+                    w_solve(&F, n, rhs, k3);
+                    float e[OMAXN] = {0};
+                    for (int c = 0; c < n; ++c) e[c] = dto6 * ((rk1[c] - 2.0f * rk2[c]) + k3[c]);
+                    for (int c = 0; c < n; ++c) {
+                        float sc = J->abstol + jl_max(fabsf(uprev[c]), fabsf(unew[c])) * J->reltol;
+                        e[c] = e[c] / sc;
+                    }
+                    EEst = rms_norm(e, n);
+                } else {
+                    if (J->method == OM_VERN7) S->f(uprev, p, t, k[1]); /* gpu_vern7_perform_step.jl:117 */
+                    erk_attempt(T, S, p, uprev, t, dt, k, unew);
+                    EEst = erk_eest(T, n, dt, k, uprev, unew, J->abstol, J->reltol);
+                }
+                /* PI controller: gpu_tsit5_perform_step.jl:124-137 */
+                if (EEst == 0.0f) {
+                    q = 1.0f / qmax;
+                } else {
+                    q11 = ctl_pow(EEst, beta1, J->detpow);
+                    q = q11 / ctl_pow(qold, beta2, J->detpow);
+                }
+                if (EEst > 1.0f) {
+                    dt = dt / jl_min(1.0f / qmin, q11 / gamma);
+                    ++nrej;
+                } else {
+                    q = jl_max(1.0f / qmax, jl_min(1.0f / qmin, q / gamma));
+                    qold = jl_max(EEst, qoldinit);
+                    float dtnew = dt / q;
+                    dtnew = jl_min(fabsf(dtnew), fabsf((tf - t) - dt));
+                    dtstep = dt;
+                    tprev = t;
+                    memcpy(u, unew, sizeof(float) * n);
+                    if ((tf - t) - dt < 1.0e-14f) {
+                        t = tf;
+                        tcomp = 0.0f;
+                    } else {
+                        t = advance_t(t, dt, &tcomp, J->kahan);
+                    }
+                    dt = dtnew;
+                    ++nacc;
+                    have_k7 = 1;
+                }
+            }
+            if (stop) break;
+            /* savevalues! after the accepted step */
+            if (use_saveat) {
+                while (cur < J->nsaveat && J->saveat[cur] <= t) {
+                    float st = J->saveat[cur];
+                    float th = (st - tprev) / dtstep;
+                    float out[OMAXN];
+                    if (J->method == OM_TSIT5) tsit5_interp(th, dtstep, n, uprev, k, out);
+                    else if (J->method == OM_RB23) rb23_interp(th, dtstep, n, d, uprev, rk1, rk2, out);
+                    else vern7_interp(S, p, th, dtstep, tprev, n, uprev, k, out);
+                    if (cur < J->nsave) {
+                        put_u(J, cur, i, out);
+                        J->ts[(int64_t)cur * J->N + i] = st;
+                    }
+                    ++cur;
+                }
+            }
+        }
+    }
+    /* kernels.jl:59-62: endpoint-only output keeps the last state */
+    if (!use_saveat && !J->save_everystep) {
+        put_u(J, 1, i, u);
+        J->ts[J->N + i] = t;
+    }
+    if (retcode == ORET_SUCCESS) {
+        int fin = isfinite(t);
+        for (int c = 0; c < n; ++c) fin = fin && isfinite(u[c]);
+        if (!fin) retcode = ORET_NONFINITE;
+    }
+    *nacc_out = nacc;
+    *nrej_out = nrej;
+    (void)dtstep;
+    return retcode;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Exported: Float32 restatement over an ensemble (SoA layout, same as the HIP solver)   */
+/* ------------------------------------------------------------------------------------ */
+/* u0[c*N + i], p[j*N + i]; outputs ts[row*N + i], us[(row*n + c)*N + i].
+ * t0s/tfs may be NULL (then the shared t0/tf apply). saveat may be NULL.
+ * Returns 0, or a negative value for an unsupported combination. */
+int deg_oracle_solve(int system, int method, int adaptive, int64_t N,
+                     const float *u0, const float *p, const float *t0s, const float *tfs,
+                     float t0, float tf, float dt, float abstol, float reltol,
+                     const float *saveat, int nsaveat, int nsave, int save_everystep,
+                     int flags, int64_t max_attempts,
+                     float *ts, float *us, int32_t *retcode, int32_t *naccept, int32_t *nreject,
+                     int nthreads) {
+    if (system < 0 || system >= OSYS_COUNT) return -1;
+    if (method < OM_TSIT5 || method > OM_RB23) return -1;
+    init_tabs();
+    const osys *S = &OSYSTEMS[system];
+    ojob J;
+    J.S = S;
+    J.method = method;
+    J.adaptive = adaptive;
+    J.N = N;
+    J.dt = dt;
+    J.abstol = abstol;
+    J.reltol = reltol;
+    J.saveat = saveat;
+    J.nsaveat = nsaveat;
+    J.nsave = nsave;
+    J.save_everystep = save_everystep;
+    J.kahan = (flags & DEG_O_KAHAN) != 0;
+    J.detpow = (flags & DEG_O_DETPOW) != 0;
+    J.max_attempts = max_attempts > 0 ? max_attempts : (int64_t)1 << 62;
+    J.ts = ts;
+    J.us = us;
+    int bad = 0;
+    if (nthreads <= 0) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads) reduction(| : bad)
+    for (int64_t i = 0; i < N; ++i) {
+        float uu[OMAXN], pp[OMAXN];
+        for (int c = 0; c < S->n; ++c) uu[c] = u0[(int64_t)c * N + i];
+        for (int j = 0; j < S->np; ++j) pp[j] = p[(int64_t)j * N + i];
+        float a = t0s ? t0s[i] : t0;
+        float b = tfs ? tfs[i] : tf;
+        int32_t na = 0, nr = 0;
+        int rc = solve_traj(&J, i, uu, pp, a, b, &na, &nr);
+        if (rc < 0) bad = 1;
+        if (retcode) retcode[i] = rc < 0 ? -1 : rc;
+        if (naccept) naccept[i] = na;
+        if (nreject) nreject[i] = nr;
+    }
+    return bad ? -2 : 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* FP64 Vern9 with the same PI controller, stepping exactly onto each save time          */
+/* ------------------------------------------------------------------------------------ */
+static int vern9_traj(orhs_fn_d f, int n, const double *u0, const double *p, double t0,
+                      double abstol, double reltol, double dt0, const double *tsave, int nsave,
+                      double *out, int64_t max_steps, int64_t *nsteps_out) {
+    const int s = OV9_S;
+    const double beta1 = 7.0 / 90.0, beta2 = 2.0 / 45.0, qmax = 10.0, qmin = 0.2, gamma = 0.9;
+    double u[OMAXN], un[OMAXN], tmp[OMAXN], k[OMAXS + 1][OMAXN];
+    memcpy(u, u0, sizeof(double) * n);
+    double t = t0, dt = dt0, qold = 1e-4;
+    int64_t steps = 0;
+    int next = 0;
+    while (next < nsave && tsave[next] <= t0) {
+        for (int c = 0; c < n; ++c) out[(int64_t)next * n + c] = u0[c];
+        ++next;
+    }
+    while (next < nsave) {
+        double target = tsave[next];
+        double h = dt;
+        int hit = 0;
+        if (t + h >= target) {
+            h = target - t;
+            hit = 1;
+        }
+        if (++steps > max_steps) return 2;
+        f(u, p, t, k[1]);
+        for (int i = 2; i <= s; ++i) {
+            for (int c = 0; c < n; ++c) {
+                double acc = 0.0;
+                for (int j = 1; j < i; ++j)
+                    if (OV9_AM[i][j]) acc += OV9_A[i][j] * k[j][c];
+                tmp[c] = u[c] + h * acc;
+            }
+            f(tmp, p, t + OV9_C[i] * h, k[i]);
+        }
+        double err = 0.0;
+        for (int c = 0; c < n; ++c) {
+            double acc = 0.0, eacc = 0.0;
+            for (int j = 1; j <= s; ++j) {
+                if (OV9_BM[j]) acc += OV9_B[j] * k[j][c];
+                if (OV9_BTM[j]) eacc += OV9_BT[j] * k[j][c];
+            }
+            un[c] = u[c] + h * acc;
+            double sc = abstol + fmax(fabs(u[c]), fabs(un[c])) * reltol;
+            double e = h * eacc / sc;
+            err += e * e;
+        }
+        double EEst = sqrt(err / n);
+        if (!(EEst == EEst)) return 3;
+        if (EEst > 1.0) {
+            double q11 = pow(EEst, beta1);
+            dt = h / fmin(1.0 / qmin, q11 / gamma);
+            if (dt < 1e-300) return 1;
+            continue;
+        }
+        double q = (EEst == 0.0) ? 1.0 / qmax : pow(EEst, beta1) / pow(qold, beta2);
+        q = fmax(1.0 / qmax, fmin(1.0 / qmin, q / gamma));
+        qold = fmax(EEst, 1e-4);
+        memcpy(u, un, sizeof(double) * n);
+        if (hit) {
+            t = target;
+            for (int c = 0; c < n; ++c) out[(int64_t)next * n + c] = u[c];
+            ++next;
+            /* keep the controller's proposal for a full step, not the clipped one */
+            double prop = h / q;
+            dt = (prop > dt) ? prop : dt;
+        } else {
+            t = t + h;
+            dt = h / q;
+        }
+    }
+    if (nsteps_out) *nsteps_out = steps;
+    return 0;
+}
+
+/* out[(row*n + c)*N + i] in double. tsave (shared, double) may be NULL: then one row at the
+ * per-trajectory final time (tfs[i] or tf). */
+int deg_oracle_vern9(int system, int64_t N, const float *u0, const float *p, const float *t0s,
+                     const float *tfs, double t0, double tf, double abstol, double reltol,
+                     double dt0, const double *tsave, int nsave, double *out, int32_t *status,
+                     int64_t *nsteps, int nthreads) {
+    if (system < 0 || system >= OSYS_COUNT) return -1;
+    const osys *S = &OSYSTEMS[system];
+    orhs_fn_d f = OSYSTEMS_D[system];
+    const int n = S->n;
+    if (nthreads <= 0) nthreads = 1;
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads) reduction(| : bad)
+    for (int64_t i = 0; i < N; ++i) {
+        double uu[OMAXN], pp[OMAXN];
+        double ts1[1];
+        for (int c = 0; c < n; ++c) uu[c] = (double)u0[(int64_t)c * N + i];
+        for (int j = 0; j < S->np; ++j) pp[j] = (double)p[(int64_t)j * N + i];
+        double a = t0s ? (double)t0s[i] : t0;
+        double b = tfs ? (double)tfs[i] : tf;
+        const double *tsv = tsave;
+        int ns = nsave;
+        if (!tsave) {
+            ts1[0] = b;
+            tsv = ts1;
+            ns = 1;
+        }
+        double *row = (double *)malloc(sizeof(double) * (size_t)ns * (size_t)n);
+        if (!row) {
+            bad = 1;
+            continue;
+        }
+        int64_t st = 0;
+        int rc = vern9_traj(f, n, uu, pp, a, abstol, reltol, dt0, tsv, ns, row, (int64_t)1 << 40, &st);
+        if (status) status[i] = rc;
+        if (nsteps) nsteps[i] = st;
+        for (int r = 0; r < ns; ++r)
+            for (int c = 0; c < n; ++c) out[((int64_t)r * n + c) * N + i] = row[r * n + c];
+        free(row);
+    }
+    return bad ? -2 : 0;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Introspection used by the order-condition and helper tests                           */
+/* ------------------------------------------------------------------------------------ */
+float deg_oracle_det_pow(float x, float y) { return det_pow(x, y); }
+
+/* which: 0 Tsit5, 1 Vern7, 2 Vern9. c[18], A[18*18], b[18], bt[18] in double. Returns s. */
+int deg_oracle_tableau(int which, double *c, double *A, double *b, double *bt) {
+    int s;
+    const double *C, *AA, *B, *BT;
+    if (which == 0) { s = OT5_S; C = OT5_C; AA = &OT5_A[0][0]; B = OT5_B; BT = OT5_BT; }
+    else if (which == 1) { s = OV7_S; C = OV7_C; AA = &OV7_A[0][0]; B = OV7_B; BT = OV7_BT; }
+    else if (which == 2) { s = OV9_S; C = OV9_C; AA = &OV9_A[0][0]; B = OV9_B; BT = OV9_BT; }
+    else return -1;
+    for (int i = 0; i <= OMAXS; ++i) {
+        c[i] = 0.0; b[i] = 0.0; bt[i] = 0.0;
+        for (int j = 0; j <= OMAXS; ++j) A[i * (OMAXS + 1) + j] = 0.0;
+    }
+    for (int i = 0; i <= s; ++i) {
+        c[i] = C[i]; b[i] = B[i]; bt[i] = BT[i];
+        for (int j = 0; j <= s; ++j) A[i * (OMAXS + 1) + j] = AA[i * (s + 1) + j];
+    }
+    return s;
+}
+
+/* Tsit5 dense-output weights b_i(theta), i = 1..7, in Float32 as the solver evaluates them. */
+void deg_oracle_tsit5_btheta(float th, float *out7) {
+    init_tabs();
+    out7[0] = th * fmaf(th, fmaf(th, fmaf(th, T5R[1][4], T5R[1][3]), T5R[1][2]), T5R[1][1]);
+    for (int i = 2; i <= 7; ++i) out7[i - 1] = th * (th * fmaf(th, fmaf(th, T5R[i][4], T5R[i][3]), T5R[i][2]));
+}
+double deg_oracle_tsit5_r(int i, int k) { return OT5_R[i][k]; }

@@ -1,0 +1,148 @@
+"""ctypes binding of the oracle library (oracle/_build/libdeg_oracle.so).
+
+TEST INFRASTRUCTURE: used only by tests/, the smoke check and the benchmark's CPU-baseline
+leg, as the checker of the HIP solver. The C source is oracle/deg_oracle.c.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libdeg_oracle.so")
+
+SYSTEM_IDS = {"lorenz": 0, "decay": 1, "growth": 2, "quad": 3, "robertson": 4, "vdp": 5, "tdep": 6, "rot": 7}
+SYSTEM_DIMS = {"lorenz": (3, 3), "decay": (1, 1), "growth": (1, 1), "quad": (1, 1), "robertson": (3, 3),
+               "vdp": (2, 1), "tdep": (1, 1), "rot": (2, 1)}
+METHOD_IDS = {"tsit5": 0, "vern7": 1, "rb23": 2}
+KAHAN, DETPOW = 1, 2
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lp = ctypes.POINTER(ctypes.c_int64)
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.deg_oracle_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _fp, _fp, _fp, _fp,
+                                       ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                       _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                       _fp, _fp, _ip, _ip, _ip, ctypes.c_int]
+        L.deg_oracle_solve.restype = ctypes.c_int
+        L.deg_oracle_vern9.argtypes = [ctypes.c_int, ctypes.c_int64, _fp, _fp, _fp, _fp, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
+                                       ctypes.c_int, _dp, _ip, _lp, ctypes.c_int]
+        L.deg_oracle_vern9.restype = ctypes.c_int
+        L.deg_oracle_det_pow.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.deg_oracle_det_pow.restype = ctypes.c_float
+        L.deg_oracle_tableau.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.deg_oracle_tableau.restype = ctypes.c_int
+        L.deg_oracle_tsit5_btheta.argtypes = [ctypes.c_float, _fp]
+        L.deg_oracle_tsit5_r.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.deg_oracle_tsit5_r.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+def solve(system: str, method: str, adaptive: bool, u0, p, tspan, dt, *, abstol=1e-6, reltol=1e-3, saveat=None,
+          nsave=None, save_everystep=False, t0s=None, tfs=None, kahan=False, detpow=False, max_attempts=0,
+          nthreads=None):
+    """Float32 restatement over an ensemble. u0 (n, N), p (np, N) float32 SoA. Returns dict with
+    ts (nsave, N), us (nsave, n, N), retcode, naccept, nreject. Rows the reference leaves
+    uninitialised are NaN here."""
+    n, np_ = SYSTEM_DIMS[system]
+    u0 = _f32(u0)
+    p = _f32(p) if np_ > 0 else np.zeros((0, u0.shape[1]), np.float32)
+    N = u0.shape[1]
+    sv = _f32(saveat)
+    if nsave is None:
+        nsave = len(sv) if sv is not None else 2
+    ts = np.full((nsave, N), np.float32(tspan[0]), np.float32)
+    us = np.full((nsave, n, N), np.nan, np.float32)
+    rc = np.zeros(N, np.int32)
+    na = np.zeros(N, np.int32)
+    nr = np.zeros(N, np.int32)
+    flags = (KAHAN if kahan else 0) | (DETPOW if detpow else 0)
+    t0s, tfs = _f32(t0s), _f32(tfs)
+    r = lib().deg_oracle_solve(SYSTEM_IDS[system], METHOD_IDS[method], int(adaptive), N, _p(u0, _fp), _p(p, _fp),
+                               _p(t0s, _fp), _p(tfs, _fp), float(np.float32(tspan[0])), float(np.float32(tspan[1])),
+                               float(np.float32(dt)), float(np.float32(abstol)), float(np.float32(reltol)),
+                               _p(sv, _fp), 0 if sv is None else len(sv), int(nsave), int(bool(save_everystep)), flags,
+                               int(max_attempts), _p(ts, _fp), _p(us, _fp), _p(rc, _ip), _p(na, _ip), _p(nr, _ip),
+                               int(nthreads or os.cpu_count() or 1))
+    if r != 0:
+        raise RuntimeError(f"oracle solve failed ({r})")
+    return dict(ts=ts, us=us, retcode=rc, naccept=na, nreject=nr)
+
+
+def vern9(system: str, u0, p, tspan, *, abstol=1e-12, reltol=1e-12, saveat=None, t0s=None, tfs=None, dt0=1e-3,
+          nthreads=None):
+    """FP64 Vern9 reference solution at the saveat times (or the end time). Returns (rows, n, N)."""
+    n, np_ = SYSTEM_DIMS[system]
+    u0 = _f32(u0)
+    p = _f32(p) if np_ > 0 else np.zeros((0, u0.shape[1]), np.float32)
+    N = u0.shape[1]
+    sv = None if saveat is None else np.ascontiguousarray(saveat, dtype=np.float64)
+    rows = 1 if sv is None else len(sv)
+    out = np.zeros((rows, n, N), np.float64)
+    st = np.zeros(N, np.int32)
+    ns = np.zeros(N, np.int64)
+    t0s, tfs = _f32(t0s), _f32(tfs)
+    r = lib().deg_oracle_vern9(SYSTEM_IDS[system], N, _p(u0, _fp), _p(p, _fp), _p(t0s, _fp), _p(tfs, _fp),
+                               float(tspan[0]), float(tspan[1]), abstol, reltol, dt0, _p(sv, _dp),
+                               0 if sv is None else len(sv), _p(out, _dp), _p(st, _ip), _p(ns, _lp),
+                               int(nthreads or os.cpu_count() or 1))
+    if r != 0:
+        raise RuntimeError(f"oracle vern9 failed ({r})")
+    return out, st, ns
+
+
+def det_pow(x: float, y: float) -> float:
+    return float(lib().deg_oracle_det_pow(float(x), float(y)))
+
+
+def tableau(which: str):
+    idx = {"tsit5": 0, "vern7": 1, "vern9": 2}[which]
+    c = np.zeros(18)
+    A = np.zeros(18 * 18)
+    b = np.zeros(18)
+    bt = np.zeros(18)
+    s = lib().deg_oracle_tableau(idx, _p(c, _dp), _p(A, _dp), _p(b, _dp), _p(bt, _dp))
+    A = A.reshape(18, 18)
+    # 1-based stage arrays -> 0-based
+    return c[1:s + 1], A[1:s + 1, 1:s + 1], b[1:s + 1], bt[1:s + 1]
+
+
+def tsit5_btheta(theta: float) -> np.ndarray:
+    out = np.zeros(7, np.float32)
+    lib().deg_oracle_tsit5_btheta(float(np.float32(theta)), _p(out, _fp))
+    return out
+
+
+def tsit5_r():
+    R = np.zeros((8, 5))
+    for i in range(8):
+        for k in range(5):
+            R[i, k] = lib().deg_oracle_tsit5_r(i, k)
+    return R

@@ -1,0 +1,191 @@
+/* Right-hand sides used by the oracle (test infrastructure only).
+ *
+ * Each system is written out-of-place in Float32 with the association order of the
+ * reference test problems, so that the oracle and a kernel built from the same
+ * expression tree perform the same IEEE operations:
+ *   lorenz    test/gpu_kernel_de/gpu_ode_regression.jl:4-12, jac test/lower_level_api.jl:18-35
+ *   decay     test/gpu_kernel_de/stiff_ode/gpu_ode_regression.jl:5-8 (u' = -p u)
+ *   growth    test/public_interface.jl:9-11 (u' = u)
+ *   quad      test/gpu_kernel_de/finite_diff.jl:6-9 (u' = -p u^2)
+ *   robertson test/gpu_kernel_de/stiff_ode/gpu_ode_mass_matrix.jl (ODE form, no mass matrix)
+ *   vdp       Van der Pol, u1' = u2, u2' = mu (1 - u1^2) u2 - u1
+ *   tdep      u' = p t u (non-autonomous, closed form u0 exp(p t^2 / 2))
+ *   rot       u1' = -p u2, u2' = p u1 (closed form rotation)
+ * Jacobians are row-major J[r*n + c] = d f_r / d u_c.
+ */
+#ifndef DEG_ORACLE_SYSTEMS_H
+#define DEG_ORACLE_SYSTEMS_H
+
+enum { OSYS_LORENZ = 0, OSYS_DECAY = 1, OSYS_GROWTH = 2, OSYS_QUAD = 3, OSYS_ROBERTSON = 4,
+       OSYS_VDP = 5, OSYS_TDEP = 6, OSYS_ROT = 7, OSYS_COUNT = 8 };
+
+typedef void (*orhs_fn)(const float *u, const float *p, float t, float *du);
+typedef void (*ojac_fn)(const float *u, const float *p, float t, float *J);
+
+typedef struct {
+    int n, np;
+    orhs_fn f;
+    ojac_fn jac;
+    orhs_fn tgrad;
+} osys;
+
+static void lorenz_f(const float *u, const float *p, float t, float *du) {
+    (void)t;
+    float s = p[0], r = p[1], b = p[2];
+    du[0] = s * (u[1] - u[0]);
+    du[1] = u[0] * (r - u[2]) - u[1];
+    du[2] = u[0] * u[1] - b * u[2];
+}
+static void lorenz_jac(const float *u, const float *p, float t, float *J) {
+    (void)t;
+    float s = p[0], r = p[1], b = p[2];
+    J[0] = -s;       J[1] = s;     J[2] = 0.0f;
+    J[3] = r - u[2]; J[4] = -1.0f; J[5] = -u[0];
+    J[6] = u[1];     J[7] = u[0];  J[8] = -b;
+}
+static void zero_tgrad3(const float *u, const float *p, float t, float *g) {
+    (void)u; (void)p; (void)t;
+    g[0] = 0.0f; g[1] = 0.0f; g[2] = 0.0f;
+}
+static void zero_tgrad2(const float *u, const float *p, float t, float *g) {
+    (void)u; (void)p; (void)t;
+    g[0] = 0.0f; g[1] = 0.0f;
+}
+static void zero_tgrad1(const float *u, const float *p, float t, float *g) {
+    (void)u; (void)p; (void)t;
+    g[0] = 0.0f;
+}
+
+static void decay_f(const float *u, const float *p, float t, float *du) {
+    (void)t;
+    du[0] = -p[0] * u[0];
+}
+static void decay_jac(const float *u, const float *p, float t, float *J) {
+    (void)u; (void)t;
+    J[0] = -p[0];
+}
+
+static void growth_f(const float *u, const float *p, float t, float *du) {
+    (void)p; (void)t;
+    du[0] = u[0];
+}
+static void growth_jac(const float *u, const float *p, float t, float *J) {
+    (void)u; (void)p; (void)t;
+    J[0] = 1.0f;
+}
+
+static void quad_f(const float *u, const float *p, float t, float *du) {
+    (void)t;
+    du[0] = -p[0] * u[0] * u[0];
+}
+static void quad_jac(const float *u, const float *p, float t, float *J) {
+    (void)t;
+    J[0] = -2.0f * p[0] * u[0];
+}
+
+static void robertson_f(const float *u, const float *p, float t, float *du) {
+    (void)t;
+    float k1 = p[0], k2 = p[1], k3 = p[2];
+    du[0] = -k1 * u[0] + k3 * u[1] * u[2];
+    du[1] = k1 * u[0] - k2 * u[1] * u[1] - k3 * u[1] * u[2];
+    du[2] = k2 * u[1] * u[1];
+}
+static void robertson_jac(const float *u, const float *p, float t, float *J) {
+    (void)t;
+    float k1 = p[0], k2 = p[1], k3 = p[2];
+    J[0] = -k1; J[1] = k3 * u[2];                          J[2] = k3 * u[1];
+    J[3] = k1;  J[4] = -2.0f * k2 * u[1] - k3 * u[2];      J[5] = -k3 * u[1];
+    J[6] = 0.0f; J[7] = 2.0f * k2 * u[1];                  J[8] = 0.0f;
+}
+
+static void vdp_f(const float *u, const float *p, float t, float *du) {
+    (void)t;
+    float mu = p[0];
+    du[0] = u[1];
+    du[1] = mu * (1.0f - u[0] * u[0]) * u[1] - u[0];
+}
+static void vdp_jac(const float *u, const float *p, float t, float *J) {
+    (void)t;
+    float mu = p[0];
+    J[0] = 0.0f;                                J[1] = 1.0f;
+    J[2] = -2.0f * mu * u[0] * u[1] - 1.0f;     J[3] = mu * (1.0f - u[0] * u[0]);
+}
+
+static void tdep_f(const float *u, const float *p, float t, float *du) {
+    du[0] = p[0] * t * u[0];
+}
+static void tdep_jac(const float *u, const float *p, float t, float *J) {
+    (void)u;
+    J[0] = p[0] * t;
+}
+static void tdep_tgrad(const float *u, const float *p, float t, float *g) {
+    (void)t;
+    g[0] = p[0] * u[0];
+}
+
+static void rot_f(const float *u, const float *p, float t, float *du) {
+    (void)t;
+    du[0] = -p[0] * u[1];
+    du[1] = p[0] * u[0];
+}
+static void rot_jac(const float *u, const float *p, float t, float *J) {
+    (void)u; (void)t;
+    J[0] = 0.0f;  J[1] = -p[0];
+    J[2] = p[0];  J[3] = 0.0f;
+}
+
+static const osys OSYSTEMS[OSYS_COUNT] = {
+    {3, 3, lorenz_f, lorenz_jac, zero_tgrad3},
+    {1, 1, decay_f, decay_jac, zero_tgrad1},
+    {1, 1, growth_f, growth_jac, zero_tgrad1},
+    {1, 1, quad_f, quad_jac, zero_tgrad1},
+    {3, 3, robertson_f, robertson_jac, zero_tgrad3},
+    {2, 1, vdp_f, vdp_jac, zero_tgrad2},
+    {1, 1, tdep_f, tdep_jac, tdep_tgrad},
+    {2, 1, rot_f, rot_jac, zero_tgrad2},
+};
+
+/* The same right-hand sides in double precision, for the FP64 oracle. */
+static void lorenz_fd(const double *u, const double *p, double t, double *du) {
+    (void)t;
+    du[0] = p[0] * (u[1] - u[0]);
+    du[1] = u[0] * (p[1] - u[2]) - u[1];
+    du[2] = u[0] * u[1] - p[2] * u[2];
+}
+static void decay_fd(const double *u, const double *p, double t, double *du) {
+    (void)t;
+    du[0] = -p[0] * u[0];
+}
+static void growth_fd(const double *u, const double *p, double t, double *du) {
+    (void)p; (void)t;
+    du[0] = u[0];
+}
+static void quad_fd(const double *u, const double *p, double t, double *du) {
+    (void)t;
+    du[0] = -p[0] * u[0] * u[0];
+}
+static void robertson_fd(const double *u, const double *p, double t, double *du) {
+    (void)t;
+    du[0] = -p[0] * u[0] + p[2] * u[1] * u[2];
+    du[1] = p[0] * u[0] - p[1] * u[1] * u[1] - p[2] * u[1] * u[2];
+    du[2] = p[1] * u[1] * u[1];
+}
+static void vdp_fd(const double *u, const double *p, double t, double *du) {
+    (void)t;
+    du[0] = u[1];
+    du[1] = p[0] * (1.0 - u[0] * u[0]) * u[1] - u[0];
+}
+static void tdep_fd(const double *u, const double *p, double t, double *du) {
+    du[0] = p[0] * t * u[0];
+}
+static void rot_fd(const double *u, const double *p, double t, double *du) {
+    (void)t;
+    du[0] = -p[0] * u[1];
+    du[1] = p[0] * u[0];
+}
+typedef void (*orhs_fn_d)(const double *u, const double *p, double t, double *du);
+static const orhs_fn_d OSYSTEMS_D[OSYS_COUNT] = {
+    lorenz_fd, decay_fd, growth_fd, quad_fd, robertson_fd, vdp_fd, tdep_fd, rot_fd,
+};
+
+#endif
