@@ -401,6 +401,13 @@ __device__ __forceinline__ long long deg_queue_next(unsigned int* q) {
     return (long long)(base + rank);
 }
 
+// the next float above x (x finite)
+__device__ __forceinline__ float deg_next_up(float x) {
+    if (x == 0.0f) return 1.4e-45f;
+    const unsigned int b = __builtin_bit_cast(unsigned int, x);
+    return __builtin_bit_cast(float, (x > 0.0f) ? b + 1u : b - 1u);
+}
+
 // t <- t + dt, optionally compensated
 __device__ __forceinline__ float deg_advance(float t, float dt, float& comp) {
 #if DEG_KAHAN
@@ -544,7 +551,16 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
                         t = tf;
                         tc = 0.0f;
                     } else {
-                        t = deg_advance(t, dt, tc);
+                        float tn = deg_advance(t, dt, tc);
+#if !DEG_KAHAN
+                        // A step below the resolution of t would repeat forever: snap to tf
+                        // when it is within two ulps, otherwise stop with DtLessThanMin.
+                        if (tn == t) {
+                            if (tf - t <= 2.0f * (deg_next_up(t) - t)) tn = tf;
+                            else status = DEG_RET_DTMIN;
+                        }
+#endif
+                        t = tn;
                     }
                     dt = dtnew;
                     ++nacc;
@@ -553,10 +569,21 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
 #else
                 // ---- one fixed step (gpu_*_perform_step.jl fixed versions)
                 const float told = t;
+                const float tcomp_old = tc;
+                const float tn = deg_advance(t, dt, tc);
+#if !DEG_KAHAN
+                if (tn == told) {
+                    // the step is below the resolution of t: stop instead of looping forever
+                    status = DEG_RET_DTMIN;
+                    tc = tcomp_old;
+                    --attempts;
+                } else
+#endif
+                {
                 DEG_FOR(c) uprev[c] = u[c];
                 tprev = told;
                 dtacc = dt;
-                t = deg_advance(t, dt, tc);
+                t = tn;
 #if DEG_METHOD == 0
                 if (!have_k) {
                     deg_rhs(uprev, p, told, k[1]);
@@ -574,6 +601,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
                 DEG_FOR(c) u[c] = unew[c];
                 ++nacc;
                 accepted = true;
+                }
 #endif
             }
             if (accepted) {

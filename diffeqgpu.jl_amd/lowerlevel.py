@@ -111,7 +111,8 @@ _AUTOTUNE_BLOCK = {}
 
 
 def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, abstol, reltol, saveat_np,
-            nsave, save_everystep, t0_fill, timing=False, stream=None, mlp=None, steplog=None):
+            nsave, save_everystep, t0_fill, timing=False, stream=None, mlp=None, steplog=None, block_size=None,
+            grid_blocks=0):
     import torch
 
     if not isinstance(alg, GPUODEAlgorithm):
@@ -154,7 +155,8 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
     a.nsaveat = int(sv.numel()) if sv is not None else 0
     a.nsave = int(nsave)
     a.save_everystep = int(bool(save_everystep))
-    a.block_size = int(dev.block_size or _AUTOTUNE_BLOCK.get((alg.method, flags, n), 0))
+    a.block_size = int(block_size or dev.block_size or _AUTOTUNE_BLOCK.get((alg.method, flags, n), 0))
+    a.grid_blocks = int(grid_blocks)
     a.ts = ts.data_ptr()
     a.us = us_soa.data_ptr()
     a.retcode = retcode.data_ptr()
@@ -176,12 +178,17 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
         a.steplog = buf.data_ptr()
         a.steplog_count = counter.data_ptr()
         a.steplog_cap = int(buf.shape[0])
-    # keep the host-side tensors alive across the asynchronous launch
     kern.solve(a)
     if timing:
         ms = float(box.value)
+
+    def relaunch():
+        # same arguments, same output buffers: recomputes and overwrites every output
+        kern.solve(a)
+
+    # the result keeps the argument block and every tensor it points to alive
     return SolveResult(ts, us_soa.permute(0, 2, 1), retcode=retcode, naccept=nacc, nreject=nrej, us_soa=us_soa,
-                       kernel_ms=ms, saveat=sv, kernel=kern, flags=flags)
+                       kernel_ms=ms, saveat=sv, kernel=kern, flags=flags, relaunch=relaunch, args=a)
 
 
 def vectorized_solve(probs, prob, alg, *, dt, saveat=None, save_everystep=True, debug=False, callback=None,

@@ -19,6 +19,9 @@
  *   - a lane that would hit the reference's in-kernel error("dt<dtmin") stops with a
  *     per-trajectory return code instead of aborting the launch;
  *   - a per-trajectory attempt cap (max_attempts) ends a lane that would loop forever;
+ *   - an accepted step that cannot advance t (t + dt rounds to t) snaps t to tf when tf is
+ *     within two ulps of t and otherwise ends the lane with DtLessThanMin (the reference
+ *     repeats such a step forever);
  *   - Rosenbrock23 forms W = I - gamma J and solves it in FP64 (LU with partial
  *     pivoting) instead of Float32 Cramer's rule (linalg/linsolve.jl:31-57);
  *   - optional compensated (Kahan) accumulation of t (flag DEG_O_KAHAN).
@@ -410,6 +413,29 @@ static inline float advance_t(float t, float dt, float *comp, int kahan) {
     return tn;
 }
 
+/* the next float above x (x finite) */
+static inline float next_up(float x) {
+    if (x == 0.0f) return 1.4e-45f;
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    b = (x > 0.0f) ? b + 1u : b - 1u;
+    float r;
+    memcpy(&r, &b, 4);
+    return r;
+}
+
+/* An accepted step that leaves t unchanged (t + dt rounds to t) would repeat forever in the
+ * reference's loop. Here the lane snaps to tf when tf is within two units of t's resolution,
+ * and otherwise stops with DtLessThanMin. Returns 1 when the lane must stop. */
+static inline int no_progress(float *t, float tf) {
+    float ulp = next_up(*t) - *t;
+    if (tf - *t <= 2.0f * ulp) {
+        *t = tf;
+        return 0;
+    }
+    return 1;
+}
+
 static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p, float t0, float tf,
                       int32_t *nacc_out, int32_t *nrej_out) {
     const osys *S = J->S;
@@ -453,6 +479,11 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
             tprev = t;
             float told = t;
             t = advance_t(t, dt, &tcomp, J->kahan);
+            if (t == told && !J->kahan) {
+                retcode = ORET_DTMIN; /* the fixed step is below the resolution of t */
+                --attempts;
+                break;
+            }
             if (J->method == OM_RB23) {
                 /* gpu_rosenbrock23_perform_step.jl:1-70 */
                 float Jm[OMAXN * OMAXN], dT[OMAXN], F0[OMAXN], rhs[OMAXN], tmp[OMAXN], F1[OMAXN], x[OMAXN];
@@ -601,11 +632,17 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                         t = tf;
                         tcomp = 0.0f;
                     } else {
-                        t = advance_t(t, dt, &tcomp, J->kahan);
+                        float tn = advance_t(t, dt, &tcomp, J->kahan);
+                        if (tn == t && !J->kahan && no_progress(&tn, tf)) {
+                            retcode = ORET_DTMIN;
+                            stop = 1;
+                        }
+                        t = tn;
                     }
                     dt = dtnew;
                     ++nacc;
                     have_k7 = 1;
+                    if (stop) break;
                 }
             }
             if (stop) break;

@@ -322,3 +322,23 @@ def test_kahan_time_accumulation():
     err_kahan = abs(float(rk["ts"][1, 0]) - exact)
     assert err_kahan <= float(np.spacing(F32(exact)))
     assert err_plain > 100 * err_kahan + 1e-5
+
+
+def test_no_progress_rule_ends_every_lane():
+    """Lanes whose remaining span falls below the resolution of t (the reference repeats such a
+    step forever) snap to tf: the uneven Lorenz workload finishes every trajectory at its tf."""
+    import __graft_entry__ as g
+
+    g.load_package()
+    from diffeqgpu_amd import workloads
+
+    cfg = workloads.config("lorenz_uneven", N=4000, seed=5)
+    o = pyoracle.solve("lorenz", "vern7", True, cfg["u0"], cfg["p"], (0.0, 50.0), 0.01, tfs=cfg["tfs"],
+                       abstol=1e-6, reltol=1e-6, detpow=True, max_attempts=100000)
+    assert np.all(o["retcode"] == 0)
+    np.testing.assert_array_equal(o["ts"][1], cfg["tfs"])
+    assert (o["naccept"] + o["nreject"]).max() < 10000
+    # a fixed step below the resolution of t stops with DtLessThanMin instead of looping
+    r = pyoracle.solve("growth", "tsit5", False, np.array([[1.0]], F32), np.array([[1.0]], F32), (1e6, 1e6 + 1.0),
+                       1e-3)
+    assert r["retcode"][0] == 1 and r["naccept"][0] == 0
