@@ -181,6 +181,7 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
     kern.solve(a)
     if timing:
         ms = float(box.value)
+        a.elapsed_ms = None  # the box does not outlive this call
 
     def relaunch():
         # same arguments, same output buffers: recomputes and overwrites every output
@@ -188,7 +189,8 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
 
     # the result keeps the argument block and every tensor it points to alive
     return SolveResult(ts, us_soa.permute(0, 2, 1), retcode=retcode, naccept=nacc, nreject=nrej, us_soa=us_soa,
-                       kernel_ms=ms, saveat=sv, kernel=kern, flags=flags, relaunch=relaunch, args=a)
+                       kernel_ms=ms, saveat=sv, kernel=kern, flags=flags, relaunch=relaunch, args=a,
+                       probs=probs)
 
 
 def vectorized_solve(probs, prob, alg, *, dt, saveat=None, save_everystep=True, debug=False, callback=None,
