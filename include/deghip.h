@@ -83,7 +83,7 @@ typedef struct deg_solve_args {
     int32_t *naccept;        /* [stride] or NULL: accepted steps */
     int32_t *nreject;        /* [stride] or NULL: rejected attempts */
     int64_t max_attempts;    /* per-trajectory attempt cap; 0: 10^7 (ends lanes whose t stops advancing) */
-    void *stream;            /* hipStream_t; NULL: the context's stream */
+    void *stream;            /* hipStream_t to launch on; NULL: the null (default) stream */
     int32_t grid_blocks;     /* persistent kernels: 0 fills the device */
     int32_t pad0;
     float *elapsed_ms;       /* optional: kernel time from HIP events around the launch (synchronises) */
@@ -93,7 +93,7 @@ typedef struct deg_solve_args {
     int64_t steplog_cap;
 } deg_solve_args;
 
-/* Context: one device, one default stream, the JIT cache directory. */
+/* Context: one device (launches go to the stream given per call). */
 int deg_ctx_create(int device, deg_ctx **out);
 int deg_ctx_destroy(deg_ctx *ctx);
 /* Number of compute units of the context's device. */
@@ -122,12 +122,12 @@ int deg_jit_precompile(const char *rhs_src, int n, int np, int method, uint32_t 
  * Returns the length; copies at most cap-1 bytes plus a terminator into buf when non-NULL. */
 int64_t deg_jit_source(const char *rhs_src, int n, int np, int method, uint32_t flags, char *buf, int64_t cap);
 
-/* Launch one ensemble solve (asynchronous on the stream unless elapsed_ms is set).
+/* Launch one ensemble solve on args->stream (asynchronous unless elapsed_ms is set).
  * Replaces the kernel launch of vectorized_solve / vectorized_asolve
  * (lowerlevel_solve.jl:119-123, :339-343). */
 int deg_solve(deg_kernel *k, const deg_solve_args *args);
 
-/* Wait for the context's stream. */
+/* Wait for all work on the context's device. */
 int deg_sync(deg_ctx *ctx);
 
 /* Last error message of the calling thread ("" when none). */
