@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--contract", default="fast", choices=["fast", "off"])
     ap.add_argument("--persistent", type=int, default=1)
     ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -108,7 +109,7 @@ def main():
     N = cfg["N"]
     f, n, np_, _ = systems.SYSTEMS["lorenz"]
     dev = deg.HIPDevice(0, contract=args.contract, powmode=args.powmode, persistent=bool(args.persistent),
-                        block_size=args.block_size or None)
+                        block_size=args.block_size or None, refill=args.refill)
     probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev)
     prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
     alg = deg.GPUTsit5()

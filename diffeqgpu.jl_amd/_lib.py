@@ -24,6 +24,8 @@ F_PERSISTENT = 1 << 4
 F_LEARNED = 1 << 5
 F_FASTPOW = 1 << 6
 F_STEPLOG = 1 << 7
+F_SAVEAT_GLOBAL = 1 << 8
+SAVEAT_LDS_CAP = 4096
 
 RET_SUCCESS, RET_DTMIN, RET_MAXITERS, RET_NONFINITE = 0, 1, 2, 3
 RETCODE_NAMES = {RET_SUCCESS: "Success", RET_DTMIN: "DtLessThanMin", RET_MAXITERS: "MaxIters",
@@ -65,12 +67,14 @@ class SolveArgs(ctypes.Structure):
         ("max_attempts", ctypes.c_int64),
         ("stream", ctypes.c_void_p),
         ("grid_blocks", ctypes.c_int32),
-        ("pad0", ctypes.c_int32),
+        ("refill_batch", ctypes.c_int32),
         ("elapsed_ms", ctypes.c_void_p),
         ("mlp", ctypes.c_void_p),
         ("steplog", ctypes.c_void_p),
         ("steplog_count", ctypes.c_void_p),
         ("steplog_cap", ctypes.c_int64),
+        ("chunk", ctypes.c_int32),
+        ("pad1", ctypes.c_int32),
     ]
 
 

@@ -78,6 +78,9 @@ class HIPDevice:
       block_size  threads per block (64..256); None: 256 or the autotuned value.
       max_attempts per-trajectory attempt cap (0: 10^7). The reference has none and loops forever
                   when t + dt rounds to t; here such a lane ends with retcode MaxIters.
+      refill      persistent grids: a wave takes new trajectories once this many of its lanes
+                  are idle (1..64; 0: 8).
+      chunk       persistent grids: trajectory ids a wave claims per queue atomic (0: 64).
     """
 
     device: int = 0
@@ -87,6 +90,8 @@ class HIPDevice:
     kahan: bool = False
     block_size: int | None = None
     max_attempts: int = 0
+    refill: int = 0
+    chunk: int = 0
 
     def flags(self, adaptive: bool) -> int:
         f = 0

@@ -27,5 +27,7 @@ struct deg_kargs {
     int nsaveat;
     int nsave;
     int save_everystep;
-    int pad0;
+    int refill;               // persistent grids: idle lanes per wave that trigger a refill (1..64)
+    int chunk;                // persistent grids: ids claimed per queue atomic
+    int pad1;
 };

@@ -17,7 +17,8 @@
 //   DEG_FASTPOW                  controller powers via hardware log2/exp2
 //   DEG_KAHAN                    compensated accumulation of t
 //   DEG_PERSISTENT               lanes refill from a device work counter
-//   DEG_LEARNED                  MLP step-size proposal (see deg_mlp.h)
+//   DEG_LEARNED                  MLP step-size proposal
+//   DEG_SAVEAT_GLOBAL            read saveat times from global memory (more than DEG_SAVEAT_LDS)
 //   DEG_KERNEL                   kernel symbol name
 //   DEG_BLOCK_MAX                launch-bounds block size
 // and the functions deg_rhs(u, p, t, du) (+ deg_jac, deg_tgrad for Rosenbrock23).
@@ -109,6 +110,19 @@ __device__ __forceinline__ float scaled_error(float (&e)[DEG_N], const float (&u
     return rms_norm(e);
 }
 
+// Fast error norm (DEG_FASTPOW): the squared norm E2 = EEst^2 with reciprocal-multiply
+// scaling. The controller then works on log2(E2) (see deg_fast_control).
+__device__ __forceinline__ float scaled_error2_fast(float (&e)[DEG_N], const float (&uprev)[DEG_N],
+                                                    const float (&u)[DEG_N], float abstol, float reltol) {
+    float acc = 0.0f;
+    DEG_FOR(c) {
+        const float sc = abstol + __builtin_fmaxf(__builtin_fabsf(uprev[c]), __builtin_fabsf(u[c])) * reltol;
+        const float r = e[c] * __builtin_amdgcn_rcpf(sc);
+        acc = acc + r * r;
+    }
+    return acc * (1.0f / (float)DEG_N);
+}
+
 // ------------------------------------------------------------------------------------
 // PI controller (integrator_utils.jl:1-11; gpu_tsit5_perform_step.jl:124-137)
 // ------------------------------------------------------------------------------------
@@ -156,7 +170,11 @@ __device__ __forceinline__ float deg_error(float dt, const float (&k)[DEG_NK + 1
     float e[DEG_N];
     DEG_FOR(c) e[c] = dt * (T5_bt1 * k[1][c] + T5_bt2 * k[2][c] + T5_bt3 * k[3][c] + T5_bt4 * k[4][c] +
                             T5_bt5 * k[5][c] + T5_bt6 * k[6][c] + T5_bt7 * k[7][c]);
+#if DEG_FASTPOW
+    return scaled_error2_fast(e, uprev, unew, abstol, reltol);
+#else
     return scaled_error(e, uprev, unew, abstol, reltol);
+#endif
 }
 // y0 + dt * (b1(th) k1 + ... + b7(th) k7) with b_i evaluated by fused Horner steps
 __device__ __forceinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
@@ -216,7 +234,11 @@ __device__ __forceinline__ float deg_error(float dt, const float (&k)[DEG_NK + 1
     DEG_FOR(c) e[c] = dt * (V7_btilde1 * k[1][c] + V7_btilde4 * k[4][c] + V7_btilde5 * k[5][c] + V7_btilde6 * k[6][c] +
                             V7_btilde7 * k[7][c] + V7_btilde8 * k[8][c] + V7_btilde9 * k[9][c] +
                             V7_btilde10 * k[10][c]);
+#if DEG_FASTPOW
+    return scaled_error2_fast(e, uprev, unew, abstol, reltol);
+#else
     return scaled_error(e, uprev, unew, abstol, reltol);
+#endif
 }
 // Vern7 dense output (interpolants.jl:28-190): extra stages k11..k16 from uprev, then
 // y0 + th*step_sum + th*(th-1)*correction with q_i(th) from the synthetic division of the
@@ -369,7 +391,11 @@ __device__ __forceinline__ float deg_rb23(const float (&uprev)[DEG_N], const flo
     DEG_FOR(c) rhs[c] = ((F2[c] - e32 * (k[2][c] - F1[c])) - 2.0f * (k[1][c] - F0[c])) + dt * dT[c];
     w_solve(W, piv, rhs, k3);
     DEG_FOR(c) e[c] = dto6 * ((k[1][c] - 2.0f * k[2][c]) + k3[c]);
+#if DEG_FASTPOW
+    return scaled_error2_fast(e, uprev, unew, abstol, reltol);
+#else
     return scaled_error(e, uprev, unew, abstol, reltol);
+#endif
 }
 // y0 + dt * (c1 k1 + c2 k2), c1 = th(1-th)/(1-2d), c2 = th(th-2d)/(1-2d)
 __device__ __forceinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
@@ -390,15 +416,16 @@ __device__ __forceinline__ void deg_interp(float th, float dt, float tprev, cons
 // Persistent work queue: lanes that need a trajectory take ids from one device counter,
 // one atomic per wave (ballot + popcount + prefix rank).
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ long long deg_queue_next(unsigned int* q) {
-    const unsigned long long mask = __ballot(1);
-    const int lane = (int)__lane_id();
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    unsigned int base = 0;
-    if (lane == leader) base = atomicAdd(q, (unsigned int)__popcll(mask));
-    base = __shfl(base, leader);
-    const unsigned int rank = (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
-    return (long long)(base + rank);
+// Persistent scheduling: DEG_NQUEUE counters, each owning a slab of trajectory ids, spaced
+// DEG_QSTRIDE words apart (separate memory channels). A wave claims chunks of ids from
+// its counter with one atomic (issued by one lane) and hands them to its idle lanes by
+// ballot rank; when its slab is exhausted it moves on to the next counter.
+#define DEG_NQUEUE 8
+#define DEG_QSTRIDE 64
+__device__ __forceinline__ unsigned int deg_claim(unsigned int* q, int leader, int lane, unsigned int amount) {
+    unsigned int r = 0;
+    if (lane == leader) r = atomicAdd(q, amount);
+    return __shfl(r, leader);
 }
 
 // the next float above x (x finite)
@@ -406,6 +433,28 @@ __device__ __forceinline__ float deg_next_up(float x) {
     if (x == 0.0f) return 1.4e-45f;
     const unsigned int b = __builtin_bit_cast(unsigned int, x);
     return __builtin_bit_cast(float, (x > 0.0f) ? b + 1u : b - 1u);
+}
+
+// PI controller on the squared error norm in log2 space (DEG_FASTPOW): the reference's
+// q11 = EEst^beta1, q = q11 / qold^beta2, reject dt /= min(1/qmin, q11/gamma), accept
+// dt /= clamp(q/gamma, 1/qmax, 1/qmin) become one log2 and one exp2:
+//   reject: dt * max(qmin, gamma * 2^(-beta1/2 * L))
+//   accept: dt * clamp(gamma * 2^(beta2 * Lq - beta1/2 * L), qmin, qmax), Lq = log2 qold
+// with L = log2(EEst^2); EEst == 0 gives the reference's factor 1/((1/qmax)/gamma).
+// Returns the step factor; lq carries log2(qold) and is updated on acceptance.
+__device__ __forceinline__ float deg_fast_control(float E2, bool accept, float& lq) {
+    const float L = __builtin_amdgcn_logf(E2);
+    const float x = (accept ? DEG_BETA2 * lq : 0.0f) - (0.5f * DEG_BETA1) * L;
+    const float f = DEG_GAMMA * __builtin_amdgcn_exp2f(x);
+    float factor;
+    if (accept) {
+        factor = __builtin_fminf(__builtin_fmaxf(f, DEG_QMIN), DEG_QMAX);
+        if (E2 == 0.0f) factor = 1.0f / __builtin_fmaxf(1.0f / DEG_QMAX, __builtin_fminf(1.0f / DEG_QMIN, (1.0f / DEG_QMAX) / DEG_GAMMA));
+        lq = __builtin_fmaxf(0.5f * L, -13.2877123795f);  // log2(max(EEst, 1e-4))
+    } else {
+        factor = __builtin_fmaxf(f, DEG_QMIN);
+    }
+    return factor;
 }
 
 // t <- t + dt, optionally compensated
@@ -429,14 +478,32 @@ __device__ __forceinline__ float deg_advance(float t, float dt, float& comp) {
 // The ensemble kernel (kernels.jl:1-152), one trajectory per lane, one attempt per
 // loop iteration so that lanes stay in step and a finished lane can be refilled.
 // ------------------------------------------------------------------------------------
+#ifndef DEG_SAVEAT_LDS
+#define DEG_SAVEAT_LDS 4096
+#endif
+#ifndef DEG_SAVEAT_GLOBAL
+#define DEG_SAVEAT_GLOBAL 0
+#endif
+
 extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg_kargs a) {
     const long long S = a.stride;
+    // saveat times staged in LDS: reading them then needs no vector-memory wait (which on
+    // CDNA would also wait for every store still in flight)
+#if DEG_SAVEAT_GLOBAL
+    const float* __restrict__ svp = a.saveat;  // more save points than the LDS cache holds
+#else
+    __shared__ float svp[DEG_SAVEAT_LDS];
+    if (a.saveat != nullptr) {
+        for (int j = threadIdx.x; j < a.nsaveat; j += blockDim.x) svp[j] = a.saveat[j];
+    }
+    __syncthreads();
+#endif
 #if DEG_PERSISTENT
-    long long gid = deg_queue_next(a.queue);
+    long long gid = 0;
 #else
     long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-#endif
     if (gid >= a.n_traj) return;
+#endif
     const bool use_saveat = a.saveat != nullptr;
     const float finf = __builtin_inff();
 
@@ -450,9 +517,64 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
     long long attempts = 0;
     bool have_k = false;
     bool fsal_pending = false;
+#if DEG_PERSISTENT
+    bool need_init = false;
+    bool have = false;  // the lane holds a trajectory
+    bool want = true;   // the lane waits for one
+    // wave-uniform scheduler state
+    const long long slab = (a.n_traj + DEG_NQUEUE - 1) / DEG_NQUEUE;
+    int qi = blockIdx.x % DEG_NQUEUE;   // blocks are dispatched round-robin over the XCDs
+    int qtries = 0;
+    long long c_next = 0, c_end = 0;    // the wave's claimed chunk [c_next, c_end)
+    const int lane = (int)__lane_id();
+#else
     bool need_init = true;
+#endif
 
     for (;;) {
+#if DEG_PERSISTENT
+        {
+            // Finished lanes wait until `refill` lanes of the wave are idle (or none is
+            // running) and then take new trajectories together: one queue atomic and one
+            // load latency per batch instead of per trajectory.
+            const unsigned long long wmask = __ballot(want);
+            const unsigned long long hmask = __ballot(have);
+            if (wmask != 0ull && (hmask == 0ull || __popcll(wmask) >= (unsigned long long)a.refill)) {
+                // hand ids of the wave's chunk to the waiting lanes in rank order, claiming
+                // further chunks (and moving on to other counters) as needed
+                const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
+                unsigned long long pending = wmask;
+                while (pending != 0ull) {
+                    if (c_next >= c_end) {
+                        if (qtries >= DEG_NQUEUE) break;
+                        const long long lim = min((long long)(qi + 1) * slab, a.n_traj);
+                        const long long base = (long long)qi * slab +
+                            (long long)deg_claim(a.queue + qi * DEG_QSTRIDE, leader, lane, (unsigned int)a.chunk);
+                        if (base >= lim) {
+                            qi = (qi + 1) % DEG_NQUEUE;
+                            ++qtries;
+                            continue;
+                        }
+                        c_next = base;
+                        c_end = min(base + (long long)a.chunk, lim);
+                    }
+                    const long long avail = c_end - c_next;
+                    const bool mine = (pending >> lane) & 1ull;
+                    const long long rank = (long long)__popcll(pending & ((1ull << lane) - 1ull));
+                    if (mine && rank < avail) {
+                        gid = c_next + rank;
+                        want = false;
+                        need_init = true;
+                    }
+                    const long long np_ = (long long)__popcll(pending);
+                    c_next += (np_ < avail) ? np_ : avail;
+                    pending = __ballot(want) & pending;
+                }
+                if (want) break;  // every slab is exhausted: this lane retires
+            }
+            if (!have && !need_init) continue;
+        }
+#endif
         if (need_init) {
             // ---- init (nonstiff/types.jl:305-402, stiff/types.jl:32-177; kernels.jl:35-45, :116-126)
             need_init = false;
@@ -466,7 +588,9 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
             dtacc = dt;
             tc = 0.0f;
             qold = DEG_QOLDINIT;
-#if DEG_ADAPTIVE
+#if DEG_ADAPTIVE && DEG_FASTPOW
+            qpow = -13.2877123795f;  // log2(qoldinit)
+#elif DEG_ADAPTIVE
             qpow = ctl_pow(qold, DEG_BETA2);
 #endif
             nacc = 0;
@@ -478,19 +602,22 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
             DEG_FOR(c) uprev[c] = u[c];
             if (use_saveat) {
                 cur = 0;
-                if (a.nsaveat > 0 && t == a.saveat[0]) {
+                if (a.nsaveat > 0 && t == svp[0]) {
                     if (a.nsave > 0) {
                         DEG_FOR(c) a.us[(long long)c * S + i] = u[c];
                         a.ts[i] = a.t0;
                     }
                     cur = 1;
                 }
-                tnext = (cur < a.nsaveat) ? a.saveat[cur] : finf;
+                tnext = (cur < a.nsaveat) ? svp[cur] : finf;
             } else {
                 a.ts[i] = t;
                 DEG_FOR(c) a.us[(long long)c * S + i] = u[c];
                 step_idx = 1;
             }
+#if DEG_PERSISTENT
+            have = true;
+#endif
         }
 
         bool done = !(t < tf);
@@ -525,6 +652,18 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
 #else
                 float EEst = deg_rb23<true>(u, p, t, dt, k, unew, a.abstol, a.reltol);
 #endif
+#if DEG_FASTPOW
+                // EEst holds EEst^2 here (scaled_error2_fast)
+                if (EEst != EEst) status = DEG_RET_NONFINITE;
+                const bool acc_step = !(EEst > 1.0f);
+                const float factor = deg_fast_control(EEst, acc_step, qpow);
+                if (!acc_step) {
+                    dt = dt * factor;
+                    ++nrej;
+                } else {
+                    float dtnew = dt * factor;
+                    dtnew = __builtin_fminf(__builtin_fabsf(dtnew), __builtin_fabsf((tf - t) - dt));
+#else
                 float q11 = 0.0f, q;
                 if (EEst == 0.0f) {
                     q = 1.0f / DEG_QMAX;
@@ -541,6 +680,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
                     qpow = ctl_pow(qold, DEG_BETA2);
                     float dtnew = dt / q;
                     dtnew = jl_min(__builtin_fabsf(dtnew), __builtin_fabsf((tf - t) - dt));
+#endif
                     dtacc = dt;
                     tprev = t;
                     DEG_FOR(c) {
@@ -617,7 +757,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
                             a.ts[(long long)cur * S + i] = tnext;
                         }
                         ++cur;
-                        tnext = (cur < a.nsaveat) ? a.saveat[cur] : finf;
+                        tnext = (cur < a.nsaveat) ? svp[cur] : finf;
                     }
 #endif
                 } else if (a.save_everystep) {
@@ -664,9 +804,8 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
             if (a.naccept) a.naccept[i] = nacc;
             if (a.nreject) a.nreject[i] = nrej;
 #if DEG_PERSISTENT
-            gid = deg_queue_next(a.queue);
-            if (gid >= a.n_traj) break;
-            need_init = true;
+            have = false;
+            want = true;
 #else
             break;
 #endif
@@ -677,5 +816,5 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
 // Clears the persistent work counter before a launch (a kernel, not a memset, so the
 // sequence can be captured in a graph).
 extern "C" __global__ void DEG_QUEUE_RESET(unsigned int* q) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) q[0] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < DEG_NQUEUE) q[threadIdx.x * DEG_QSTRIDE] = 0u;
 }

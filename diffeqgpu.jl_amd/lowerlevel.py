@@ -123,6 +123,8 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
         flags |= _lib.F_LEARNED
     if steplog is not None:
         flags |= _lib.F_STEPLOG
+    if saveat_np is not None and len(saveat_np) > _lib.SAVEAT_LDS_CAP:
+        flags |= _lib.F_SAVEAT_GLOBAL
     with_jac = isinstance(alg, GPURosenbrock23)
     src, _flops = probs.f.hip_source(probs.n, probs.np, with_jac)
     ctx = _lib.context(dev.device)
@@ -163,6 +165,8 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
     a.naccept = nacc.data_ptr()
     a.nreject = nrej.data_ptr()
     a.max_attempts = int(dev.max_attempts)
+    a.refill_batch = int(dev.refill)
+    a.chunk = int(dev.chunk)
     st = stream if stream is not None else torch.cuda.current_stream(device)
     a.stream = st.cuda_stream
     ms = None
