@@ -79,8 +79,8 @@ class HIPDevice:
       max_attempts per-trajectory attempt cap (0: 10^7). The reference has none and loops forever
                   when t + dt rounds to t; here such a lane ends with retcode MaxIters.
       refill      persistent grids: a wave takes new trajectories once this many of its lanes
-                  are idle (1..64; 0: 8).
-      chunk       persistent grids: trajectory ids a wave claims per queue atomic (0: 64).
+                  are idle (1..64; 0: 4).
+      chunk       persistent grids: trajectory ids a wave claims per queue atomic (0: 16).
     """
 
     device: int = 0

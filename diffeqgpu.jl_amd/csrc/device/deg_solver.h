@@ -533,6 +533,10 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
 
     for (;;) {
 #if DEG_PERSISTENT
+        // Scheduling block, executed by every live lane of the wave. It is written without
+        // continue/break inside so that the wave-uniform chunk state (c_next, c_end, qi,
+        // qtries) is updated by all live lanes together.
+        bool retire = false;
         {
             // Finished lanes wait until `refill` lanes of the wave are idle (or none is
             // running) and then take new trajectories together: one queue atomic and one
@@ -544,36 +548,36 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
                 // further chunks (and moving on to other counters) as needed
                 const int leader = __ffsll((unsigned long long)__ballot(1)) - 1;
                 unsigned long long pending = wmask;
-                while (pending != 0ull) {
+                while (pending != 0ull && qtries < DEG_NQUEUE) {
                     if (c_next >= c_end) {
-                        if (qtries >= DEG_NQUEUE) break;
                         const long long lim = min((long long)(qi + 1) * slab, a.n_traj);
                         const long long base = (long long)qi * slab +
                             (long long)deg_claim(a.queue + qi * DEG_QSTRIDE, leader, lane, (unsigned int)a.chunk);
                         if (base >= lim) {
                             qi = (qi + 1) % DEG_NQUEUE;
                             ++qtries;
-                            continue;
+                        } else {
+                            c_next = base;
+                            c_end = min(base + (long long)a.chunk, lim);
                         }
-                        c_next = base;
-                        c_end = min(base + (long long)a.chunk, lim);
+                    } else {
+                        const long long avail = c_end - c_next;
+                        const bool mine = (pending >> lane) & 1ull;
+                        const long long rank = (long long)__popcll(pending & ((1ull << lane) - 1ull));
+                        if (mine && rank < avail) {
+                            gid = c_next + rank;
+                            want = false;
+                            need_init = true;
+                        }
+                        const long long np_ = (long long)__popcll(pending);
+                        c_next += (np_ < avail) ? np_ : avail;
+                        pending = __ballot(want) & pending;
                     }
-                    const long long avail = c_end - c_next;
-                    const bool mine = (pending >> lane) & 1ull;
-                    const long long rank = (long long)__popcll(pending & ((1ull << lane) - 1ull));
-                    if (mine && rank < avail) {
-                        gid = c_next + rank;
-                        want = false;
-                        need_init = true;
-                    }
-                    const long long np_ = (long long)__popcll(pending);
-                    c_next += (np_ < avail) ? np_ : avail;
-                    pending = __ballot(want) & pending;
                 }
-                if (want) break;  // every slab is exhausted: this lane retires
+                retire = want;  // every slab is exhausted
             }
-            if (!have && !need_init) continue;
         }
+        if (retire) break;
 #endif
         if (need_init) {
             // ---- init (nonstiff/types.jl:305-402, stiff/types.jl:32-177; kernels.jl:35-45, :116-126)
@@ -620,6 +624,11 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
 #endif
         }
 
+#if DEG_PERSISTENT
+        if (have) {
+#else
+        {
+#endif
         bool done = !(t < tf);
         if (!done) {
             bool accepted = false;
@@ -810,6 +819,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
             break;
 #endif
         }
+        }  // lanes holding a trajectory
     }
 }
 

@@ -86,13 +86,13 @@ typedef struct deg_solve_args {
     int64_t max_attempts;    /* per-trajectory attempt cap; 0: 10^7 (ends lanes whose t stops advancing) */
     void *stream;            /* hipStream_t to launch on; NULL: the null (default) stream */
     int32_t grid_blocks;     /* persistent kernels: 0 fills the device */
-    int32_t refill_batch;    /* persistent kernels: a wave refills once this many lanes are idle (1..64); 0: 8 */
+    int32_t refill_batch;    /* persistent kernels: a wave refills once this many lanes are idle (1..64); 0: 4 */
     float *elapsed_ms;       /* optional: kernel time from HIP events around the launch (synchronises) */
     const void *mlp;         /* DEG_F_LEARNED: packed proposer weights (device), see deg_mlp_* */
     float *steplog;          /* DEG_F_STEPLOG: [steplog_cap][DEG_STEPLOG_COLS] rows (device) */
     uint32_t *steplog_count; /* DEG_F_STEPLOG: device row counter (zeroed by the caller) */
     int64_t steplog_cap;
-    int32_t chunk;           /* persistent kernels: trajectory ids a wave claims per queue atomic; 0: 64 */
+    int32_t chunk;           /* persistent kernels: trajectory ids a wave claims per queue atomic; 0: 16 */
     int32_t pad1;
 } deg_solve_args;
 
