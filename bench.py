@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n-traj", type=int, default=1_000_000)
-    ap.add_argument("--powmode", default="ocml", choices=["ocml", "fast", "det"])
+    ap.add_argument("--powmode", default="fast", choices=["ocml", "fast", "det"])
     ap.add_argument("--contract", default="fast", choices=["fast", "off"])
     ap.add_argument("--persistent", type=int, default=1)
     ap.add_argument("--block-size", type=int, default=0)

@@ -70,8 +70,10 @@ class HIPDevice:
       contract    "off": no FP contraction (the reference's unfused Float32 sequence; bitwise
                   comparable with the CPU restatement). "fast": FMA contraction. None: "off" for
                   fixed-step solves, "fast" for adaptive solves.
-      powmode     controller powers: "ocml" (device powf, the same library call AMDGPU.jl makes
-                  for Float32 ^), "fast" (hardware log2/exp2) or "det" (deterministic polynomial).
+      powmode     controller powers: "fast" (default: the PI controller evaluated in log2 space with
+                  hardware log2/exp2 on the squared error norm), "ocml" (device powf, the same
+                  library call AMDGPU.jl makes for Float32 ^, with the reference's operation
+                  sequence) or "det" (deterministic polynomial, bitwise comparable with the oracle).
       persistent  persistent grid with a device work counter (lanes refill as trajectories end);
                   None picks it for adaptive solves.
       kahan       compensated accumulation of t.
@@ -85,7 +87,7 @@ class HIPDevice:
 
     device: int = 0
     contract: str | None = None
-    powmode: str = "ocml"
+    powmode: str = "fast"
     persistent: bool | None = None
     kahan: bool = False
     block_size: int | None = None
@@ -102,12 +104,12 @@ class HIPDevice:
             raise ValueError("contract must be 'off' or 'fast'")
         if contract == "off":
             f |= _lib.F_PARITY
-        if self.powmode == "det":
-            f |= _lib.F_DETPOW
-        elif self.powmode == "fast":
-            f |= _lib.F_FASTPOW
-        elif self.powmode != "ocml":
+        if self.powmode not in ("ocml", "fast", "det"):
             raise ValueError("powmode must be 'ocml', 'fast' or 'det'")
+        if adaptive and self.powmode == "det":
+            f |= _lib.F_DETPOW
+        elif adaptive and self.powmode == "fast":
+            f |= _lib.F_FASTPOW
         if self.kahan:
             f |= _lib.F_KAHAN
         persistent = self.persistent if self.persistent is not None else adaptive

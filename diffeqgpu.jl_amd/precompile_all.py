@@ -17,7 +17,8 @@ def variants():
         (HIPDevice(contract="off"), False),                                   # fixed step, parity
         (HIPDevice(contract="off", powmode="det", persistent=False), True),   # adaptive, bitwise vs oracle
         (HIPDevice(contract="off", powmode="det", persistent=True), True),
-        (HIPDevice(contract="off", persistent=True), True),                   # adaptive, exact ops, device powf
+        (HIPDevice(contract="off", powmode="ocml", persistent=True), True),   # adaptive, exact ops, device powf
+        (HIPDevice(powmode="ocml"), True),
         (HIPDevice(), True),                                                  # adaptive defaults (persistent)
         (HIPDevice(persistent=False), True),
         (HIPDevice(powmode="fast"), True),

@@ -216,7 +216,7 @@ def test_api_surface(deg):
     # device flags
     from diffeqgpu_amd import _lib
 
-    assert deg.HIPDevice().flags(True) == _lib.F_ADAPTIVE | _lib.F_PERSISTENT
+    assert deg.HIPDevice().flags(True) == _lib.F_ADAPTIVE | _lib.F_PERSISTENT | _lib.F_FASTPOW
     assert deg.HIPDevice().flags(False) == _lib.F_PARITY
     assert deg.HIPDevice(contract="off", powmode="det", persistent=False).flags(True) == 1 | 2 | 4
     with pytest.raises(ValueError):
