@@ -1,15 +1,31 @@
-"""Benchmark: trajectories/s of the adaptive GPUTsit5 ensemble on the 1M-trajectory Lorenz
-workload (abstol = reltol = 1e-6, saveat 0:1:10, 11 points x 3 states written SoA), one
-MI355X.
+"""Benchmark: trajectories/s of the ensemble solvers on one MI355X.
 
-A step is one ensemble solve of all 1,000,000 trajectories with the inputs resident in
-HBM. Prints one JSON line (see the keys below). The roofline entry prices the solver kernel
-against the FP32 vector peak with the algorithmic flop count 264 per Tsit5 attempt
-(66n + 18 + 6F, n = 3, F = 8) plus 85 per interpolated save point (15n + 40), using the
-kernel's own per-trajectory accept/reject counters; the kernel time comes from HIP events
-recorded on the launch stream around every timed launch.
+Default workload (the headline configuration): 1,000,000 Lorenz trajectories, adaptive
+GPUTsit5 abstol = reltol = 1e-6, dt0 = 0.1, tspan (0, 10), saveat 0:1:10 (11 points x 3 states
+written SoA), PI controller. A step is one ensemble solve of all trajectories with the inputs
+resident in HBM; the step is relaunched into the same output buffers.
 
-Options: --gpus N (only 1: one GPU), --steps K, --warmup W, --n-traj, --device-flags,
+Other workloads (``--workload``), measured the same way:
+  lorenz_uneven  4,000,000 Lorenz, rho~U(0,60), sigma~U(5,15), beta~U(1,4), tend~U(1,50), GPUVern7
+                 abstol = reltol = 1e-8, compensated t, endpoint only. One step is a replay of a
+                 HIP graph holding 10 sweeps. Each sweep re-draws (sigma, rho, beta, tend) on the
+                 device and solves all 4M trajectories, so a step solves 40M trajectories.
+  robertson      1,000,000 Robertson kinetics, k = (0.04, 3e7, 1e4) .* exp(0.3 Z), GPURosenbrock23
+                 reltol = 1e-4, abstol = 1e-8, FP64 W factorisation, 20 log-spaced save times.
+  vdp            2,000,000 Van der Pol, mu ~ logU(0.1, 30), u0 ~ U(-2, 2)^2, tspan (0, 50),
+                 GPUTsit5 abstol = reltol = 1e-6, endpoint only.
+
+Prints one JSON line. ``roofline`` prices the solver kernel against the FP32 vector peak. The
+flop count is algorithmic:
+  - Tsit5 attempt: 66n + 18 + 6F; Vern7 attempt: 116n + 24 + 10F;
+  - Rosenbrock23 attempt: 25n + 8 + 3F + FJ + FT in FP32, plus W = I - gamma dt J and its LU
+    with three solves, n^2 + n + (2/3)n^3 + 6n^2 in FP64;
+  - dense output per save point: Tsit5 15n + 40, Rosenbrock23 4n + 8.
+F, FJ and FT are the traced RHS, Jacobian and time-gradient flop counts. Attempts come from the
+kernel's own per-trajectory accept/reject counters. The kernel time comes from HIP events
+recorded on the launch stream around every timed step.
+
+Options: --gpus N (1 only: one GPU), --steps K, --warmup W, --workload, --n-traj,
 --copy-probe (one 256 MiB device copy first, to calibrate HBM counters under rocprofv3),
 --no-cpu (skip the CPU baseline).
 """
@@ -25,21 +41,40 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (includes dual-issue/packed FP32)
-FLOPS_PER_ATTEMPT = 264    # Tsit5 adaptive attempt, Lorenz (n = 3, F = 8)
-FLOPS_PER_SAVE = 85        # Tsit5 dense-output point, n = 3
+METRIC = "trajectories/sec at fixed FP64-oracle error; learned vs PI controller"
+
+WORKLOADS = {
+    "lorenz_adaptive": dict(alg="GPUTsit5", steps=20, warmup=3, golden="lorenz_adaptive_vern9.npz", cpu_sample=200_000,
+                            desc="lorenz_adaptive_1M: Lorenz, GPUTsit5 adaptive abstol=reltol=1e-6, dt0=0.1, "
+                                 "tspan (0,10), saveat 0:1:10 (11 x 3 SoA), PI controller"),
+    "lorenz_uneven": dict(alg="GPUVern7", steps=3, warmup=1, golden="lorenz_uneven_vern9.npz", cpu_sample=20_000,
+                          sweeps=10,
+                          desc="lorenz_uneven_4M: Lorenz rho~U(0,60) sigma~U(5,15) beta~U(1,4) tend~U(1,50), "
+                               "GPUVern7 adaptive 1e-8, Kahan t, endpoint only, 10 device-drawn sweeps per "
+                               "hipGraph replay"),
+    "robertson": dict(alg="GPURosenbrock23", steps=10, warmup=2, golden="robertson_radau.npz", cpu_sample=100_000,
+                      desc="robertson_1M: Robertson k=(0.04,3e7,1e4).*exp(0.3Z), GPURosenbrock23 reltol=1e-4 "
+                           "abstol=1e-8, FP64 W-LU, 20 log-spaced saves"),
+    "vdp": dict(alg="GPUTsit5", steps=10, warmup=2, golden="vdp_vern9.npz", cpu_sample=100_000,
+                desc="vdp_2M: Van der Pol mu~logU(0.1,30) u0~U(-2,2)^2 tspan (0,50), GPUTsit5 adaptive "
+                     "abstol=reltol=1e-6, endpoint only, PI controller"),
+}
+UNEVEN_DRAWS = [("p", 0, 5.0, 10.0), ("p", 1, 0.0, 60.0), ("p", 2, 1.0, 3.0), ("tfs", 0, 1.0, 49.0)]
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n-traj", type=int, default=1_000_000)
+    ap.add_argument("--workload", default="lorenz_adaptive", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=0)
+    ap.add_argument("--warmup", type=int, default=-1)
+    ap.add_argument("--n-traj", type=int, default=0)
     ap.add_argument("--powmode", default="fast", choices=["ocml", "fast", "det"])
     ap.add_argument("--contract", default="fast", choices=["fast", "off"])
     ap.add_argument("--persistent", type=int, default=1)
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -57,21 +92,42 @@ def load_traffic(name):
         return None
 
 
-def cpu_baseline(cfg, threads):
-    """The oracle (CPU restatement of the same algorithm) on the host cores, same inputs."""
+def flop_model(method, n, fc):
+    """(FP32 flops per attempt, FP64 flops per attempt, flops per save point)."""
+    F, FJ, FT = fc["F"], fc["FJ"], fc["FT"]
+    if method == "GPUTsit5":
+        return 66 * n + 18 + 6 * F, 0, 15 * n + 40
+    if method == "GPUVern7":
+        return 116 * n + 24 + 10 * F, 0, 0
+    return 25 * n + 8 + 3 * F + FJ + FT, n * n + n + (2 * n ** 3) // 3 + 6 * n * n, 4 * n + 8
+
+
+def solve_kw(cfg):
+    kw = dict(dt=cfg["dt"], abstol=cfg["abstol"], reltol=cfg["reltol"])
+    if cfg.get("saveat") is not None:
+        kw["saveat"] = cfg["saveat"]
+    return kw
+
+
+def cpu_baseline(name, cfg, threads, sample):
+    """The oracle (CPU Float32 restatement of the same algorithm) on the host cores, same inputs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
-    sv = np.arange(11, dtype=np.float32)
-    n_sample = min(cfg["N"], 200_000)
-    u0, p = cfg["u0"][:, :n_sample], cfg["p"][:, :n_sample]
-    pyoracle.solve("lorenz", "tsit5", True, u0[:, :1000], p[:, :1000], (0.0, 10.0), 0.1, abstol=1e-6, reltol=1e-6,
-                   saveat=sv, nthreads=threads)
+    method = {"GPUTsit5": "tsit5", "GPUVern7": "vern7", "GPURosenbrock23": "rb23"}[WORKLOADS[name]["alg"]]
+    n_sample = min(cfg["N"], sample)
+    sl = slice(0, n_sample)
+    kw = dict(abstol=cfg["abstol"], reltol=cfg["reltol"], nthreads=threads, kahan=bool(cfg.get("kahan")))
+    if cfg.get("saveat") is not None:
+        sv = cfg["saveat"]
+        kw["saveat"] = (np.arange(11, dtype=np.float32) if np.ndim(sv) == 0 else np.asarray(sv, np.float32))
+    if cfg.get("tfs") is not None:
+        kw["tfs"] = cfg["tfs"][sl]
+    args = (cfg["system"], method, True, cfg["u0"][:, sl], cfg["p"][:, sl], cfg["tspan"], cfg["dt"])
     reps = 0
     t0 = time.perf_counter()
     while True:
-        pyoracle.solve("lorenz", "tsit5", True, u0, p, (0.0, 10.0), 0.1, abstol=1e-6, reltol=1e-6, saveat=sv,
-                       nthreads=threads)
+        pyoracle.solve(*args, **kw)
         reps += 1
         el = time.perf_counter() - t0
         if el > 10.0 or reps >= 50:
@@ -79,6 +135,19 @@ def cpu_baseline(cfg, threads):
     return dict(value=reps * n_sample / el, unit="trajectories/s", cores=threads, kind="port",
                 sample=f"{reps} x {n_sample} trajectories of the same workload (first {n_sample} inputs), "
                        f"oracle/deg_oracle.c Float32 restatement, OpenMP {threads} threads, {el:.1f} s")
+
+
+def fp64_error(name, us):
+    """Endpoint error (max over states) against the FP64 fixture, first 1,000 trajectories."""
+    spec = WORKLOADS[name]
+    fx = os.path.join(ROOT, "tests", "golden", spec["golden"])
+    if not os.path.exists(fx):
+        return None
+    ref = np.load(fx)["u"]
+    e = np.abs(us[-1].astype(np.float64) - ref[-1]).max(axis=0)
+    return dict(median=float(np.median(e)), p99=float(np.percentile(e, 99)), max=float(e.max()), subset=int(e.size),
+                reference=f"tests/golden/{spec['golden']} (FP64; "
+                          + ("scipy Radau rtol 1e-11" if name == "robertson" else "Vern9 abstol=reltol=1e-12") + ")")
 
 
 def main():
@@ -92,11 +161,15 @@ def main():
     import __graft_entry__ as entry
 
     deg = entry.load_package()
-    from diffeqgpu_amd import systems, workloads
+    from diffeqgpu_amd import systems, tracer, workloads
 
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+    name = args.workload
+    spec = WORKLOADS[name]
+    steps = args.steps or spec["steps"]
+    warmup = spec["warmup"] if args.warmup < 0 else args.warmup
     dev_t = torch.device("cuda", 0)
     if args.copy_probe:
         a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev_t)
@@ -105,84 +178,94 @@ def main():
         torch.cuda.synchronize()
         del a, b
 
-    cfg = workloads.config("lorenz_adaptive", N=args.n_traj)
+    cfg = workloads.config(name, N=args.n_traj or None)
     N = cfg["N"]
-    f, n, np_, _ = systems.SYSTEMS["lorenz"]
+    f, n, np_, _ = systems.SYSTEMS[cfg["system"]]
     dev = deg.HIPDevice(0, contract=args.contract, powmode=args.powmode, persistent=bool(args.persistent),
-                        block_size=args.block_size or None, refill=args.refill)
-    probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev)
+                        block_size=args.block_size or None, refill=args.refill, chunk=args.chunk,
+                        kahan=bool(cfg.get("kahan")))
+    alg = getattr(deg, spec["alg"])()
     prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
-    alg = deg.GPUTsit5()
-    kw = dict(dt=0.1, abstol=1e-6, reltol=1e-6, saveat=1.0)
+    kw = solve_kw(cfg)
 
-    # The first call JIT-loads the kernel, allocates the outputs and runs one solve; every later
-    # step relaunches the same solve into the same output buffers (all outputs are recomputed).
-    res = deg.vectorized_asolve(probs, prob, alg, **kw)
-    for _ in range(args.warmup):
-        res.relaunch()
+    if name == "lorenz_uneven":
+        sweeps = spec["sweeps"]
+        probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev, tfs=cfg["tfs"])
+        sw = deg.ParameterSweep(probs, prob, alg, sweeps=sweeps, seed=303, draws=UNEVEN_DRAWS, **kw)
+        stream = sw.stream
+        run = sw.replay
+        per_step = sweeps * N
+        results = sw.results
+    else:
+        probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev)
+        stream = torch.cuda.current_stream(dev_t)
+        # The first call JIT-loads the kernel, allocates the outputs and runs one solve; every later
+        # step relaunches the same solve into the same output buffers (all outputs are recomputed).
+        res = deg.vectorized_asolve(probs, prob, alg, **kw)
+        run = res.relaunch
+        per_step = N
+        results = [res]
+
+    for _ in range(warmup):
+        run()
     torch.cuda.synchronize()
-    stream = torch.cuda.current_stream(dev_t)
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         evs[k][0].record(stream)
-        res.relaunch()
+        run()
         evs[k][1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_ms = [s.elapsed_time(e) for s, e in evs]
-    ms_per_step = wall * 1e3 / args.steps
-    value = N / (wall / args.steps)
+    ms_per_step = wall * 1e3 / steps
+    value = per_step / (wall / steps)
 
-    # work actually done (identical every step: the solve is deterministic)
-    attempts = int((res.naccept.to(torch.int64) + res.nreject.to(torch.int64)).sum().item())
-    saves_interp = N * (res.ts.shape[0] - 1)
-    flops = attempts * FLOPS_PER_ATTEMPT + saves_interp * FLOPS_PER_SAVE
+    # work actually done per step (identical every step: the solves are deterministic)
+    fc = tracer.flop_counts(f.f, n, np_, f.jac, f.tgrad)
+    fa32, fa64, fsave = flop_model(spec["alg"], n, fc)
+    attempts = sum(int((r.naccept.to(torch.int64) + r.nreject.to(torch.int64)).sum().item()) for r in results)
+    # save points after t0 are dense-output evaluations (a save point at t0 copies u0)
+    nsave_interp = sum(N * int((r.saveat > float(np.float32(cfg["tspan"][0]))).sum().item())
+                       for r in results if r.saveat is not None)
+    flops = attempts * (fa32 + fa64) + nsave_interp * fsave
     k_avg_s = float(np.mean(kernel_ms)) / 1e3
     achieved = flops / k_avg_s / 1e12
-    retcodes = np.bincount(res.retcode.cpu().numpy(), minlength=4).tolist()
+    retcodes = np.bincount(np.concatenate([r.retcode.cpu().numpy() for r in results]), minlength=4).tolist()
 
-    # FP64-oracle endpoint error on the first 1,000 trajectories
     err = None
-    fx = os.path.join(ROOT, "tests", "golden", "lorenz_adaptive_vern9.npz")
-    if os.path.exists(fx) and N >= 1000:
-        ref = np.load(fx)["u"]
-        got = res.us_soa[:, :, :1000].cpu().numpy().astype(np.float64)
-        e = np.abs(got[-1] - ref[-1]).max(axis=0)
-        err = dict(median=float(np.median(e)), p99=float(np.percentile(e, 99)), max=float(e.max()),
-                   subset=1000, reference="FP64 Vern9 abstol=reltol=1e-12 (tests/golden/lorenz_adaptive_vern9.npz)")
+    if N >= 1000:
+        err = fp64_error(name, results[0].us_soa[:, :, :1000].cpu().numpy())
 
-    traffic = load_traffic("lorenz_adaptive_tsit5_traffic.json")
+    traffic = load_traffic({"lorenz_adaptive": "lorenz_adaptive_tsit5_traffic.json"}.get(name, ""))
     out = {
-        "metric": "trajectories/sec at fixed FP64-oracle error; learned vs PI controller",
+        "metric": METRIC,
         "value": value,
         "unit": "trajectories/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded SplitMix64 parameter draws p = [10, 28, 8/3] .* U^3)",
-        "config": {"workload": "lorenz_adaptive_1M: Lorenz, GPUTsit5 adaptive abstol=reltol=1e-6, dt0=0.1, "
-                               "tspan (0,10), saveat 0:1:10 (11 x 3 SoA), PI controller",
-                   "trajectories": N, "controller": "PI", "kernel_flags": res.flags,
-                   "contract": args.contract, "powmode": args.powmode, "persistent": bool(args.persistent),
-                   "block_size": args.block_size or 256},
+        "dtype": "f32" if spec["alg"] != "GPURosenbrock23" else "f32 (W-LU f64)",
+        "data": "synthetic (seeded SplitMix64 draws of the workload's parameter law)",
+        "config": {"workload": spec["desc"], "trajectories_per_step": per_step, "controller": "PI",
+                   "kernel_flags": results[0].flags, "contract": args.contract, "powmode": args.powmode,
+                   "persistent": bool(args.persistent), "block_size": args.block_size or 256},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "kernel_ms": float(np.mean(kernel_ms)), "flops_per_launch": flops,
-                     "attempts_per_launch": attempts, "saves_per_launch": saves_interp},
+                     "kernel_ms": float(np.mean(kernel_ms)), "flops_per_step": flops,
+                     "attempts_per_step": attempts, "saves_per_step": nsave_interp,
+                     "flops_per_attempt": {"fp32": fa32, "fp64": fa64}, "rhs_flops": fc},
         "fp64_oracle_error": err,
         "retcodes": retcodes,
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(cfg, threads)
+        out["cpu_baseline"] = cpu_baseline(name, cfg, threads, spec["cpu_sample"])
     print(json.dumps(out))
 
 

@@ -11,6 +11,7 @@ from .lowerlevel import SolveResult, vectorized_asolve, vectorized_solve
 from .problems import (EnsembleProblem, HIPProbVector, ImmutableODEProblem, ODEFunction, ODEProblem, adapt,
                        get_backend, make_prob_compatible, remake)
 from .ranges import colon, linrange
+from .sweeps import Draw, ParameterSweep
 from .tracer import SVector
 from . import tracer as math  # sin, cos, exp, log, sqrt, tanh, fabs, pow, ifelse on traced values
 
@@ -19,5 +20,5 @@ __all__ = [
     "HIPDevice", "alg_order", "EnsembleContext", "EnsembleSolution", "ODESolution", "solve", "SolveResult",
     "vectorized_asolve", "vectorized_solve", "EnsembleProblem", "HIPProbVector", "ImmutableODEProblem",
     "ODEFunction", "ODEProblem", "adapt", "get_backend", "make_prob_compatible", "remake", "colon", "linrange",
-    "SVector", "math",
+    "SVector", "math", "Draw", "ParameterSweep",
 ]

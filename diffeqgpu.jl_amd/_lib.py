@@ -81,8 +81,12 @@ class SolveArgs(ctypes.Structure):
 EXPORTS = [
     "deg_ctx_create", "deg_ctx_destroy", "deg_ctx_num_cus", "deg_set_cache_dir", "deg_kernel_compile",
     "deg_kernel_destroy", "deg_kernel_attr", "deg_jit_precompile", "deg_jit_source", "deg_solve", "deg_sync",
-    "deg_last_error", "deg_version", "deg_solve_args_size",
+    "deg_last_error", "deg_version", "deg_solve_args_size", "deg_draw_uniform", "deg_graph_begin", "deg_graph_end",
+    "deg_graph_launch", "deg_graph_num_nodes", "deg_graph_destroy", "deg_mlp_packed_size", "deg_mlp_partial_cols",
+    "deg_mlp_grad", "deg_mlp_adam", "deg_mlp_forward", "deg_mlp_pack", "deg_mlp_infer",
 ]
+STEPLOG_COLS = 12
+MLP_NPARAM = 1313
 
 _lock = threading.Lock()
 _lib = None
@@ -112,8 +116,26 @@ def _declare(lib):
     lib.deg_last_error.restype = ctypes.c_char_p
     lib.deg_version.restype = ctypes.c_char_p
     lib.deg_solve_args_size.restype = ctypes.c_uint32
+    lib.deg_draw_uniform.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int32,
+                                     ctypes.c_int32, ctypes.c_float, ctypes.c_float, vp]
+    lib.deg_graph_begin.argtypes = [vp]
+    lib.deg_graph_end.argtypes = [vp, ctypes.POINTER(vp)]
+    lib.deg_graph_launch.argtypes = [vp, vp]
+    lib.deg_graph_num_nodes.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    lib.deg_graph_destroy.argtypes = [vp]
+    f32, i64, i32 = ctypes.c_float, ctypes.c_int64, ctypes.c_int32
+    lib.deg_mlp_packed_size.restype = ctypes.c_uint32
+    lib.deg_mlp_partial_cols.restype = ctypes.c_int32
+    lib.deg_mlp_grad.argtypes = [vp, vp, i64, vp, vp, f32, f32, f32, vp, i32, vp]
+    lib.deg_mlp_adam.argtypes = [vp, vp, vp, vp, i32, i32, f32, f32, f32, f32, vp, vp]
+    lib.deg_mlp_forward.argtypes = [vp, vp, vp, i64, i64, vp, vp]
+    lib.deg_mlp_pack.argtypes = [vp, vp, f32, f32, vp, vp]
+    lib.deg_mlp_infer.argtypes = [vp, vp, i64, i64, vp, vp]
     for name in ("deg_ctx_create", "deg_ctx_destroy", "deg_ctx_num_cus", "deg_set_cache_dir", "deg_kernel_compile",
-                 "deg_kernel_destroy", "deg_kernel_attr", "deg_jit_precompile", "deg_solve", "deg_sync"):
+                 "deg_kernel_destroy", "deg_kernel_attr", "deg_jit_precompile", "deg_solve", "deg_sync",
+                 "deg_draw_uniform", "deg_graph_begin", "deg_graph_end", "deg_graph_launch", "deg_graph_num_nodes",
+                 "deg_graph_destroy", "deg_mlp_grad", "deg_mlp_adam", "deg_mlp_forward", "deg_mlp_pack",
+                 "deg_mlp_infer"):
         getattr(lib, name).restype = ctypes.c_int
 
 
@@ -212,3 +234,40 @@ def context(device: int = 0) -> Context:
         with _lock:
             _contexts[device] = c
     return c
+
+
+def draw_uniform(out_ptr: int, n: int, seed: int, rows: int, row: int, lo: float, scale: float, stream_ptr,
+                 id_offset: int = 0):
+    """Device draws ``out[i] = lo + scale * U(seed, (id_offset + i) * rows + row)`` (rng.uniform_table)."""
+    check(lib().deg_draw_uniform(ctypes.c_void_p(out_ptr), n, id_offset, seed & 0xFFFFFFFFFFFFFFFF, rows, row, lo,
+                                 scale, ctypes.c_void_p(stream_ptr)), "deg_draw_uniform")
+
+
+class Graph:
+    """An instantiated HIP graph of the launches made on one stream between ``begin`` and ``end``."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @staticmethod
+    def begin(stream_ptr):
+        check(lib().deg_graph_begin(ctypes.c_void_p(stream_ptr)), "deg_graph_begin")
+
+    @staticmethod
+    def end(stream_ptr) -> "Graph":
+        h = ctypes.c_void_p()
+        check(lib().deg_graph_end(ctypes.c_void_p(stream_ptr), ctypes.byref(h)), "deg_graph_end")
+        return Graph(h)
+
+    def launch(self, stream_ptr):
+        check(lib().deg_graph_launch(self.handle, ctypes.c_void_p(stream_ptr)), "deg_graph_launch")
+
+    def num_nodes(self) -> int:
+        c = ctypes.c_int64()
+        check(lib().deg_graph_num_nodes(self.handle, ctypes.byref(c)), "deg_graph_num_nodes")
+        return c.value
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.deg_graph_destroy(self.handle)
+            self.handle = None

@@ -517,6 +517,17 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
     long long attempts = 0;
     bool have_k = false;
     bool fsal_pending = false;
+#if DEG_LEARNED || DEG_STEPLOG
+    float lx_prev = -13.2877123795f;  // log2 EEst of the previous attempt (log2 qoldinit at first)
+    float ldt_prev = 0.0f;            // log2 dt of the previous attempt
+    bool first_attempt = true;
+#endif
+#if DEG_STEPLOG
+    float fprev[DEG_MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float zpi_prev = 0.0f;
+    bool acc_prev = false;
+    bool have_prev = false;
+#endif
 #if DEG_PERSISTENT
     bool need_init = false;
     bool have = false;  // the lane holds a trajectory
@@ -577,7 +588,12 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
                 retire = want;  // every slab is exhausted
             }
         }
+#if DEG_LEARNED
+        // the matrix-core proposer needs every lane: the wave leaves together
+        if (__ballot(!retire) == 0ull) break;
+#else
         if (retire) break;
+#endif
 #endif
         if (need_init) {
             // ---- init (nonstiff/types.jl:305-402, stiff/types.jl:32-177; kernels.jl:35-45, :116-126)
@@ -603,6 +619,14 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
             status = DEG_RET_SUCCESS;
             have_k = false;
             fsal_pending = false;
+#if DEG_LEARNED || DEG_STEPLOG
+            lx_prev = -13.2877123795f;
+            ldt_prev = 0.0f;
+            first_attempt = true;
+#endif
+#if DEG_STEPLOG
+            have_prev = false;
+#endif
             DEG_FOR(c) uprev[c] = u[c];
             if (use_saveat) {
                 cur = 0;
@@ -624,14 +648,26 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
 #endif
         }
 
+        // Each iteration is one attempt (or one fixed step) per lane holding a trajectory, in
+        // two phases: (A) the stages and the error estimate, (B) the step controller, the
+        // saves and the endpoint rules. Between them the learned proposer (DEG_LEARNED) runs
+        // on the matrix cores with all lanes of the wavefront.
+        bool done = false;
+        bool accepted = false;
+#if DEG_ADAPTIVE
+        bool attempted = false;
+        float EEst = 0.0f;
+#endif
+#if DEG_LEARNED || DEG_STEPLOG
+        float feat[DEG_MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#endif
 #if DEG_PERSISTENT
         if (have) {
 #else
         {
 #endif
-        bool done = !(t < tf);
+        done = !(t < tf);
         if (!done) {
-            bool accepted = false;
             if (attempts >= a.max_attempts) {
                 status = DEG_RET_MAXITERS;
             }
@@ -643,7 +679,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
             else {
                 ++attempts;
 #if DEG_ADAPTIVE
-                // ---- one attempt of the `while EEst > 1` loop
+                // ---- one attempt of the `while EEst > 1` loop: stages and error estimate
 #if DEG_METHOD == 0
                 if (!have_k) {
                     deg_rhs(u, p, t, k[1]);
@@ -653,68 +689,31 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
                 }
                 fsal_pending = false;
                 deg_attempt(u, p, t, dt, k, unew);
-                float EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
+                EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
 #elif DEG_METHOD == 1
                 deg_rhs(u, p, t, k[1]);
                 deg_attempt(u, p, t, dt, k, unew);
-                float EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
+                EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
 #else
-                float EEst = deg_rb23<true>(u, p, t, dt, k, unew, a.abstol, a.reltol);
+                EEst = deg_rb23<true>(u, p, t, dt, k, unew, a.abstol, a.reltol);
 #endif
-#if DEG_FASTPOW
-                // EEst holds EEst^2 here (scaled_error2_fast)
-                if (EEst != EEst) status = DEG_RET_NONFINITE;
-                const bool acc_step = !(EEst > 1.0f);
-                const float factor = deg_fast_control(EEst, acc_step, qpow);
-                if (!acc_step) {
-                    dt = dt * factor;
-                    ++nrej;
-                } else {
-                    float dtnew = dt * factor;
-                    dtnew = __builtin_fminf(__builtin_fabsf(dtnew), __builtin_fabsf((tf - t) - dt));
-#else
-                float q11 = 0.0f, q;
-                if (EEst == 0.0f) {
-                    q = 1.0f / DEG_QMAX;
-                } else {
-                    q11 = ctl_pow(EEst, DEG_BETA1);
-                    q = q11 / qpow;
-                }
-                if (EEst > 1.0f) {
-                    dt = dt / jl_min(1.0f / DEG_QMIN, q11 / DEG_GAMMA);
-                    ++nrej;
-                } else {
-                    q = jl_max(1.0f / DEG_QMAX, jl_min(1.0f / DEG_QMIN, q / DEG_GAMMA));
-                    qold = jl_max(EEst, DEG_QOLDINIT);
-                    qpow = ctl_pow(qold, DEG_BETA2);
-                    float dtnew = dt / q;
-                    dtnew = jl_min(__builtin_fabsf(dtnew), __builtin_fabsf((tf - t) - dt));
-#endif
-                    dtacc = dt;
-                    tprev = t;
+                attempted = true;
+#if DEG_LEARNED || DEG_STEPLOG
+                {
+                    // proposer features (base-2 logs; EEst holds EEst^2 on this path)
+                    float su = 0.0f, sf = 0.0f;
                     DEG_FOR(c) {
-                        uprev[c] = u[c];
-                        u[c] = unew[c];
+                        su = __builtin_fmaf(u[c], u[c], su);
+                        sf = __builtin_fmaf(k[1][c], k[1][c], sf);
                     }
-                    if ((tf - t) - dt < DEG_DTMIN) {
-                        t = tf;
-                        tc = 0.0f;
-                    } else {
-                        float tn = deg_advance(t, dt, tc);
-#if !DEG_KAHAN
-                        // A step below the resolution of t would repeat forever: snap to tf
-                        // when it is within two ulps, otherwise stop with DtLessThanMin.
-                        if (tn == t) {
-                            if (tf - t <= 2.0f * (deg_next_up(t) - t)) tn = tf;
-                            else status = DEG_RET_DTMIN;
-                        }
-#endif
-                        t = tn;
-                    }
-                    dt = dtnew;
-                    ++nacc;
-                    accepted = true;
+                    feat[0] = 0.5f * __builtin_amdgcn_logf(__builtin_fmaxf(EEst, 1e-30f));
+                    feat[1] = lx_prev;
+                    feat[2] = __builtin_amdgcn_logf(dt);
+                    feat[3] = 0.5f * __builtin_amdgcn_logf(__builtin_fmaxf(su * (1.0f / DEG_N), 1e-30f));
+                    feat[4] = 0.5f * __builtin_amdgcn_logf(__builtin_fmaxf(sf * (1.0f / DEG_N), 1e-30f));
+                    feat[5] = first_attempt ? 0.0f : feat[2] - ldt_prev;
                 }
+#endif
 #else
                 // ---- one fixed step (gpu_*_perform_step.jl fixed versions)
                 const float told = t;
@@ -753,35 +752,150 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
                 }
 #endif
             }
-            if (accepted) {
-                // ---- savevalues! (integrator_utils.jl:13-50)
-                if (use_saveat) {
-#if DEG_HAS_INTERP
-                    while (tnext <= t) {
-                        float out[DEG_N];
-                        const float th = (tnext - tprev) / dtacc;
-                        deg_interp(th, dtacc, tprev, p, uprev, k, out);
-                        if (cur < a.nsave) {
-                            DEG_FOR(c) a.us[((long long)cur * DEG_N + c) * S + i] = out[c];
-                            a.ts[(long long)cur * S + i] = tnext;
-                        }
-                        ++cur;
-                        tnext = (cur < a.nsaveat) ? svp[cur] : finf;
-                    }
-#endif
-                } else if (a.save_everystep) {
-#if !DEG_ADAPTIVE
-                    if (step_idx < a.nsave) {
-                        DEG_FOR(c) a.us[((long long)step_idx * DEG_N + c) * S + i] = u[c];
-                        a.ts[(long long)step_idx * S + i] = t;
-                    }
-                    ++step_idx;
-#endif
-                }
-                fsal_pending = true;
-            }
-            done = (status != DEG_RET_SUCCESS) || !(t < tf);
         }
+        }  // phase A
+
+#if DEG_LEARNED
+        // every lane of the wavefront is here (the loop exit is wave-uniform in this variant)
+        const float zml = deg_mlp_eval((const deg_mlp_packed*)a.mlp, feat);
+#endif
+
+#if DEG_PERSISTENT
+        if (have) {
+#else
+        {
+#endif
+#if DEG_ADAPTIVE
+        if (attempted) {
+            // ---- step-size control and acceptance (gpu_*_perform_step.jl adaptive versions)
+#if DEG_FASTPOW
+            // EEst holds EEst^2 here (scaled_error2_fast)
+            if (EEst != EEst) status = DEG_RET_NONFINITE;
+            const bool acc_step = !(EEst > 1.0f);
+            float factor = deg_fast_control(EEst, acc_step, qpow);
+#if DEG_LEARNED || DEG_STEPLOG
+            const float zpi = __builtin_amdgcn_logf(factor);  // log2 of the PI factor
+#endif
+#if DEG_LEARNED
+            {
+                // the proposer corrects the PI factor in log2 space, within the PI range
+                // [qmin, qmax]; after a rejection it never proposes more than 0.9 dt
+                float zz = __builtin_fminf(__builtin_fmaxf(zpi + zml, -2.32192809f), 3.32192809f);
+                if (!acc_step) zz = __builtin_fminf(zz, -0.152003093f);
+                const float fml = __builtin_amdgcn_exp2f(zz);
+                if (fml == fml) factor = fml;
+            }
+#endif
+#if DEG_STEPLOG
+            {
+                // the previous attempt's row, completed by this attempt's outcome; slots are
+                // claimed with one atomic per wavefront
+                const bool wr = have_prev;
+                const unsigned long long m = __ballot(wr);
+                if (m != 0ull) {
+                    const int ln = (int)__lane_id();
+                    const int leader = __ffsll(m) - 1;
+                    unsigned int base = 0u;
+                    if (ln == leader) base = atomicAdd(a.steplog_count, (unsigned int)__popcll(m));
+                    base = (unsigned int)__shfl((int)base, leader);
+                    const long long slot = (long long)base + __popcll(m & ((1ull << ln) - 1ull));
+                    if (wr && slot < a.steplog_cap) {
+                        float4* row = (float4*)(a.steplog + slot * DEG_STEPLOG_COLS);
+                        row[0] = make_float4(fprev[0], fprev[1], fprev[2], fprev[3]);
+                        row[1] = make_float4(fprev[4], fprev[5], feat[2] - fprev[2], feat[0]);
+                        row[2] = make_float4(zpi_prev, acc_prev ? 1.0f : 0.0f, 0.0f, 0.0f);
+                    }
+                }
+                _Pragma("unroll") for (int j = 0; j < DEG_MLP_IN; ++j) fprev[j] = feat[j];
+                zpi_prev = zpi;
+                acc_prev = acc_step;
+                have_prev = true;
+            }
+#endif
+#if DEG_LEARNED || DEG_STEPLOG
+            lx_prev = feat[0];
+            ldt_prev = feat[2];
+            first_attempt = false;
+#endif
+            if (!acc_step) {
+                dt = dt * factor;
+                ++nrej;
+            } else {
+                float dtnew = dt * factor;
+                dtnew = __builtin_fminf(__builtin_fabsf(dtnew), __builtin_fabsf((tf - t) - dt));
+#else
+            float q11 = 0.0f, q;
+            if (EEst == 0.0f) {
+                q = 1.0f / DEG_QMAX;
+            } else {
+                q11 = ctl_pow(EEst, DEG_BETA1);
+                q = q11 / qpow;
+            }
+            if (EEst > 1.0f) {
+                dt = dt / jl_min(1.0f / DEG_QMIN, q11 / DEG_GAMMA);
+                ++nrej;
+            } else {
+                q = jl_max(1.0f / DEG_QMAX, jl_min(1.0f / DEG_QMIN, q / DEG_GAMMA));
+                qold = jl_max(EEst, DEG_QOLDINIT);
+                qpow = ctl_pow(qold, DEG_BETA2);
+                float dtnew = dt / q;
+                dtnew = jl_min(__builtin_fabsf(dtnew), __builtin_fabsf((tf - t) - dt));
+#endif
+                dtacc = dt;
+                tprev = t;
+                DEG_FOR(c) {
+                    uprev[c] = u[c];
+                    u[c] = unew[c];
+                }
+                if ((tf - t) - dt < DEG_DTMIN) {
+                    t = tf;
+                    tc = 0.0f;
+                } else {
+                    float tn = deg_advance(t, dt, tc);
+#if !DEG_KAHAN
+                    // A step below the resolution of t would repeat forever: snap to tf
+                    // when it is within two ulps, otherwise stop with DtLessThanMin.
+                    if (tn == t) {
+                        if (tf - t <= 2.0f * (deg_next_up(t) - t)) tn = tf;
+                        else status = DEG_RET_DTMIN;
+                    }
+#endif
+                    t = tn;
+                }
+                dt = dtnew;
+                ++nacc;
+                accepted = true;
+            }
+        }
+#endif
+        if (accepted) {
+            // ---- savevalues! (integrator_utils.jl:13-50)
+            if (use_saveat) {
+#if DEG_HAS_INTERP
+                while (tnext <= t) {
+                    float out[DEG_N];
+                    const float th = (tnext - tprev) / dtacc;
+                    deg_interp(th, dtacc, tprev, p, uprev, k, out);
+                    if (cur < a.nsave) {
+                        DEG_FOR(c) a.us[((long long)cur * DEG_N + c) * S + i] = out[c];
+                        a.ts[(long long)cur * S + i] = tnext;
+                    }
+                    ++cur;
+                    tnext = (cur < a.nsaveat) ? svp[cur] : finf;
+                }
+#endif
+            } else if (a.save_everystep) {
+#if !DEG_ADAPTIVE
+                if (step_idx < a.nsave) {
+                    DEG_FOR(c) a.us[((long long)step_idx * DEG_N + c) * S + i] = u[c];
+                    a.ts[(long long)step_idx * S + i] = t;
+                }
+                ++step_idx;
+#endif
+            }
+            fsal_pending = true;
+        }
+        if (!done) done = (status != DEG_RET_SUCCESS) || !(t < tf);
 
         if (done) {
             // ---- endpoint rules (kernels.jl:53-62 / :133-142) and unsaved rows
@@ -819,7 +933,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
             break;
 #endif
         }
-        }  // lanes holding a trajectory
+        }  // phase B
     }
 }
 

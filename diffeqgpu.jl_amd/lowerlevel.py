@@ -112,7 +112,10 @@ _AUTOTUNE_BLOCK = {}
 
 def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, abstol, reltol, saveat_np,
             nsave, save_everystep, t0_fill, timing=False, stream=None, mlp=None, steplog=None, block_size=None,
-            grid_blocks=0):
+            grid_blocks=0, outputs=None):
+    """Launch one solve. ``outputs`` (optional) supplies preallocated device tensors
+    {"ts", "us_soa", "retcode", "naccept", "nreject", "saveat"} so that the call allocates and
+    copies nothing (required inside a graph capture)."""
     import torch
 
     if not isinstance(alg, GPUODEAlgorithm):
@@ -131,14 +134,35 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
     kern = ctx.kernel(src, probs.n, probs.np, alg.method, flags)
     device = probs.u0.device
     N, n = probs.N, probs.n
-    ts = torch.empty((nsave, N), dtype=torch.float32, device=device)
-    us_soa = torch.empty((nsave, n, N), dtype=torch.float32, device=device)
-    retcode = torch.empty(N, dtype=torch.int32, device=device)
-    nacc = torch.empty(N, dtype=torch.int32, device=device)
-    nrej = torch.empty(N, dtype=torch.int32, device=device)
+    if outputs is None:
+        outputs = {}
+    ts = outputs.get("ts")
+    if ts is None:
+        ts = torch.empty((nsave, N), dtype=torch.float32, device=device)
+    us_soa = outputs.get("us_soa")
+    if us_soa is None:
+        us_soa = torch.empty((nsave, n, N), dtype=torch.float32, device=device)
+    retcode, nacc, nrej = (outputs.get(k) for k in ("retcode", "naccept", "nreject"))
+    if retcode is None:
+        retcode = torch.empty(N, dtype=torch.int32, device=device)
+    if nacc is None:
+        nacc = torch.empty(N, dtype=torch.int32, device=device)
+    if nrej is None:
+        nrej = torch.empty(N, dtype=torch.int32, device=device)
+    if tuple(ts.shape) != (nsave, N) or tuple(us_soa.shape) != (nsave, n, N):
+        raise ValueError("preallocated ts/us have the wrong shape")
+    for t in (ts, us_soa, retcode, nacc, nrej):
+        if not t.is_contiguous() or t.device != device:
+            raise ValueError("outputs must be contiguous tensors on the problems' device")
+    if retcode.numel() != N or nacc.numel() != N or nrej.numel() != N or retcode.dtype != torch.int32:
+        raise ValueError("retcode/naccept/nreject must be int32 tensors of length N")
     sv = None
     if saveat_np is not None:
-        sv = torch.as_tensor(np.ascontiguousarray(saveat_np, dtype=np.float32)).to(device)
+        sv = outputs.get("saveat")
+        if sv is None:
+            sv = torch.as_tensor(np.ascontiguousarray(saveat_np, dtype=np.float32)).to(device)
+        elif sv.numel() != len(saveat_np):
+            raise ValueError("preallocated saveat has the wrong length")
     a = _lib.SolveArgs()
     a.n = n
     a.n_traj = N

@@ -31,6 +31,11 @@ def variants():
             src, _ = f.hip_source(n, np_, m == _lib.DEG_RB23)
             for dev, adaptive in devs:
                 out.append((name, src, n, np_, m, dev.flags(adaptive)))
+            if m != _lib.DEG_RB23:
+                # the learned proposer and the step-log writer (default adaptive flags)
+                base = HIPDevice().flags(True)
+                for extra in (_lib.F_LEARNED, _lib.F_STEPLOG, _lib.F_LEARNED | _lib.F_STEPLOG):
+                    out.append((name, src, n, np_, m, base | extra))
     return out
 
 

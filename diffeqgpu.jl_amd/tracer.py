@@ -489,3 +489,24 @@ def evaluate(f: Callable, u, p, t):
     """Evaluate an RHS written for the tracer on plain numbers (float64), for host checks."""
     out = f(list(u), list(p), t)
     return [float(x) for x in out]
+
+
+def flop_counts(f: Callable, n: int, np_: int, jac: Callable | None = None, tgrad: Callable | None = None) -> dict:
+    """Flops of one RHS evaluation (F), of the Jacobian (FJ) and of the time gradient (FT), as
+    emitted (common subexpressions counted once, products of the Jacobian's own DAG only)."""
+    tr = trace(f, n, np_, nout=n)
+    F = _emit_body(tr.g, tr.outs, "du")[1]
+    J, T = jacobian_nodes(tr)
+    if jac is not None:
+        g, u, p, t = _new_inputs(n, np_)
+        rows = _flatten_matrix(jac(SymVec(u, "u"), SymVec(p, "p"), t), n)
+        jt = Trace(g, u, p, t, [g.lift(x) for x in rows])
+        FJ = _emit_body(jt.g, jt.outs, "J")[1]
+    else:
+        FJ = _emit_body(tr.g, [x for row in J for x in row], "J")[1]
+    if tgrad is not None:
+        tt = trace(tgrad, n, np_, nout=n)
+        FT = _emit_body(tt.g, tt.outs, "dT")[1]
+    else:
+        FT = _emit_body(tr.g, T, "dT")[1]
+    return dict(F=F, FJ=FJ, FT=FT)

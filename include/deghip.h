@@ -142,6 +142,71 @@ const char *deg_version(void);
 /* sizeof(deg_solve_args) as compiled into the library, for bindings to check their layout. */
 uint32_t deg_solve_args_size(void);
 
+/* ---- parameter sweeps (no counterpart in the reference: its sweeps re-run solve() from the host) ---- */
+
+/* Seeded uniform draws on the device: out[i] = lo + scale * U(seed, (id_offset + i) * rows + row)
+ * for i in [0, n), U the SplitMix64 counter-based U[0,1) Float32 (diffeqgpu.jl_amd/rng.py draws
+ * the same numbers on the host). Used to re-draw ensemble parameters for each sweep without a
+ * host round trip; asynchronous on `stream`. Takes the place of the host-side `prob_func`
+ * re-draws of an EnsembleProblem (src/solve.jl:187-202) for sweeps that are graph-captured. */
+int deg_draw_uniform(float *out, int64_t n, int64_t id_offset, uint64_t seed, int32_t rows, int32_t row,
+                     float lo, float scale, void *stream);
+
+typedef struct deg_graph deg_graph;
+/* Capture every launch made on `stream` (non-NULL) by this thread between begin and end --
+ * deg_draw_uniform, deg_solve, and any other stream work -- into an instantiated HIP graph.
+ * Inside a capture deg_solve must not request elapsed_ms. The captured work contains no
+ * memset nodes (the persistent queue is cleared by a kernel), so replays are exact. */
+int deg_graph_begin(void *stream);
+int deg_graph_end(void *stream, deg_graph **out);
+/* Replay the captured work on `stream` (asynchronous). Replays on one stream are ordered; the
+ * persistent kernels' work counters are per compiled kernel, so two replays that use the same
+ * kernel must not run concurrently on different streams. */
+int deg_graph_launch(deg_graph *g, void *stream);
+int deg_graph_num_nodes(deg_graph *g, int64_t *count);
+int deg_graph_destroy(deg_graph *g);
+
+/* ---- learned step-size proposer (no counterpart in the reference, whose only controller is
+ * the PI controller of integrator_utils.jl:1-11 and gpu_*_perform_step.jl; the PI controller
+ * stays the fallback, and the embedded error estimate still accepts or rejects every step) ---- */
+
+/* Step-log rows written by kernels compiled with DEG_F_STEPLOG: one row per attempt n that has
+ * a successor, [x0..x5 at n, log2(dt_{n+1}/dt_n), log2 EEst_{n+1}, z_PI at n, accepted at n, 0, 0],
+ * with x0 = log2 EEst_n, x1 = log2 EEst_{n-1}, x2 = log2 dt_n, x3 = log2 ||u||, x4 = log2 ||f(u)||,
+ * x5 = log2(dt_n / dt_{n-1}) (RMS norms) and z_PI = log2 of the PI controller's step factor.
+ * Rows past steplog_cap are dropped; the counter keeps counting. The learned solver (DEG_F_LEARNED)
+ * steps with log2(dt_{n+1}/dt_n) = clamp(z_PI + MLP(x), log2 qmin, log2 qmax). */
+#define DEG_STEPLOG_COLS 12
+#define DEG_MLP_NPARAM 1313
+
+/* sizeof the packed BF16 inference weights (the `mlp` argument of deg_solve_args). */
+uint32_t deg_mlp_packed_size(void);
+/* Columns of one partial-gradient row (DEG_MLP_NPARAM gradients, loss sum, row count). */
+int32_t deg_mlp_partial_cols(void);
+
+/* FP32 master parameters (DEG_MLP_NPARAM floats): W1[6][32], b1[32], W2[32][32], b2[32], w3[32], b3,
+ * with h1 = relu(x' W1 + b1), h2 = relu(h1 W2 + b2), z = h2 . w3 + b3 and x' = (x - mean) * inv_std
+ * (norm = [mean[6], inv_std[6]]). */
+
+/* Partial gradients of the mean squared error over `nrows` step-log rows (optionally gathered
+ * through idx) into partial[nblocks][deg_mlp_partial_cols()]; the target is the correction to the
+ * PI factor that would have hit the error ratio rho: y = clamp(col6 + expo * (log2_rho - col7),
+ * -ycap, ycap) - col8. */
+int deg_mlp_grad(const float *rows, const int32_t *idx, int64_t nrows, const float *params, const float *norm,
+                 float expo, float log2_rho, float ycap, float *partial, int32_t nblocks, void *stream);
+/* One Adam step from the partial gradients (summed in block order); loss_out (optional, device)
+ * receives the batch's mean squared error. step is 1-based. */
+int deg_mlp_adam(float *params, float *m, float *v, const float *partial, int32_t nblocks, int32_t step, float lr,
+                 float beta1, float beta2, float eps, float *loss_out, void *stream);
+/* FP32 forward pass over n feature rows of leading dimension ld (>= 6). */
+int deg_mlp_forward(const float *params, const float *norm, const float *x, int64_t ld, int64_t n, float *out,
+                    void *stream);
+/* Pack FP32 master parameters into the BF16 matrix-core layout the solver reads (device). */
+int deg_mlp_pack(const float *params, const float *norm, float zlo, float zhi, void *packed, void *stream);
+/* The solver's matrix-core evaluation over n feature rows (BF16 weights and activations, FP32
+ * accumulation), for tests and measurement. */
+int deg_mlp_infer(const void *packed, const float *x, int64_t ld, int64_t n, float *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

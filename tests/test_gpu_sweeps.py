@@ -1,0 +1,115 @@
+"""GPU: device-side parameter draws and graph-captured sweeps (the uneven 4M Lorenz Vern7
+workload's 10-sweep replay, at a size the oracle finishes in seconds).
+
+Bars:
+  * device draws equal the host generator (rng.uniform_table, Float32 lo + scale * U) bit for
+    bit, for every row of a 4-row table and with an id offset;
+  * every graph replay equals the eager run of the same sweeps bit for bit (outputs and drawn
+    parameters). The replays are separated by other eager GPU work and synchronisations;
+  * with the deterministic controller pow and no contraction, each sweep's results equal the
+    CPU oracle on the host-drawn parameters of that sweep, bit for bit.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+UNEVEN_DRAWS = [("p", 0, 5.0, 10.0), ("p", 1, 0.0, 60.0), ("p", 2, 1.0, 3.0), ("tfs", 0, 1.0, 49.0)]
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+    return torch
+
+
+def host_uneven(seed, N):
+    from diffeqgpu_amd import rng
+
+    U = rng.uniform_table(seed, 4, N)
+    p = np.stack([F32(5.0) + F32(10.0) * U[0], F32(60.0) * U[1], F32(1.0) + F32(3.0) * U[2]]).astype(F32)
+    tfs = (F32(1.0) + F32(49.0) * U[3]).astype(F32)
+    return p, tfs
+
+
+def test_device_draws_match_host(deg, torch_cuda):
+    torch = torch_cuda
+    from diffeqgpu_amd import _lib, rng
+
+    N, off = 100_003, 777
+    out = torch.empty(N, dtype=torch.float32, device="cuda")
+    st = torch.cuda.current_stream()
+    for seed in (0, 303, 2 ** 63 + 12345):
+        for row, (lo, scale) in enumerate([(5.0, 10.0), (0.0, 60.0), (1.0, 3.0), (-2.0, 4.0)]):
+            print(f"[gpu] draw seed={seed} row={row}", flush=True)
+            _lib.draw_uniform(out.data_ptr(), N, seed, 4, row, lo, scale, st.cuda_stream, id_offset=off)
+            torch.cuda.synchronize()
+            U = rng.uniform_table(seed, 4, off + N)[row, off:]
+            want = (F32(lo) + F32(scale) * U).astype(F32)
+            np.testing.assert_array_equal(out.cpu().numpy(), want)
+
+
+def _snapshot(sw):
+    return [{k: v.cpu().numpy().copy() for k, v in o.items() if v is not None} for o in sw.outputs]
+
+
+def test_sweep_graph_replays_equal_eager_and_oracle(deg, torch_cuda):
+    torch = torch_cuda
+    from diffeqgpu_amd import systems
+
+    f, n, np_, _ = systems.SYSTEMS["lorenz"]
+    N, sweeps, seed = 3000, 3, 303
+    dev = deg.HIPDevice(contract="off", powmode="det", kahan=True, persistent=True)
+    p0, tfs0 = host_uneven(seed, N)
+    u0 = np.zeros((3, N), F32)
+    u0[0] = 1.0
+    probs = deg.HIPProbVector.from_arrays(f, u0, np.zeros_like(p0), (0.0, 50.0), dev, tfs=np.ones(N, F32))
+    prob = deg.ODEProblem(f, u0[:, 0], (0.0, 50.0), p0[:, 0])
+    print("[gpu] sweep capture N=3000 sweeps=3 vern7 kahan persistent", flush=True)
+    sw = deg.ParameterSweep(probs, prob, deg.GPUVern7(), sweeps=sweeps, seed=seed, draws=UNEVEN_DRAWS,
+                            dt=0.01, abstol=1e-8, reltol=1e-8, save_everystep=False)
+    assert sw.num_nodes() >= sweeps * 5
+    eager = _snapshot(sw)
+
+    # each sweep against the oracle on host-drawn parameters
+    for k in range(sweeps):
+        pk, tk = host_uneven(seed + k, N)
+        sl = slice(0, 400)
+        o = pyoracle.solve("lorenz", "vern7", True, u0[:, sl], pk[:, sl], (0.0, 50.0), 0.01, abstol=1e-8, reltol=1e-8,
+                           tfs=tk[sl], kahan=True, detpow=True)
+        np.testing.assert_array_equal(eager[k]["ts"][:, sl], o["ts"])
+        np.testing.assert_array_equal(eager[k]["us_soa"][:, :, sl], o["us"])
+        np.testing.assert_array_equal(eager[k]["naccept"][sl], o["naccept"])
+        np.testing.assert_array_equal(eager[k]["retcode"][sl], o["retcode"])
+
+    # replays, with unrelated eager GPU work and synchronisations between them
+    for rep in range(3):
+        for o in sw.outputs:
+            for v in o.values():
+                if v is not None and v.dtype == torch.float32 and v.dim() > 1:
+                    v.fill_(float("nan"))
+        scratch = torch.randn(1 << 20, device="cuda")
+        (scratch * 2.0).sum().item()
+        print(f"[gpu] replay {rep}", flush=True)
+        sw.replay()
+        sw.synchronize()
+        got = _snapshot(sw)
+        for k in range(sweeps):
+            for key in eager[k]:
+                np.testing.assert_array_equal(got[k][key], eager[k][key], err_msg=f"replay {rep} sweep {k} {key}")
+        p_last, tfs_last = host_uneven(seed + sweeps - 1, N)
+        np.testing.assert_array_equal(probs.p.cpu().numpy(), p_last)
+        np.testing.assert_array_equal(probs.tfs.cpu().numpy(), tfs_last)
+
+
+def test_graph_capture_rejects_timing_and_null_stream(deg, torch_cuda):
+    from diffeqgpu_amd import _lib
+
+    with pytest.raises(_lib.DegError):
+        _lib.Graph.begin(0)

@@ -218,6 +218,33 @@ def test_vern9_matches_dop853():
         assert np.abs(out[:, :, i] - s.y.T).max() < 1e-8 * scale
 
 
+def test_robertson_rb23_vs_radau_and_mass():
+    """Rosenbrock23 with the FP64 W factorisation on the stiff Robertson kinetics (the 1e6
+    stiff workload's problem and tolerances: reltol 1e-4, abstol 1e-8, 20 log-spaced save
+    times to 1e5) against scipy Radau at rtol 1e-12, and the linear invariant y1 + y2 + y3 = 1
+    that every Rosenbrock step preserves up to rounding."""
+    sv = (10.0 ** (-4 + 9 * np.arange(20) / 19)).astype(F32)
+    sv[-1] = F32(1e5)
+    ks = np.array([[0.04, 0.05, 0.03], [3e7, 2e7, 4.5e7], [1e4, 1.3e4, 0.8e4]], F32)
+    u0 = np.zeros((3, 3), F32)
+    u0[0] = 1
+    r = pyoracle.solve("robertson", "rb23", True, u0, ks, (0.0, 1e5), 1e-6, abstol=1e-8, reltol=1e-4, saveat=sv)
+    assert np.all(r["retcode"] == 0)
+    for i in range(3):
+        k = ks[:, i].astype(np.float64)
+
+        def f(t, y):
+            return [-k[0] * y[0] + k[2] * y[1] * y[2], k[0] * y[0] - k[1] * y[1] ** 2 - k[2] * y[1] * y[2],
+                    k[1] * y[1] ** 2]
+
+        s = solve_ivp(f, (0, 1e5), [1.0, 0.0, 0.0], method="Radau", rtol=1e-12, atol=1e-20,
+                      t_eval=sv.astype(np.float64))
+        got = r["us"][:, :, i].astype(np.float64)
+        err = np.abs(got - s.y.T).max(axis=0)
+        assert err[0] < 1e-4 and err[2] < 1e-4 and err[1] < 1e-7, err
+        assert np.abs(got.sum(axis=1) - 1.0).max() < 1e-5
+
+
 def test_vern9_vdp_matches_radau():
     mu = np.array([[0.5, 5.0]], F32)
     u0 = np.array([[1.5, -1.0], [0.0, 0.5]], F32)
@@ -342,3 +369,49 @@ def test_no_progress_rule_ends_every_lane():
     r = pyoracle.solve("growth", "tsit5", False, np.array([[1.0]], F32), np.array([[1.0]], F32), (1e6, 1e6 + 1.0),
                        1e-3)
     assert r["retcode"][0] == 1 and r["naccept"][0] == 0
+
+
+# ---------------------------------------------------------------------------------------
+# the workload fixtures (tests/golden/) against the Float32 restatement at the workloads'
+# own tolerances: pins the fixtures and the oracle together at benchmark scale
+# ---------------------------------------------------------------------------------------
+def _workload(name, N=1000):
+    from diffeqgpu_amd import workloads
+
+    return workloads.config(name, N=N)
+
+
+def _golden(name):
+    import os
+
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name))
+
+
+def test_robertson_fixture_vs_oracle():
+    cfg, gold = _workload("robertson"), _golden("robertson_radau.npz")
+    r = pyoracle.solve("robertson", "rb23", True, cfg["u0"], cfg["p"], cfg["tspan"], cfg["dt"], abstol=1e-8,
+                       reltol=1e-4, saveat=np.asarray(cfg["saveat"], F32))
+    assert np.all(r["retcode"] == 0)
+    err = np.abs(r["us"].astype(np.float64) - gold["u"]).max(axis=(0, 1))
+    assert err.max() < 1e-4, err.max()
+
+
+def test_vdp_fixture_vs_oracle():
+    cfg, gold = _workload("vdp"), _golden("vdp_vern9.npz")
+    r = pyoracle.solve("vdp", "tsit5", True, cfg["u0"], cfg["p"], cfg["tspan"], cfg["dt"], abstol=1e-6, reltol=1e-6)
+    assert np.all(r["retcode"] == 0)
+    err = np.abs(r["us"][-1].astype(np.float64) - gold["u"][0]).max(axis=0)
+    assert np.median(err) < 5e-5 and np.percentile(err, 99) < 2e-3, (np.median(err), np.percentile(err, 99))
+
+
+def test_lorenz_uneven_fixture_vs_oracle():
+    # the non-chaotic part (rho < 10: every trajectory settles on a fixed point) must agree
+    # closely; chaotic trajectories only in distribution
+    cfg, gold = _workload("lorenz_uneven"), _golden("lorenz_uneven_vern9.npz")
+    r = pyoracle.solve("lorenz", "vern7", True, cfg["u0"], cfg["p"], cfg["tspan"], cfg["dt"], abstol=1e-8,
+                       reltol=1e-8, tfs=cfg["tfs"], kahan=True)
+    assert np.all(r["retcode"] == 0)
+    err = np.abs(r["us"][-1].astype(np.float64) - gold["u"][0]).max(axis=0)
+    calm = cfg["p"][1] < 10
+    assert calm.sum() > 100 and err[calm].max() < 1e-4, err[calm].max()
+    assert np.median(err) < 0.05
