@@ -75,6 +75,8 @@ def parse():
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick block size, register cap, refill and chunk by timing (before the timed region)")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -187,6 +189,12 @@ def main():
     alg = getattr(deg, spec["alg"])()
     prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
     kw = solve_kw(cfg)
+    tuned = None
+    if args.autotune:
+        tprobs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev, tfs=cfg.get("tfs"))
+        dev = deg.autotune.tune(tprobs, prob, alg, **kw)
+        tuned = dict(block_size=dev.block_size, waves=dev.waves, refill=dev.refill, chunk=dev.chunk)
+        del tprobs
 
     if name == "lorenz_uneven":
         sweeps = spec["sweeps"]
@@ -253,7 +261,8 @@ def main():
         "data": "synthetic (seeded SplitMix64 draws of the workload's parameter law)",
         "config": {"workload": spec["desc"], "trajectories_per_step": per_step, "controller": "PI",
                    "kernel_flags": results[0].flags, "contract": args.contract, "powmode": args.powmode,
-                   "persistent": bool(args.persistent), "block_size": args.block_size or 256},
+                   "persistent": bool(args.persistent), "block_size": dev.block_size or 256,
+                   "waves": dev.waves, "refill": dev.refill or 4, "chunk": dev.chunk or 16, "autotuned": tuned},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,

@@ -12,6 +12,7 @@ from .problems import (EnsembleProblem, HIPProbVector, ImmutableODEProblem, ODEF
                        get_backend, make_prob_compatible, remake)
 from .ranges import colon, linrange
 from .sweeps import Draw, ParameterSweep
+from . import autotune, learned
 from .tracer import SVector
 from . import tracer as math  # sin, cos, exp, log, sqrt, tanh, fabs, pow, ifelse on traced values
 
@@ -20,5 +21,5 @@ __all__ = [
     "HIPDevice", "alg_order", "EnsembleContext", "EnsembleSolution", "ODESolution", "solve", "SolveResult",
     "vectorized_asolve", "vectorized_solve", "EnsembleProblem", "HIPProbVector", "ImmutableODEProblem",
     "ODEFunction", "ODEProblem", "adapt", "get_backend", "make_prob_compatible", "remake", "colon", "linrange",
-    "SVector", "math", "Draw", "ParameterSweep",
+    "SVector", "math", "Draw", "ParameterSweep", "autotune", "learned",
 ]

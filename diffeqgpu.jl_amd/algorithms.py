@@ -83,6 +83,8 @@ class HIPDevice:
       refill      persistent grids: a wave takes new trajectories once this many of its lanes
                   are idle (1..64; 0: 4).
       chunk       persistent grids: trajectory ids a wave claims per queue atomic (0: 16).
+      waves       register cap: compile the solver for at least this many resident waves per
+                  SIMD (1..7; 0: the compiler's choice).
     """
 
     device: int = 0
@@ -94,6 +96,7 @@ class HIPDevice:
     max_attempts: int = 0
     refill: int = 0
     chunk: int = 0
+    waves: int = 0
 
     def flags(self, adaptive: bool) -> int:
         f = 0
@@ -115,6 +118,9 @@ class HIPDevice:
         persistent = self.persistent if self.persistent is not None else adaptive
         if persistent:
             f |= _lib.F_PERSISTENT
+        if not 0 <= self.waves <= 7:
+            raise ValueError("waves must be in 0..7")
+        f |= self.waves << _lib.F_WAVES_SHIFT
         return f
 
 

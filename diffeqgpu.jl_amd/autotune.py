@@ -1,0 +1,116 @@
+"""Launch-configuration autotuner.
+
+Picks, per (right-hand side, method, solver flags, ensemble size), the configuration with the
+lowest measured kernel time:
+  block size       threads per block (64, 128, 192, 256);
+  register cap     DEG_F_WAVES: the solver compiled for at least w waves per SIMD (0 = the
+                   compiler's own allocation, 4..7 = capped, possibly spilling);
+  refill batch     persistent grids: idle lanes per wave that trigger a refill;
+  chunk            persistent grids: trajectory ids claimed per queue atomic.
+The LDS depth of the saveat cache needs no tuning: it is sized to the save grid at every launch.
+
+The search is coordinate descent in that order, starting from the defaults. Each candidate is
+timed with HIP events around the kernel (the best of ``repeats`` launches) on the caller's
+problems, so the result reflects the real trajectory mix. Results are cached in
+``_jitcache/autotune.json``, keyed by the traced source, n, np, method, flags and the ensemble
+size rounded to a power of two. The reference has no tuner: KernelAbstractions picks its own
+workgroup size.
+"""
+from __future__ import annotations
+
+import copy
+import dataclasses
+import hashlib
+import json
+import math
+import os
+
+from . import _lib
+from .algorithms import GPURosenbrock23, HIPDevice
+
+CACHE_PATH = os.path.join(_lib.CACHE_DIR, "autotune.json")
+BLOCKS = (64, 128, 192, 256)
+WAVES = (0, 4, 5, 6, 7)
+REFILL = (1, 2, 4, 8, 16)
+CHUNK = (4, 16, 64)
+
+
+def _key(probs, alg, adaptive, nsave_hint) -> str:
+    src, _ = probs.f.hip_source(probs.n, probs.np, isinstance(alg, GPURosenbrock23))
+    h = hashlib.sha1(src.encode()).hexdigest()[:16]
+    base = dataclasses.replace(probs.dev, block_size=None, refill=0, chunk=0, waves=0)
+    nb = int(round(math.log2(max(1, probs.N))))
+    return f"{h}:{probs.n}:{probs.np}:{alg.method}:{base.flags(adaptive)}:{nb}:{nsave_hint}"
+
+
+def _load() -> dict:
+    try:
+        with open(CACHE_PATH) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(table: dict):
+    os.makedirs(os.path.dirname(CACHE_PATH), exist_ok=True)
+    tmp = CACHE_PATH + f".tmp{os.getpid()}"
+    with open(tmp, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    os.replace(tmp, CACHE_PATH)
+
+
+def _with_dev(probs, dev):
+    q = copy.copy(probs)
+    q.dev = dev
+    return q
+
+
+def tune(probs, prob, alg, *, adaptive: bool = True, repeats: int = 2, use_cache: bool = True, log=None,
+         **solve_kw) -> HIPDevice:
+    """Returns the fastest HIPDevice for this ensemble (also stored in the cache)."""
+    from .lowerlevel import vectorized_asolve, vectorized_solve
+
+    fn = vectorized_asolve if adaptive else vectorized_solve
+    nsave_hint = "saveat" if solve_kw.get("saveat") is not None else ("every" if solve_kw.get("save_everystep") else "end")
+    key = _key(probs, alg, adaptive, nsave_hint)
+    table = _load() if use_cache else {}
+    if key in table:
+        return dataclasses.replace(probs.dev, **table[key]["config"])
+
+    persistent = bool(probs.dev.flags(adaptive) & _lib.F_PERSISTENT)
+    trials = {}
+
+    def measure(cfg):
+        k = tuple(sorted(cfg.items()))
+        if k not in trials:
+            dev = dataclasses.replace(probs.dev, **cfg)
+            q = _with_dev(probs, dev)
+            best = math.inf
+            for _ in range(repeats + 1):  # the first launch also loads (or JIT-compiles) the variant
+                r = fn(q, prob, alg, timing=True, **solve_kw)
+                best = min(best, r.kernel_ms)
+            trials[k] = best
+            if log:
+                log(f"[autotune] {cfg} {best:.3f} ms")
+        return trials[k]
+
+    cfg = dict(block_size=256, waves=0, refill=4 if persistent else 0, chunk=16 if persistent else 0)
+    axes = [("block_size", BLOCKS), ("waves", WAVES)]
+    if persistent:
+        axes += [("refill", REFILL), ("chunk", CHUNK)]
+    for name, values in axes:
+        best_v, best_t = cfg[name], measure(cfg)
+        for v in values:
+            c = dict(cfg, **{name: v})
+            t = measure(c)
+            if t < best_t:
+                best_v, best_t = v, t
+        cfg[name] = best_v
+    table = _load() if use_cache else table
+    table[key] = dict(config=cfg, kernel_ms=trials[tuple(sorted(cfg.items()))],
+                      default_ms=trials.get(tuple(sorted(dict(block_size=256, waves=0, refill=4 if persistent else 0,
+                                                              chunk=16 if persistent else 0).items()))),
+                      trials=len(trials))
+    if use_cache:
+        _store(table)
+    return dataclasses.replace(probs.dev, **cfg)

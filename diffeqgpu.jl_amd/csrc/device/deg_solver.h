@@ -18,7 +18,10 @@
 //   DEG_KAHAN                    compensated accumulation of t
 //   DEG_PERSISTENT               lanes refill from a device work counter
 //   DEG_LEARNED                  MLP step-size proposal
-//   DEG_SAVEAT_GLOBAL            read saveat times from global memory (more than DEG_SAVEAT_LDS)
+//   DEG_SAVEAT_GLOBAL            read saveat times from global memory (more than fit the LDS)
+//   DEG_WAVES_MIN                register cap: at least this many waves per SIMD (0: none)
+//   DEG_PARITY                   1: the reference's exact Float32 operations (no contraction, IEEE
+//                                division in the dense-output abscissa)
 //   DEG_KERNEL                   kernel symbol name
 //   DEG_BLOCK_MAX                launch-bounds block size
 // and the functions deg_rhs(u, p, t, du) (+ deg_jac, deg_tgrad for Rosenbrock23).
@@ -478,21 +481,27 @@ __device__ __forceinline__ float deg_advance(float t, float dt, float& comp) {
 // The ensemble kernel (kernels.jl:1-152), one trajectory per lane, one attempt per
 // loop iteration so that lanes stay in step and a finished lane can be refilled.
 // ------------------------------------------------------------------------------------
-#ifndef DEG_SAVEAT_LDS
-#define DEG_SAVEAT_LDS 4096
+#ifndef DEG_WAVES_MIN
+#define DEG_WAVES_MIN 0
+#endif
+#if DEG_WAVES_MIN > 0
+// register cap: at least DEG_WAVES_MIN resident waves per SIMD
+#define DEG_ATTR_WAVES __attribute__((amdgpu_waves_per_eu(DEG_WAVES_MIN)))
+#else
+#define DEG_ATTR_WAVES
 #endif
 #ifndef DEG_SAVEAT_GLOBAL
 #define DEG_SAVEAT_GLOBAL 0
 #endif
 
-extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg_kargs a) {
+extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_KERNEL(const deg_kargs a) {
     const long long S = a.stride;
     // saveat times staged in LDS: reading them then needs no vector-memory wait (which on
     // CDNA would also wait for every store still in flight)
 #if DEG_SAVEAT_GLOBAL
     const float* __restrict__ svp = a.saveat;  // more save points than the LDS cache holds
 #else
-    __shared__ float svp[DEG_SAVEAT_LDS];
+    extern __shared__ float svp[];  // nsaveat floats, sized at launch
     if (a.saveat != nullptr) {
         for (int j = threadIdx.x; j < a.nsaveat; j += blockDim.x) svp[j] = a.saveat[j];
     }
@@ -514,6 +523,8 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
     float t = 0.0f, tf = 0.0f, tprev = 0.0f, dt = 0.0f, dtacc = 0.0f, tc = 0.0f, tnext = finf;
     float qold = DEG_QOLDINIT, qpow = 1.0f;
     int cur = 0, step_idx = 1, nacc = 0, nrej = 0, status = DEG_RET_SUCCESS;
+    float* ts_w = nullptr;  // this lane's ts/us entries of save row `cur`
+    float* us_w = nullptr;
     long long attempts = 0;
     bool have_k = false;
     bool fsal_pending = false;
@@ -638,6 +649,8 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
                     cur = 1;
                 }
                 tnext = (cur < a.nsaveat) ? svp[cur] : finf;
+                ts_w = a.ts + (long long)cur * S + i;
+                us_w = a.us + (long long)cur * DEG_N * S + i;
             } else {
                 a.ts[i] = t;
                 DEG_FOR(c) a.us[(long long)c * S + i] = u[c];
@@ -648,10 +661,10 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
 #endif
         }
 
-        // Each iteration is one attempt (or one fixed step) per lane holding a trajectory, in
-        // two phases: (A) the stages and the error estimate, (B) the step controller, the
-        // saves and the endpoint rules. Between them the learned proposer (DEG_LEARNED) runs
-        // on the matrix cores with all lanes of the wavefront.
+        // Each iteration is one attempt (or one fixed step) per lane holding a trajectory:
+        // (A) the stages and the error estimate, (B) the step controller, the saves and the
+        // endpoint rules. In the learned variant the lane block is closed between A and B so
+        // that the proposer runs on the matrix cores with all lanes of the wavefront.
         bool done = false;
         bool accepted = false;
 #if DEG_ADAPTIVE
@@ -661,112 +674,13 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
 #if DEG_LEARNED || DEG_STEPLOG
         float feat[DEG_MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #endif
-#if DEG_PERSISTENT
-        if (have) {
-#else
-        {
-#endif
-        done = !(t < tf);
-        if (!done) {
-            if (attempts >= a.max_attempts) {
-                status = DEG_RET_MAXITERS;
-            }
-#if DEG_ADAPTIVE
-            else if (dt < DEG_DTMIN) {
-                status = DEG_RET_DTMIN;
-            }
-#endif
-            else {
-                ++attempts;
-#if DEG_ADAPTIVE
-                // ---- one attempt of the `while EEst > 1` loop: stages and error estimate
-#if DEG_METHOD == 0
-                if (!have_k) {
-                    deg_rhs(u, p, t, k[1]);
-                    have_k = true;
-                } else if (fsal_pending) {
-                    DEG_FOR(c) k[1][c] = k[7][c];
-                }
-                fsal_pending = false;
-                deg_attempt(u, p, t, dt, k, unew);
-                EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
-#elif DEG_METHOD == 1
-                deg_rhs(u, p, t, k[1]);
-                deg_attempt(u, p, t, dt, k, unew);
-                EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
-#else
-                EEst = deg_rb23<true>(u, p, t, dt, k, unew, a.abstol, a.reltol);
-#endif
-                attempted = true;
-#if DEG_LEARNED || DEG_STEPLOG
-                {
-                    // proposer features (base-2 logs; EEst holds EEst^2 on this path)
-                    float su = 0.0f, sf = 0.0f;
-                    DEG_FOR(c) {
-                        su = __builtin_fmaf(u[c], u[c], su);
-                        sf = __builtin_fmaf(k[1][c], k[1][c], sf);
-                    }
-                    feat[0] = 0.5f * __builtin_amdgcn_logf(__builtin_fmaxf(EEst, 1e-30f));
-                    feat[1] = lx_prev;
-                    feat[2] = __builtin_amdgcn_logf(dt);
-                    feat[3] = 0.5f * __builtin_amdgcn_logf(__builtin_fmaxf(su * (1.0f / DEG_N), 1e-30f));
-                    feat[4] = 0.5f * __builtin_amdgcn_logf(__builtin_fmaxf(sf * (1.0f / DEG_N), 1e-30f));
-                    feat[5] = first_attempt ? 0.0f : feat[2] - ldt_prev;
-                }
-#endif
-#else
-                // ---- one fixed step (gpu_*_perform_step.jl fixed versions)
-                const float told = t;
-                const float tcomp_old = tc;
-                const float tn = deg_advance(t, dt, tc);
-#if !DEG_KAHAN
-                if (tn == told) {
-                    // the step is below the resolution of t: stop instead of looping forever
-                    status = DEG_RET_DTMIN;
-                    tc = tcomp_old;
-                    --attempts;
-                } else
-#endif
-                {
-                DEG_FOR(c) uprev[c] = u[c];
-                tprev = told;
-                dtacc = dt;
-                t = tn;
-#if DEG_METHOD == 0
-                if (!have_k) {
-                    deg_rhs(uprev, p, told, k[1]);
-                    have_k = true;
-                } else {
-                    DEG_FOR(c) k[1][c] = k[7][c];
-                }
-                deg_attempt(uprev, p, told, dt, k, unew);
-#elif DEG_METHOD == 1
-                deg_rhs(uprev, p, told, k[1]);
-                deg_attempt(uprev, p, told, dt, k, unew);
-#else
-                deg_rb23<false>(uprev, p, told, dt, k, unew, 0.0f, 0.0f);
-#endif
-                DEG_FOR(c) u[c] = unew[c];
-                ++nacc;
-                accepted = true;
-                }
-#endif
-            }
-        }
-        }  // phase A
-
 #if DEG_LEARNED
-        // every lane of the wavefront is here (the loop exit is wave-uniform in this variant)
-        const float zml = deg_mlp_eval((const deg_mlp_packed*)a.mlp, feat);
-#endif
-
-#if DEG_PERSISTENT
-        if (have) {
-#else
-        {
+        float zml = 0.0f;
 #endif
 #if DEG_ADAPTIVE
-        if (attempted) {
+        // ---- step-size control and acceptance (gpu_*_perform_step.jl adaptive versions); run
+        // right after the error estimate, or after the proposer in the learned variant
+        auto control = [&]() {
             // ---- step-size control and acceptance (gpu_*_perform_step.jl adaptive versions)
 #if DEG_FASTPOW
             // EEst holds EEst^2 here (scaled_error2_fast)
@@ -866,7 +780,111 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
                 ++nacc;
                 accepted = true;
             }
+        };
+#endif
+#if DEG_PERSISTENT
+        if (have) {
+#else
+        {
+#endif
+        done = !(t < tf);
+        if (!done) {
+            if (attempts >= a.max_attempts) {
+                status = DEG_RET_MAXITERS;
+            }
+#if DEG_ADAPTIVE
+            else if (dt < DEG_DTMIN) {
+                status = DEG_RET_DTMIN;
+            }
+#endif
+            else {
+                ++attempts;
+#if DEG_ADAPTIVE
+                // ---- one attempt of the `while EEst > 1` loop: stages and error estimate
+#if DEG_METHOD == 0
+                if (!have_k) {
+                    deg_rhs(u, p, t, k[1]);
+                    have_k = true;
+                } else if (fsal_pending) {
+                    DEG_FOR(c) k[1][c] = k[7][c];
+                }
+                fsal_pending = false;
+                deg_attempt(u, p, t, dt, k, unew);
+                EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
+#elif DEG_METHOD == 1
+                deg_rhs(u, p, t, k[1]);
+                deg_attempt(u, p, t, dt, k, unew);
+                EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
+#else
+                EEst = deg_rb23<true>(u, p, t, dt, k, unew, a.abstol, a.reltol);
+#endif
+                attempted = true;
+#if DEG_LEARNED || DEG_STEPLOG
+                {
+                    // proposer features (base-2 logs; EEst holds EEst^2 on this path)
+                    float su = 0.0f, sf = 0.0f;
+                    DEG_FOR(c) {
+                        su = __builtin_fmaf(u[c], u[c], su);
+                        sf = __builtin_fmaf(k[1][c], k[1][c], sf);
+                    }
+                    feat[0] = 0.5f * __builtin_amdgcn_logf(__builtin_fmaxf(EEst, 1e-30f));
+                    feat[1] = lx_prev;
+                    feat[2] = __builtin_amdgcn_logf(dt);
+                    feat[3] = 0.5f * __builtin_amdgcn_logf(__builtin_fmaxf(su * (1.0f / DEG_N), 1e-30f));
+                    feat[4] = 0.5f * __builtin_amdgcn_logf(__builtin_fmaxf(sf * (1.0f / DEG_N), 1e-30f));
+                    feat[5] = first_attempt ? 0.0f : feat[2] - ldt_prev;
+                }
+#endif
+#if !DEG_LEARNED
+                control();
+#endif
+#else
+                // ---- one fixed step (gpu_*_perform_step.jl fixed versions)
+                const float told = t;
+                const float tcomp_old = tc;
+                const float tn = deg_advance(t, dt, tc);
+#if !DEG_KAHAN
+                if (tn == told) {
+                    // the step is below the resolution of t: stop instead of looping forever
+                    status = DEG_RET_DTMIN;
+                    tc = tcomp_old;
+                    --attempts;
+                } else
+#endif
+                {
+                DEG_FOR(c) uprev[c] = u[c];
+                tprev = told;
+                dtacc = dt;
+                t = tn;
+#if DEG_METHOD == 0
+                if (!have_k) {
+                    deg_rhs(uprev, p, told, k[1]);
+                    have_k = true;
+                } else {
+                    DEG_FOR(c) k[1][c] = k[7][c];
+                }
+                deg_attempt(uprev, p, told, dt, k, unew);
+#elif DEG_METHOD == 1
+                deg_rhs(uprev, p, told, k[1]);
+                deg_attempt(uprev, p, told, dt, k, unew);
+#else
+                deg_rb23<false>(uprev, p, told, dt, k, unew, 0.0f, 0.0f);
+#endif
+                DEG_FOR(c) u[c] = unew[c];
+                ++nacc;
+                accepted = true;
+                }
+#endif
+            }
         }
+#if DEG_LEARNED
+        }  // phase A
+
+        // every lane of the wavefront is here (the loop exit is wave-uniform in this variant)
+        zml = deg_mlp_eval((const deg_mlp_packed*)a.mlp, feat);
+
+        if (have) {
+        if (attempted) control();
 #endif
         if (accepted) {
             // ---- savevalues! (integrator_utils.jl:13-50)
@@ -874,13 +892,19 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
 #if DEG_HAS_INTERP
                 while (tnext <= t) {
                     float out[DEG_N];
+#if DEG_PARITY
                     const float th = (tnext - tprev) / dtacc;
+#else
+                    const float th = (tnext - tprev) * __builtin_amdgcn_rcpf(dtacc);
+#endif
                     deg_interp(th, dtacc, tprev, p, uprev, k, out);
                     if (cur < a.nsave) {
-                        DEG_FOR(c) a.us[((long long)cur * DEG_N + c) * S + i] = out[c];
-                        a.ts[(long long)cur * S + i] = tnext;
+                        DEG_FOR(c) us_w[(long long)c * S] = out[c];
+                        *ts_w = tnext;
                     }
                     ++cur;
+                    ts_w += S;
+                    us_w += (long long)DEG_N * S;
                     tnext = (cur < a.nsaveat) ? svp[cur] : finf;
                 }
 #endif
@@ -933,7 +957,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_KERNEL(const deg
             break;
 #endif
         }
-        }  // phase B
+        }  // lanes holding a trajectory
     }
 }
 

@@ -107,9 +107,6 @@ def _check_unsupported(callback, tstops):
         raise NotImplementedError("tstops are not provided by the HIP path")
 
 
-_AUTOTUNE_BLOCK = {}
-
-
 def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, abstol, reltol, saveat_np,
             nsave, save_everystep, t0_fill, timing=False, stream=None, mlp=None, steplog=None, block_size=None,
             grid_blocks=0, outputs=None):
@@ -181,7 +178,7 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
     a.nsaveat = int(sv.numel()) if sv is not None else 0
     a.nsave = int(nsave)
     a.save_everystep = int(bool(save_everystep))
-    a.block_size = int(block_size or dev.block_size or _AUTOTUNE_BLOCK.get((alg.method, flags, n), 0))
+    a.block_size = int(block_size or dev.block_size or 0)
     a.grid_blocks = int(grid_blocks)
     a.ts = ts.data_ptr()
     a.us = us_soa.data_ptr()

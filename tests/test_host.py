@@ -250,3 +250,33 @@ def test_rng_and_workloads(deg):
     for name in ("lorenz_fixed", "lorenz_uneven", "robertson", "vdp"):
         c = workloads.config(name, N=64)
         assert c["u0"].shape[1] == 64
+
+
+def test_autotune_cache_key_and_lookup(deg, tmp_path, monkeypatch):
+    """The tuner's cache key separates configurations and a cached entry is returned as is."""
+    import dataclasses
+
+    from diffeqgpu_amd import autotune, systems
+
+    monkeypatch.setattr(autotune, "CACHE_PATH", str(tmp_path / "autotune.json"))
+    f, n, np_, _ = systems.SYSTEMS["lorenz"]
+
+    class Probs:
+        pass
+
+    probs = Probs()
+    probs.f, probs.n, probs.np, probs.N, probs.dev = f, n, np_, 1_000_000, deg.HIPDevice()
+    k1 = autotune._key(probs, deg.GPUTsit5(), True, "saveat")
+    k2 = autotune._key(probs, deg.GPUVern7(), True, "saveat")
+    probs2 = Probs()
+    probs2.__dict__.update(probs.__dict__, dev=deg.HIPDevice(kahan=True))
+    k3 = autotune._key(probs2, deg.GPUTsit5(), True, "saveat")
+    probs3 = Probs()
+    probs3.__dict__.update(probs.__dict__, dev=deg.HIPDevice(block_size=64, waves=5))  # tuned knobs: same key
+    assert len({k1, k2, k3}) == 3 and autotune._key(probs3, deg.GPUTsit5(), True, "saveat") == k1
+    autotune._store({k1: dict(config=dict(block_size=128, waves=6, refill=2, chunk=64), kernel_ms=1.0)})
+    got = autotune.tune(probs, None, deg.GPUTsit5(), saveat=1.0)
+    assert got == dataclasses.replace(deg.HIPDevice(), block_size=128, waves=6, refill=2, chunk=64)
+    assert deg.HIPDevice(waves=6).flags(True) >> 9 == 6
+    with pytest.raises(ValueError):
+        deg.HIPDevice(waves=9).flags(True)
