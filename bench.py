@@ -25,9 +25,13 @@ F, FJ and FT are the traced RHS, Jacobian and time-gradient flop counts. Attempt
 kernel's own per-trajectory accept/reject counters. The kernel time comes from HIP events
 recorded on the launch stream around every timed step.
 
-Options: --gpus N (1 only: one GPU), --steps K, --warmup W, --workload, --n-traj,
+Besides ``value`` (inputs and outputs resident in HBM) the line reports the end-to-end rate
+through the C ABI from host arrays (SoA pack, H2D, solve, D2H of ts/us/retcode) and the cold
+hiprtc compile time of the workload's solver; neither is part of ``value``.
+
+Options: --gpus N (1 only: one GPU), --steps K, --warmup W, --workload, --n-traj, --no-autotune,
 --copy-probe (one 256 MiB device copy first, to calibrate HBM counters under rocprofv3),
---no-cpu (skip the CPU baseline).
+--no-cpu (skip the CPU baseline), --no-extras (skip the end-to-end and compile timings).
 """
 import argparse
 import json
@@ -51,7 +55,7 @@ WORKLOADS = {
                           sweeps=10,
                           desc="lorenz_uneven_4M: Lorenz rho~U(0,60) sigma~U(5,15) beta~U(1,4) tend~U(1,50), "
                                "GPUVern7 adaptive 1e-8, Kahan t, endpoint only, 10 device-drawn sweeps per "
-                               "hipGraph replay"),
+                               "hipGraph replay, longest span first"),
     "robertson": dict(alg="GPURosenbrock23", steps=10, warmup=2, golden="robertson_radau.npz", cpu_sample=100_000,
                       desc="robertson_1M: Robertson k=(0.04,3e7,1e4).*exp(0.3Z), GPURosenbrock23 reltol=1e-4 "
                            "abstol=1e-8, FP64 W-LU, 20 log-spaced saves"),
@@ -75,10 +79,13 @@ def parse():
     ap.add_argument("--block-size", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--autotune", action="store_true",
-                    help="pick block size, register cap, refill and chunk by timing (before the timed region)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the default launch configuration instead of the tuned one (the tuner's cache or "
+                         "the shipped MI355X table; tuned before the timed region when neither has it)")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the end-to-end and JIT-compile timings")
+    ap.add_argument("--no-order", action="store_true", help="lorenz_uneven: ascending ids instead of longest first")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
@@ -139,6 +146,40 @@ def cpu_baseline(name, cfg, threads, sample):
                        f"oracle/deg_oracle.c Float32 restatement, OpenMP {threads} threads, {el:.1f} s")
 
 
+def end_to_end(deg, torch, f, cfg, prob, alg, dev, kw, reps=3):
+    """Host arrays in, host arrays out: SoA pack + H2D, the solve, D2H of ts/us/retcode (ms)."""
+    best = float("inf")
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev, tfs=cfg.get("tfs"))
+        res = deg.vectorized_asolve(probs, prob, alg, **kw)
+        out = (res.ts.cpu(), res.us_soa.cpu(), res.retcode.cpu())
+        torch.cuda.synchronize()
+        best = min(best, (time.perf_counter() - t0) * 1e3)
+        del probs, res, out
+    return best
+
+
+def jit_compile_seconds(deg, f, n, np_, alg, dev):
+    """Cold hiprtc compile of the workload's solver (a private cache directory)."""
+    import tempfile
+
+    from diffeqgpu_amd import _lib
+    from diffeqgpu_amd.algorithms import GPURosenbrock23
+
+    src, _ = f.hip_source(n, np_, isinstance(alg, GPURosenbrock23))
+    L = _lib.lib()
+    with tempfile.TemporaryDirectory() as tmp:
+        L.deg_set_cache_dir(tmp.encode())
+        try:
+            t0 = time.perf_counter()
+            _lib.precompile(src, n, np_, alg.method, dev.flags(True))
+            return time.perf_counter() - t0
+        finally:
+            L.deg_set_cache_dir(_lib.CACHE_DIR.encode())
+
+
 def fp64_error(name, us):
     """Endpoint error (max over states) against the FP64 fixture, first 1,000 trajectories."""
     spec = WORKLOADS[name]
@@ -190,16 +231,18 @@ def main():
     prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
     kw = solve_kw(cfg)
     tuned = None
-    if args.autotune:
+    if not args.no_autotune and not (args.block_size or args.refill or args.chunk):
         tprobs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev, tfs=cfg.get("tfs"))
         dev = deg.autotune.tune(tprobs, prob, alg, **kw)
-        tuned = dict(block_size=dev.block_size, waves=dev.waves, refill=dev.refill, chunk=dev.chunk)
+        tuned = dict(block_size=dev.block_size, waves=dev.waves, refill=dev.refill, chunk=dev.chunk,
+                     blocks_per_cu=dev.blocks_per_cu)
         del tprobs
 
     if name == "lorenz_uneven":
         sweeps = spec["sweeps"]
         probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev, tfs=cfg["tfs"])
-        sw = deg.ParameterSweep(probs, prob, alg, sweeps=sweeps, seed=303, draws=UNEVEN_DRAWS, **kw)
+        sw = deg.ParameterSweep(probs, prob, alg, sweeps=sweeps, seed=303, draws=UNEVEN_DRAWS,
+                                longest_first=not args.no_order, **kw)
         stream = sw.stream
         run = sw.replay
         per_step = sweeps * N
@@ -272,6 +315,12 @@ def main():
         "fp64_oracle_error": err,
         "retcodes": retcodes,
     }
+    if name != "lorenz_uneven" and not args.no_extras:
+        e2e_ms = end_to_end(deg, torch, f, cfg, prob, alg, dev, kw)
+        out["end_to_end"] = {"ms": e2e_ms, "trajectories_per_s": N / (e2e_ms / 1e3),
+                             "what": "host SoA arrays -> H2D -> solve -> D2H of ts, us, retcode (PCIe-inclusive)"}
+    if not args.no_extras:
+        out["jit_compile_s"] = jit_compile_seconds(deg, f, n, np_, alg, dev)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(name, cfg, threads, spec["cpu_sample"])

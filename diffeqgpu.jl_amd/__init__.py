@@ -7,7 +7,7 @@ suite, the benchmark and the build entry point load it as ``diffeqgpu_amd``.
 from .algorithms import (EnsembleGPUKernel, EnsembleKernelAlgorithm, GPUODEAlgorithm, GPURosenbrock23, GPUTsit5,
                          GPUVern7, HIPDevice, alg_order)
 from .ensemble import EnsembleContext, EnsembleSolution, ODESolution, solve
-from .lowerlevel import SolveResult, vectorized_asolve, vectorized_solve
+from .lowerlevel import SolveResult, order_by_span, vectorized_asolve, vectorized_solve
 from .problems import (EnsembleProblem, HIPProbVector, ImmutableODEProblem, ODEFunction, ODEProblem, adapt,
                        get_backend, make_prob_compatible, remake)
 from .ranges import colon, linrange
@@ -18,7 +18,7 @@ from . import tracer as math  # sin, cos, exp, log, sqrt, tanh, fabs, pow, ifels
 
 __all__ = [
     "EnsembleGPUKernel", "EnsembleKernelAlgorithm", "GPUODEAlgorithm", "GPURosenbrock23", "GPUTsit5", "GPUVern7",
-    "HIPDevice", "alg_order", "EnsembleContext", "EnsembleSolution", "ODESolution", "solve", "SolveResult",
+    "HIPDevice", "alg_order", "EnsembleContext", "EnsembleSolution", "ODESolution", "solve", "SolveResult", "order_by_span",
     "vectorized_asolve", "vectorized_solve", "EnsembleProblem", "HIPProbVector", "ImmutableODEProblem",
     "ODEFunction", "ODEProblem", "adapt", "get_backend", "make_prob_compatible", "remake", "colon", "linrange",
     "SVector", "math", "Draw", "ParameterSweep", "autotune", "learned",

@@ -76,6 +76,7 @@ class SolveArgs(ctypes.Structure):
         ("steplog_cap", ctypes.c_int64),
         ("chunk", ctypes.c_int32),
         ("pad1", ctypes.c_int32),
+        ("order", ctypes.c_void_p),
     ]
 
 
@@ -84,7 +85,7 @@ EXPORTS = [
     "deg_kernel_destroy", "deg_kernel_attr", "deg_jit_precompile", "deg_jit_source", "deg_solve", "deg_sync",
     "deg_last_error", "deg_version", "deg_solve_args_size", "deg_draw_uniform", "deg_graph_begin", "deg_graph_end",
     "deg_graph_launch", "deg_graph_num_nodes", "deg_graph_destroy", "deg_mlp_packed_size", "deg_mlp_partial_cols",
-    "deg_mlp_grad", "deg_mlp_adam", "deg_mlp_forward", "deg_mlp_pack", "deg_mlp_infer",
+    "deg_mlp_grad", "deg_mlp_adam", "deg_mlp_forward", "deg_mlp_pack", "deg_mlp_infer", "deg_order_by_span",
 ]
 STEPLOG_COLS = 12
 MLP_NPARAM = 1313
@@ -119,6 +120,7 @@ def _declare(lib):
     lib.deg_solve_args_size.restype = ctypes.c_uint32
     lib.deg_draw_uniform.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int32,
                                      ctypes.c_int32, ctypes.c_float, ctypes.c_float, vp]
+    lib.deg_order_by_span.argtypes = [vp, vp, ctypes.c_float, ctypes.c_float, ctypes.c_int64, vp, vp, vp]
     lib.deg_graph_begin.argtypes = [vp]
     lib.deg_graph_end.argtypes = [vp, ctypes.POINTER(vp)]
     lib.deg_graph_launch.argtypes = [vp, vp]
@@ -136,7 +138,7 @@ def _declare(lib):
                  "deg_kernel_destroy", "deg_kernel_attr", "deg_jit_precompile", "deg_solve", "deg_sync",
                  "deg_draw_uniform", "deg_graph_begin", "deg_graph_end", "deg_graph_launch", "deg_graph_num_nodes",
                  "deg_graph_destroy", "deg_mlp_grad", "deg_mlp_adam", "deg_mlp_forward", "deg_mlp_pack",
-                 "deg_mlp_infer"):
+                 "deg_mlp_infer", "deg_order_by_span"):
         getattr(lib, name).restype = ctypes.c_int
 
 
@@ -272,3 +274,13 @@ class Graph:
         if getattr(self, "handle", None) and _lib is not None:
             _lib.deg_graph_destroy(self.handle)
             self.handle = None
+
+
+ORDER_WORK_INTS = 4096
+
+
+def order_by_span(t0s_ptr, tfs_ptr, t0: float, tf: float, n: int, order_ptr: int, work_ptr: int, stream_ptr):
+    """Longest-span-first permutation of 0..n-1 into order (int32), see deg_order_by_span."""
+    check(lib().deg_order_by_span(ctypes.c_void_p(t0s_ptr) if t0s_ptr else None,
+                                  ctypes.c_void_p(tfs_ptr) if tfs_ptr else None, t0, tf, n, ctypes.c_void_p(order_ptr),
+                                  ctypes.c_void_p(work_ptr), ctypes.c_void_p(stream_ptr)), "deg_order_by_span")

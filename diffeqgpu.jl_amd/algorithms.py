@@ -85,6 +85,9 @@ class HIPDevice:
       chunk       persistent grids: trajectory ids a wave claims per queue atomic (0: 16).
       waves       register cap: compile the solver for at least this many resident waves per
                   SIMD (1..7; 0: the compiler's choice).
+      blocks_per_cu persistent grids: launch this many blocks per compute unit (0: as many as
+                  fit). Fewer resident lanes means more trajectories per lane, which shortens
+                  the tail where lanes sit idle.
     """
 
     device: int = 0
@@ -97,6 +100,7 @@ class HIPDevice:
     refill: int = 0
     chunk: int = 0
     waves: int = 0
+    blocks_per_cu: int = 0
 
     def flags(self, adaptive: bool) -> int:
         f = 0

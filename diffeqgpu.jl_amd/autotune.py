@@ -6,15 +6,17 @@ lowest measured kernel time:
   register cap     DEG_F_WAVES: the solver compiled for at least w waves per SIMD (0 = the
                    compiler's own allocation, 4..7 = capped, possibly spilling);
   refill batch     persistent grids: idle lanes per wave that trigger a refill;
-  chunk            persistent grids: trajectory ids claimed per queue atomic.
+  chunk            persistent grids: trajectory ids claimed per queue atomic;
+  blocks per CU    persistent grids: resident blocks per compute unit (0 = as many as fit).
 The LDS depth of the saveat cache needs no tuning: it is sized to the save grid at every launch.
 
 The search is coordinate descent in that order, starting from the defaults. Each candidate is
 timed with HIP events around the kernel (the best of ``repeats`` launches) on the caller's
 problems, so the result reflects the real trajectory mix. Results are cached in
 ``_jitcache/autotune.json``, keyed by the traced source, n, np, method, flags and the ensemble
-size rounded to a power of two. The reference has no tuner: KernelAbstractions picks its own
-workgroup size.
+size rounded to a power of two. ``autotune_mi355x.json`` beside this file ships the results measured
+on MI355X for the benchmark workloads; the local cache takes precedence over it. The reference has
+no tuner: KernelAbstractions picks its own workgroup size.
 """
 from __future__ import annotations
 
@@ -29,26 +31,45 @@ from . import _lib
 from .algorithms import GPURosenbrock23, HIPDevice
 
 CACHE_PATH = os.path.join(_lib.CACHE_DIR, "autotune.json")
+SHIPPED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_mi355x.json")
 BLOCKS = (64, 128, 192, 256)
 WAVES = (0, 4, 5, 6, 7)
 REFILL = (1, 2, 4, 8, 16)
 CHUNK = (4, 16, 64)
+WAVES_PER_CU = (8, 12, 16)   # blocks-per-CU candidates are these resident waves per CU
+
+
+def blocks_per_cu_candidates(block: int):
+    return (0,) + tuple(max(1, w * 64 // block) for w in WAVES_PER_CU)
 
 
 def _key(probs, alg, adaptive, nsave_hint) -> str:
     src, _ = probs.f.hip_source(probs.n, probs.np, isinstance(alg, GPURosenbrock23))
     h = hashlib.sha1(src.encode()).hexdigest()[:16]
-    base = dataclasses.replace(probs.dev, block_size=None, refill=0, chunk=0, waves=0)
+    base = dataclasses.replace(probs.dev, block_size=None, refill=0, chunk=0, waves=0, blocks_per_cu=0)
     nb = int(round(math.log2(max(1, probs.N))))
     return f"{h}:{probs.n}:{probs.np}:{alg.method}:{base.flags(adaptive)}:{nb}:{nsave_hint}"
 
 
-def _load() -> dict:
+def _read(path) -> dict:
     try:
-        with open(CACHE_PATH) as f:
+        with open(path) as f:
             return json.load(f)
     except (OSError, ValueError):
         return {}
+
+
+def _load() -> dict:
+    """The local cache."""
+    return _read(CACHE_PATH)
+
+
+def lookup(key: str):
+    """A tuned configuration: the local cache first, then the shipped MI355X table."""
+    for table in (_load(), _read(SHIPPED_PATH)):
+        if key in table:
+            return table[key]["config"]
+    return None
 
 
 def _store(table: dict):
@@ -73,9 +94,10 @@ def tune(probs, prob, alg, *, adaptive: bool = True, repeats: int = 2, use_cache
     fn = vectorized_asolve if adaptive else vectorized_solve
     nsave_hint = "saveat" if solve_kw.get("saveat") is not None else ("every" if solve_kw.get("save_everystep") else "end")
     key = _key(probs, alg, adaptive, nsave_hint)
-    table = _load() if use_cache else {}
-    if key in table:
-        return dataclasses.replace(probs.dev, **table[key]["config"])
+    if use_cache:
+        hit = lookup(key)
+        if hit is not None:
+            return dataclasses.replace(probs.dev, **hit)
 
     persistent = bool(probs.dev.flags(adaptive) & _lib.F_PERSISTENT)
     trials = {}
@@ -94,11 +116,15 @@ def tune(probs, prob, alg, *, adaptive: bool = True, repeats: int = 2, use_cache
                 log(f"[autotune] {cfg} {best:.3f} ms")
         return trials[k]
 
-    cfg = dict(block_size=256, waves=0, refill=4 if persistent else 0, chunk=16 if persistent else 0)
+    default = dict(block_size=256, waves=0, refill=4 if persistent else 0, chunk=16 if persistent else 0,
+                   blocks_per_cu=0)
+    cfg = dict(default)
     axes = [("block_size", BLOCKS), ("waves", WAVES)]
     if persistent:
-        axes += [("refill", REFILL), ("chunk", CHUNK)]
+        axes += [("refill", REFILL), ("chunk", CHUNK), ("blocks_per_cu", None)]
     for name, values in axes:
+        if values is None:
+            values = blocks_per_cu_candidates(cfg["block_size"])
         best_v, best_t = cfg[name], measure(cfg)
         for v in values:
             c = dict(cfg, **{name: v})
@@ -106,11 +132,9 @@ def tune(probs, prob, alg, *, adaptive: bool = True, repeats: int = 2, use_cache
             if t < best_t:
                 best_v, best_t = v, t
         cfg[name] = best_v
-    table = _load() if use_cache else table
+    table = _load() if use_cache else {}
     table[key] = dict(config=cfg, kernel_ms=trials[tuple(sorted(cfg.items()))],
-                      default_ms=trials.get(tuple(sorted(dict(block_size=256, waves=0, refill=4 if persistent else 0,
-                                                              chunk=16 if persistent else 0).items()))),
-                      trials=len(trials))
+                      default_ms=trials.get(tuple(sorted(default.items()))), trials=len(trials))
     if use_cache:
         _store(table)
     return dataclasses.replace(probs.dev, **cfg)

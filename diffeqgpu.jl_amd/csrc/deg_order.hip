@@ -1,0 +1,87 @@
+// deg_order.hip -- longest-first start order for the persistent queue.
+//
+// A counting sort of the trajectories by their span tf_i - t0_i, descending, into 2048
+// buckets given by the span's leading float bits (sign 0, 8 exponent bits, 3 mantissa bits:
+// non-negative floats order like their bit patterns). Four kernels on one stream:
+// clear the counters, histogram, exclusive scan in descending bucket order (one block),
+// scatter. Order within a bucket follows the atomics and is not fixed; the solver's results do
+// not depend on it (each lane's arithmetic is independent of where and when it runs).
+#include "deghip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+int deg_rng_fail(int code, const char* msg);  // thread-local last error (deghip.cpp)
+
+namespace {
+
+constexpr int kBuckets = 2048;
+
+__device__ __forceinline__ int span_bucket(const float* t0s, const float* tfs, float t0, float tf, long long i) {
+    const float s = (tfs ? tfs[i] : tf) - (t0s ? t0s[i] : t0);
+    if (!(s > 0.0f)) return 0;                       // zero, negative or NaN spans go last
+    const unsigned int b = __builtin_bit_cast(unsigned int, s) >> 20;  // < 2048 for finite s > 0
+    return b < (unsigned int)kBuckets ? (int)b : kBuckets - 1;
+}
+
+__global__ void __launch_bounds__(256) clear_kernel(int* __restrict__ work) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < 2 * kBuckets) work[j] = 0;
+}
+
+__global__ void __launch_bounds__(256) histogram_kernel(const float* t0s, const float* tfs, float t0, float tf,
+                                                        long long n, int* __restrict__ count) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&count[span_bucket(t0s, tfs, t0, tf, i)], 1);
+}
+
+// cursor[b] = number of trajectories in buckets above b (descending order); one block of 1024
+__global__ void __launch_bounds__(1024) scan_kernel(const int* __restrict__ count, int* __restrict__ cursor) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    // each thread sums two buckets, taken from the top: thread t covers b = 2047-2t, 2046-2t
+    const int b0 = kBuckets - 1 - 2 * t, b1 = b0 - 1;
+    const int c0 = count[b0], c1 = count[b1];
+    part[t] = c0 + c1;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+        const int v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    const int before = part[t] - (c0 + c1);
+    cursor[b0] = before;
+    cursor[b1] = before + c0;
+}
+
+__global__ void __launch_bounds__(256) scatter_kernel(const float* t0s, const float* tfs, float t0, float tf,
+                                                      long long n, int* __restrict__ cursor, int* __restrict__ order) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int pos = atomicAdd(&cursor[span_bucket(t0s, tfs, t0, tf, i)], 1);
+        order[pos] = (int)i;
+    }
+}
+
+}  // namespace
+
+extern "C" int deg_order_by_span(const float* t0s, const float* tfs, float t0, float tf, int64_t n, int32_t* order,
+                                 int32_t* work, void* stream) {
+    if (n < 0 || n > 0x7fffffffll) return deg_rng_fail(DEG_E_INVALID, "n must be in 0..2^31-1");
+    if (n == 0) return DEG_OK;
+    if (!order || !work) return deg_rng_fail(DEG_E_INVALID, "order and work are required");
+    hipStream_t st = (hipStream_t)stream;
+    int* count = work;
+    int* cursor = work + kBuckets;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(clear_kernel, dim3((2 * kBuckets + 255) / 256), dim3(256), 0, st, work);
+    hipLaunchKernelGGL(histogram_kernel, dim3(blocks), dim3(256), 0, st, t0s, tfs, t0, tf, (long long)n, count);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, count, cursor);
+    hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(256), 0, st, t0s, tfs, t0, tf, (long long)n, cursor,
+                       order);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return deg_rng_fail(DEG_E_HIP, (std::string("deg_order_by_span: ") + hipGetErrorString(e)).c_str());
+    return DEG_OK;
+}

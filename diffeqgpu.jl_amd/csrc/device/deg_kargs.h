@@ -30,4 +30,5 @@ struct deg_kargs {
     int refill;               // persistent grids: idle lanes per wave that trigger a refill (1..64)
     int chunk;                // persistent grids: ids claimed per queue atomic
     int pad1;
+    const int* order;         // optional start order: k-th trajectory started = id_offset + order[k]
 };

@@ -109,10 +109,11 @@ def _check_unsupported(callback, tstops):
 
 def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, abstol, reltol, saveat_np,
             nsave, save_everystep, t0_fill, timing=False, stream=None, mlp=None, steplog=None, block_size=None,
-            grid_blocks=0, outputs=None):
+            grid_blocks=0, outputs=None, order=None):
     """Launch one solve. ``outputs`` (optional) supplies preallocated device tensors
     {"ts", "us_soa", "retcode", "naccept", "nreject", "saveat"} so that the call allocates and
-    copies nothing (required inside a graph capture)."""
+    copies nothing (required inside a graph capture). ``order`` (optional, int32 device tensor)
+    is the start order of the persistent queue, e.g. from ``order_by_span``."""
     import torch
 
     if not isinstance(alg, GPUODEAlgorithm):
@@ -179,6 +180,8 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
     a.nsave = int(nsave)
     a.save_everystep = int(bool(save_everystep))
     a.block_size = int(block_size or dev.block_size or 0)
+    if not grid_blocks and dev.blocks_per_cu:
+        grid_blocks = int(dev.blocks_per_cu) * ctx.num_cus
     a.grid_blocks = int(grid_blocks)
     a.ts = ts.data_ptr()
     a.us = us_soa.data_ptr()
@@ -198,6 +201,10 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
         a.elapsed_ms = ctypes.addressof(box)
     if mlp is not None:
         a.mlp = mlp.data_ptr()
+    if order is not None:
+        if order.dtype != torch.int32 or order.numel() != N or order.device != device:
+            raise ValueError("order must be an int32 tensor of N trajectory indices on the problems' device")
+        a.order = order.data_ptr()
     if steplog is not None:
         buf, counter = steplog
         a.steplog = buf.data_ptr()
@@ -255,3 +262,20 @@ def vectorized_asolve(probs, prob, alg, *, dt=0.1, saveat=None, save_everystep=F
     # save_everystep = false); the kernel only skips the endpoint write in that case.
     return _launch(probs, alg, True, dt=dt32, abstol=abstol, reltol=reltol, saveat_np=sv, nsave=nsave,
                    save_everystep=save_everystep and sv is None, t0_fill=t0, timing=timing, **kwargs)
+
+
+def order_by_span(probs, stream=None):
+    """Longest-span-first start order of the trajectories (int32 device tensor), computed on the
+    device (deg_order_by_span). Pass it as ``order=`` to the solve: long trajectories start first,
+    which shortens the tail in which lanes idle; results do not depend on the order."""
+    import torch
+
+    dev = probs.u0.device
+    order = torch.empty(probs.N, dtype=torch.int32, device=dev)
+    work = torch.empty(_lib.ORDER_WORK_INTS, dtype=torch.int32, device=dev)
+    st = stream if stream is not None else torch.cuda.current_stream(dev)
+    _lib.order_by_span(probs.t0s.data_ptr() if probs.t0s is not None else 0,
+                       probs.tfs.data_ptr() if probs.tfs is not None else 0, float(probs.tspan[0]),
+                       float(probs.tspan[1]), probs.N, order.data_ptr(), work.data_ptr(), st.cuda_stream)
+    order.work = work  # keep the workspace alive with the order until the stream has used it
+    return order

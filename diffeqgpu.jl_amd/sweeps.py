@@ -9,6 +9,9 @@ results are available after one replay.
 The reference has no counterpart. There, a sweep is a fresh ``solve(EnsembleProblem, ...)``
 per parameter set, with the draws in ``prob_func`` on the host (src/solve.jl:187-202).
 
+With ``longest_first`` every sweep also orders its trajectories by descending span on the
+device (``deg_order_by_span``) so that the persistent queue starts the longest ones first.
+
 Draw j of sweep k fills its target row with ``lo + scale * U(seed + k, i * rows + j)``, where
 ``rows = len(draws)``. This is the same table as ``rng.uniform_table(seed + k, rows, N)``, so
 sweep 0 with ``seed`` reproduces a host-drawn workload built from that table.
@@ -44,7 +47,7 @@ class ParameterSweep:
     private stream."""
 
     def __init__(self, probs: HIPProbVector, prob: ODEProblem, alg: GPUODEAlgorithm, *, sweeps: int, seed: int,
-                 draws, adaptive: bool = True, **solve_kw):
+                 draws, adaptive: bool = True, longest_first: bool = False, **solve_kw):
         import torch
 
         if sweeps < 1:
@@ -64,6 +67,11 @@ class ParameterSweep:
         self.solve_kw = dict(solve_kw)
         self.device = probs.u0.device
         self.stream = torch.cuda.Stream(device=self.device)
+        self.order = self.work = None
+        if longest_first:
+            # re-ordered on the device after every draw: longest spans start first
+            self.order = torch.empty(probs.N, dtype=torch.int32, device=self.device)
+            self.work = torch.empty(_lib.ORDER_WORK_INTS, dtype=torch.int32, device=self.device)
         # eager pass: compiles the kernel, allocates every sweep's outputs
         self.results = []
         with torch.cuda.stream(self.stream):
@@ -102,7 +110,14 @@ class ParameterSweep:
 
     def _solve(self, outputs):
         fn = vectorized_asolve if self.adaptive else vectorized_solve
-        return fn(self.probs, self.prob, self.alg, stream=self.stream, outputs=outputs, **self.solve_kw)
+        if self.order is not None:
+            pv = self.probs
+            _lib.order_by_span(pv.t0s.data_ptr() if pv.t0s is not None else 0,
+                               pv.tfs.data_ptr() if pv.tfs is not None else 0, float(pv.tspan[0]),
+                               float(pv.tspan[1]), pv.N, self.order.data_ptr(), self.work.data_ptr(),
+                               self.stream.cuda_stream)
+        return fn(self.probs, self.prob, self.alg, stream=self.stream, outputs=outputs, order=self.order,
+                  **self.solve_kw)
 
     def replay(self, stream=None):
         """Launch the captured sweeps (asynchronous) on ``stream`` (default: the sweep's own stream)."""

@@ -97,6 +97,8 @@ typedef struct deg_solve_args {
     int64_t steplog_cap;
     int32_t chunk;           /* persistent kernels: trajectory ids a wave claims per queue atomic; 0: 16 */
     int32_t pad1;
+    const int32_t *order;    /* optional [n_traj] permutation: the k-th trajectory started is id_offset + order[k]
+                                (e.g. deg_order_by_span, longest first); NULL: ascending ids */
 } deg_solve_args;
 
 /* Context: one device (launches go to the stream given per call). */
@@ -154,6 +156,15 @@ uint32_t deg_solve_args_size(void);
  * re-draws of an EnsembleProblem (src/solve.jl:187-202) for sweeps that are graph-captured. */
 int deg_draw_uniform(float *out, int64_t n, int64_t id_offset, uint64_t seed, int32_t rows, int32_t row,
                      float lo, float scale, void *stream);
+
+/* Longest-first order of trajectories for the persistent queue: writes order[0..n) with a
+ * permutation of 0..n-1 sorted by descending span tfs[i] - t0s[i] (NULL arrays: the shared t0/tf),
+ * bucketed by the span's leading float bits (2048 buckets, 1/8-octave resolution; no order
+ * within a bucket). work: 4096 int32 of device memory. Asynchronous on `stream`, graph-capturable.
+ * Passed as deg_solve_args.order it shortens the tail in which lanes of finished trajectories
+ * idle (no counterpart in the reference, which launches one thread per trajectory). */
+int deg_order_by_span(const float *t0s, const float *tfs, float t0, float tf, int64_t n, int32_t *order,
+                      int32_t *work, void *stream);
 
 typedef struct deg_graph deg_graph;
 /* Capture every launch made on `stream` (non-NULL) by this thread between begin and end --

@@ -25,6 +25,7 @@ def test_configurations_bitwise_equal_and_tuner(deg, torch_cuda, tmp_path, monke
     from diffeqgpu_amd import autotune, systems, workloads
 
     monkeypatch.setattr(autotune, "CACHE_PATH", str(tmp_path / "autotune.json"))
+    monkeypatch.setattr(autotune, "SHIPPED_PATH", str(tmp_path / "none.json"))
     f, n, np_, _ = systems.SYSTEMS["lorenz"]
     cfg = workloads.config("lorenz_adaptive", N=30_000)
     prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
@@ -40,7 +41,7 @@ def test_configurations_bitwise_equal_and_tuner(deg, torch_cuda, tmp_path, monke
 
     ref = run()
     for devkw in (dict(block_size=64), dict(block_size=192, refill=1), dict(waves=7), dict(waves=5, chunk=4),
-                  dict(refill=16, chunk=64), dict(persistent=False, block_size=128)):
+                  dict(refill=16, chunk=64), dict(blocks_per_cu=1), dict(persistent=False, block_size=128)):
         got = run(**devkw)
         for a, b in zip(ref, got):
             np.testing.assert_array_equal(a, b, err_msg=str(devkw))
@@ -49,10 +50,12 @@ def test_configurations_bitwise_equal_and_tuner(deg, torch_cuda, tmp_path, monke
     tuned = autotune.tune(probs, prob, deg.GPUTsit5(), log=print, **kw)
     assert tuned.block_size in autotune.BLOCKS and tuned.waves in autotune.WAVES
     assert tuned.refill in autotune.REFILL and tuned.chunk in autotune.CHUNK
+    assert tuned.blocks_per_cu in autotune.blocks_per_cu_candidates(tuned.block_size)
     entry = next(iter(autotune._load().values()))
     assert entry["kernel_ms"] <= entry["default_ms"] * 1.10
     # cached: the second call returns the same configuration without timing
     assert autotune.tune(probs, prob, deg.GPUTsit5(), **kw) == tuned
-    got = run(block_size=tuned.block_size, waves=tuned.waves, refill=tuned.refill, chunk=tuned.chunk)
+    got = run(block_size=tuned.block_size, waves=tuned.waves, refill=tuned.refill, chunk=tuned.chunk,
+              blocks_per_cu=tuned.blocks_per_cu)
     for a, b in zip(ref, got):
         np.testing.assert_array_equal(a, b)

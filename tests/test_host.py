@@ -42,7 +42,7 @@ def test_solve_args_offsets_match_header(deg):
     off = {f[0]: getattr(_lib.SolveArgs, f[0]).offset for f in _lib.SolveArgs._fields_}
     assert off["n_traj"] == 8 and off["u0"] == 32 and off["t0"] == 64 and off["saveat"] == 88
     assert off["ts"] == 112 and off["stream"] == 160 and off["steplog_cap"] == 208 and off["chunk"] == 216
-    assert ctypes.sizeof(_lib.SolveArgs) == 224
+    assert off["order"] == 224 and ctypes.sizeof(_lib.SolveArgs) == 232
 
 
 @pytest.mark.parametrize("method", [0, 1, 2])
@@ -259,6 +259,7 @@ def test_autotune_cache_key_and_lookup(deg, tmp_path, monkeypatch):
     from diffeqgpu_amd import autotune, systems
 
     monkeypatch.setattr(autotune, "CACHE_PATH", str(tmp_path / "autotune.json"))
+    monkeypatch.setattr(autotune, "SHIPPED_PATH", str(tmp_path / "shipped.json"))
     f, n, np_, _ = systems.SYSTEMS["lorenz"]
 
     class Probs:
@@ -274,9 +275,16 @@ def test_autotune_cache_key_and_lookup(deg, tmp_path, monkeypatch):
     probs3 = Probs()
     probs3.__dict__.update(probs.__dict__, dev=deg.HIPDevice(block_size=64, waves=5))  # tuned knobs: same key
     assert len({k1, k2, k3}) == 3 and autotune._key(probs3, deg.GPUTsit5(), True, "saveat") == k1
-    autotune._store({k1: dict(config=dict(block_size=128, waves=6, refill=2, chunk=64), kernel_ms=1.0)})
+    autotune._store({k1: dict(config=dict(block_size=128, waves=6, refill=2, chunk=64, blocks_per_cu=3),
+                              kernel_ms=1.0)})
     got = autotune.tune(probs, None, deg.GPUTsit5(), saveat=1.0)
-    assert got == dataclasses.replace(deg.HIPDevice(), block_size=128, waves=6, refill=2, chunk=64)
+    assert got == dataclasses.replace(deg.HIPDevice(), block_size=128, waves=6, refill=2, chunk=64, blocks_per_cu=3)
+    # the shipped table is consulted when the local cache has no entry
+    import json
+
+    with open(tmp_path / "shipped.json", "w") as fh:
+        json.dump({k2: dict(config=dict(block_size=64, waves=0, refill=1, chunk=4, blocks_per_cu=0))}, fh)
+    assert autotune.tune(probs, None, deg.GPUVern7(), saveat=1.0).block_size == 64
     assert deg.HIPDevice(waves=6).flags(True) >> 9 == 6
     with pytest.raises(ValueError):
         deg.HIPDevice(waves=9).flags(True)
