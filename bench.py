@@ -86,6 +86,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the end-to-end and JIT-compile timings")
     ap.add_argument("--no-order", action="store_true", help="lorenz_uneven: ascending ids instead of longest first")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposer leg")
+    ap.add_argument("--mlp-every", type=int, default=2, help="learned leg: evaluate the proposer every k iterations")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
@@ -178,6 +180,37 @@ def jit_compile_seconds(deg, f, n, np_, alg, dev):
             return time.perf_counter() - t0
         finally:
             L.deg_set_cache_dir(_lib.CACHE_DIR.encode())
+
+
+def learned_leg(deg, torch, name, f, cfg, prob, alg, dev, kw, every, steps=10):
+    """The same workload with the learned step-size proposer: logs from a held-out ensemble
+    (different seed) -> GPU training -> the learned kernel. Reported beside the PI value."""
+    from diffeqgpu_amd import learned, workloads
+
+    train = workloads.config(name, N=min(cfg["N"], 50_000), seed=9001)
+    tprobs = deg.HIPProbVector.from_arrays(f, train["u0"], train["p"], train["tspan"], dev)
+    t0 = time.perf_counter()
+    rows, _ = learned.collect_steplog(tprobs, prob, alg, 12_000_000, **kw)
+    expo = learned.ERR_EXPONENT[type(alg)]
+    tr = learned.ProposerTrainer(rows, expo=expo, rho=0.2, seed=0)
+    tr.train(1500, batch=32768, lr=3e-3)
+    packed = tr.pack()
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev)
+    res = deg.vectorized_asolve(probs, prob, alg, mlp=packed, mlp_every=every, **kw)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(steps):
+        r = deg.vectorized_asolve(probs, prob, alg, mlp=packed, mlp_every=every, timing=True, **kw)
+        ms.append(r.kernel_ms)
+    k_ms = float(np.median(ms))
+    att = int((res.naccept.to(torch.int64) + res.nreject.to(torch.int64)).sum().item())
+    return dict(value=cfg["N"] / (k_ms / 1e3), kernel_ms=k_ms, attempts=att, rejections=int(res.nreject.sum().item()),
+                retcodes=np.bincount(res.retcode.cpu().numpy(), minlength=4).tolist(), train_rows=int(rows.shape[0]),
+                train_s=train_s, rho=0.2, mlp_every=every,
+                fp64_oracle_error=fp64_error(name, res.us_soa[:, :, :1000].cpu().numpy()) if cfg["N"] >= 1000 else None,
+                note="value = trajectories / median kernel time (HIP events), same inputs and tolerances as the PI line")
 
 
 def fp64_error(name, us):
@@ -321,6 +354,13 @@ def main():
                              "what": "host SoA arrays -> H2D -> solve -> D2H of ts, us, retcode (PCIe-inclusive)"}
     if not args.no_extras:
         out["jit_compile_s"] = jit_compile_seconds(deg, f, n, np_, alg, dev)
+    if not args.no_learned and spec["alg"] in ("GPUTsit5", "GPUVern7") and name != "lorenz_uneven":
+        # learned vs PI on the same GPU, inputs and tolerances; the PI line above is `value`
+        lrn = learned_leg(deg, torch, name, f, cfg, prob, alg, dev, kw, args.mlp_every)
+        pi_ms = float(np.mean(kernel_ms))
+        lrn["speedup_vs_pi"] = pi_ms / lrn["kernel_ms"]
+        lrn["attempts_vs_pi"] = lrn["attempts"] / attempts
+        out["learned"] = lrn
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(name, cfg, threads, spec["cpu_sample"])

@@ -75,7 +75,7 @@ class SolveArgs(ctypes.Structure):
         ("steplog_count", ctypes.c_void_p),
         ("steplog_cap", ctypes.c_int64),
         ("chunk", ctypes.c_int32),
-        ("pad1", ctypes.c_int32),
+        ("mlp_every", ctypes.c_int32),
         ("order", ctypes.c_void_p),
     ]
 

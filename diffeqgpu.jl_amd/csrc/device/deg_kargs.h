@@ -29,6 +29,6 @@ struct deg_kargs {
     int save_everystep;
     int refill;               // persistent grids: idle lanes per wave that trigger a refill (1..64)
     int chunk;                // persistent grids: ids claimed per queue atomic
-    int pad1;
+    int mlp_every;            // learned: proposer evaluated every k-th loop iteration
     const int* order;         // optional start order: k-th trajectory started = id_offset + order[k]
 };

@@ -528,6 +528,10 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
     long long attempts = 0;
     bool have_k = false;
     bool fsal_pending = false;
+#if DEG_LEARNED
+    float zml_keep = 0.0f;  // the proposer's last correction for this lane
+    int mlp_tick = 0;       // wave-uniform iteration counter modulo mlp_every
+#endif
 #if DEG_LEARNED || DEG_STEPLOG
     float lx_prev = -13.2877123795f;  // log2 EEst of the previous attempt (log2 qoldinit at first)
     float ldt_prev = 0.0f;            // log2 dt of the previous attempt
@@ -634,6 +638,9 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
             lx_prev = -13.2877123795f;
             ldt_prev = 0.0f;
             first_attempt = true;
+#endif
+#if DEG_LEARNED
+            zml_keep = 0.0f;  // a new trajectory starts from the plain PI proposal
 #endif
 #if DEG_STEPLOG
             have_prev = false;
@@ -880,8 +887,11 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
 #if DEG_LEARNED
         }  // phase A
 
-        // every lane of the wavefront is here (the loop exit is wave-uniform in this variant)
-        zml = deg_mlp_eval((const deg_mlp_packed*)a.mlp, feat);
+        // every lane of the wavefront is here (the loop exit is wave-uniform in this variant);
+        // between evaluations each lane keeps its last correction
+        if (mlp_tick == 0) zml_keep = deg_mlp_eval((const deg_mlp_packed*)a.mlp, feat);
+        if (++mlp_tick >= a.mlp_every) mlp_tick = 0;
+        zml = zml_keep;
 
         if (have) {
         if (attempted) control();

@@ -109,7 +109,7 @@ def _check_unsupported(callback, tstops):
 
 def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, abstol, reltol, saveat_np,
             nsave, save_everystep, t0_fill, timing=False, stream=None, mlp=None, steplog=None, block_size=None,
-            grid_blocks=0, outputs=None, order=None):
+            grid_blocks=0, outputs=None, order=None, mlp_every=1):
     """Launch one solve. ``outputs`` (optional) supplies preallocated device tensors
     {"ts", "us_soa", "retcode", "naccept", "nreject", "saveat"} so that the call allocates and
     copies nothing (required inside a graph capture). ``order`` (optional, int32 device tensor)
@@ -201,6 +201,7 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
         a.elapsed_ms = ctypes.addressof(box)
     if mlp is not None:
         a.mlp = mlp.data_ptr()
+        a.mlp_every = int(mlp_every)
     if order is not None:
         if order.dtype != torch.int32 or order.numel() != N or order.device != device:
             raise ValueError("order must be an int32 tensor of N trajectory indices on the problems' device")

@@ -96,7 +96,8 @@ typedef struct deg_solve_args {
     uint32_t *steplog_count; /* DEG_F_STEPLOG: device row counter (zeroed by the caller) */
     int64_t steplog_cap;
     int32_t chunk;           /* persistent kernels: trajectory ids a wave claims per queue atomic; 0: 16 */
-    int32_t pad1;
+    int32_t mlp_every;       /* DEG_F_LEARNED: evaluate the proposer every k-th iteration (reusing its last
+                                correction in between); 0 or 1: every attempt */
     const int32_t *order;    /* optional [n_traj] permutation: the k-th trajectory started is id_offset + order[k]
                                 (e.g. deg_order_by_span, longest first); NULL: ascending ids */
 } deg_solve_args;
