@@ -4,7 +4,8 @@
 // buckets given by the span's leading float bits (sign 0, 8 exponent bits, 3 mantissa bits:
 // non-negative floats order like their bit patterns). Four kernels on one stream:
 // clear the counters, histogram, exclusive scan in descending bucket order (one block),
-// scatter. Order within a bucket follows the atomics and is not fixed; the solver's results do
+// scatter. Histogram and scatter count in LDS per block of 4096 trajectories, so the global
+// atomics are one per non-empty bucket per block (spans often fall into a few dozen buckets). Order within a bucket follows the atomics and is not fixed; the solver's results do
 // not depend on it (each lane's arithmetic is independent of where and when it runs).
 #include "deghip.h"
 
@@ -25,15 +26,27 @@ __device__ __forceinline__ int span_bucket(const float* t0s, const float* tfs, f
     return b < (unsigned int)kBuckets ? (int)b : kBuckets - 1;
 }
 
+constexpr int kPerBlock = 4096;  // trajectories per block in the histogram and scatter passes
+
 __global__ void __launch_bounds__(256) clear_kernel(int* __restrict__ work) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < 2 * kBuckets) work[j] = 0;
 }
 
+// per-block histogram in LDS, then one global atomic per non-empty bucket of the block
 __global__ void __launch_bounds__(256) histogram_kernel(const float* t0s, const float* tfs, float t0, float tf,
                                                         long long n, int* __restrict__ count) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&count[span_bucket(t0s, tfs, t0, tf, i)], 1);
+    __shared__ int h[kBuckets];
+    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kPerBlock;
+    for (int j = threadIdx.x; j < kPerBlock; j += blockDim.x) {
+        const long long i = base + j;
+        if (i < n) atomicAdd(&h[span_bucket(t0s, tfs, t0, tf, i)], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x)
+        if (h[b] != 0) atomicAdd(&count[b], h[b]);
 }
 
 // cursor[b] = number of trajectories in buckets above b (descending order); one block of 1024
@@ -56,12 +69,31 @@ __global__ void __launch_bounds__(1024) scan_kernel(const int* __restrict__ coun
     cursor[b1] = before + c0;
 }
 
+// each block reserves one contiguous range per non-empty bucket with a single global atomic,
+// then places its trajectories inside the ranges through LDS atomics
 __global__ void __launch_bounds__(256) scatter_kernel(const float* t0s, const float* tfs, float t0, float tf,
                                                       long long n, int* __restrict__ cursor, int* __restrict__ order) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const int pos = atomicAdd(&cursor[span_bucket(t0s, tfs, t0, tf, i)], 1);
-        order[pos] = (int)i;
+    __shared__ int h[kBuckets];
+    __shared__ int start[kBuckets];
+    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kPerBlock;
+    for (int j = threadIdx.x; j < kPerBlock; j += blockDim.x) {
+        const long long i = base + j;
+        if (i < n) atomicAdd(&h[span_bucket(t0s, tfs, t0, tf, i)], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) {
+        start[b] = (h[b] != 0) ? atomicAdd(&cursor[b], h[b]) : 0;
+        h[b] = 0;
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kPerBlock; j += blockDim.x) {
+        const long long i = base + j;
+        if (i < n) {
+            const int b = span_bucket(t0s, tfs, t0, tf, i);
+            order[start[b] + atomicAdd(&h[b], 1)] = (int)i;
+        }
     }
 }
 
@@ -75,7 +107,7 @@ extern "C" int deg_order_by_span(const float* t0s, const float* tfs, float t0, f
     hipStream_t st = (hipStream_t)stream;
     int* count = work;
     int* cursor = work + kBuckets;
-    const unsigned blocks = (unsigned)((n + 255) / 256);
+    const unsigned blocks = (unsigned)((n + kPerBlock - 1) / kPerBlock);
     hipLaunchKernelGGL(clear_kernel, dim3((2 * kBuckets + 255) / 256), dim3(256), 0, st, work);
     hipLaunchKernelGGL(histogram_kernel, dim3(blocks), dim3(256), 0, st, t0s, tfs, t0, tf, (long long)n, count);
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, count, cursor);

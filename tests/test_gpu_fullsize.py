@@ -1,0 +1,140 @@
+"""GPU: the four benchmark workloads at their full sizes, checked through properties that do
+not depend on the size, plus bitwise oracle parity on trajectories sampled across the whole id
+range (this exercises the persistent queue's slab and chunk boundaries at scale).
+
+  * every trajectory returns Success;
+  * saved times equal the normalised saveat grid exactly; row 0 equals u0 when saveat starts
+    at t0; end times equal the per-trajectory tend exactly (t snaps to tf);
+  * Robertson: y1 + y2 + y3 = 1 (the linear invariant every Rosenbrock step preserves) to 1e-4
+    at every save point, y3 non-decreasing and y1 non-increasing (to 1e-6);
+  * Van der Pol: |u1| stays within the limit-cycle bound 2.2 at t = 50;
+  * with the deterministic controller pow and no contraction, 1,500 sampled trajectories equal
+    the CPU oracle bit for bit.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+    return torch
+
+
+def _sample(N, k=1500, seed=0):
+    r = np.random.default_rng(seed)
+    idx = np.unique(np.concatenate([r.choice(N, k - 10, replace=False), np.arange(5), np.arange(N - 5, N)]))
+    return idx
+
+
+def _solve(deg, torch, name, dev, **extra):
+    from diffeqgpu_amd import systems, workloads
+
+    cfg = workloads.config(name)
+    f, n, np_, _ = systems.SYSTEMS[cfg["system"]]
+    probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev, tfs=cfg.get("tfs"))
+    prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
+    alg = {"tsit5": deg.GPUTsit5, "vern7": deg.GPUVern7, "rb23": deg.GPURosenbrock23}[cfg["method"]]()
+    kw = dict(dt=cfg["dt"], abstol=cfg["abstol"], reltol=cfg["reltol"])
+    if cfg.get("saveat") is not None:
+        kw["saveat"] = cfg["saveat"]
+    print(f"[gpu] full size {name} N={cfg['N']} dev={dev} {sorted(extra)}", flush=True)
+    r = deg.vectorized_asolve(probs, prob, alg, **kw, **extra)
+    torch.cuda.synchronize()
+    return cfg, r
+
+
+def _oracle_subset(cfg, idx, saveat=None):
+    kw = dict(abstol=cfg["abstol"], reltol=cfg["reltol"], detpow=True, kahan=bool(cfg.get("kahan")))
+    if saveat is not None:
+        kw["saveat"] = saveat
+    if cfg.get("tfs") is not None:
+        kw["tfs"] = cfg["tfs"][idx]
+    return pyoracle.solve(cfg["system"], cfg["method"], True, cfg["u0"][:, idx], cfg["p"][:, idx], cfg["tspan"],
+                          cfg["dt"], **kw)
+
+
+def test_lorenz_adaptive_1M(deg, torch_cuda):
+    torch = torch_cuda
+    for dev, bitwise in ((deg.HIPDevice(), False), (deg.HIPDevice(contract="off", powmode="det"), True)):
+        cfg, r = _solve(deg, torch, "lorenz_adaptive", dev)
+        N = cfg["N"]
+        rc = r.retcode.cpu().numpy()
+        assert np.all(rc == 0), np.bincount(rc)
+        ts = r.ts.cpu().numpy()
+        sv = r.saveat.cpu().numpy()
+        assert ts.shape == (11, N) and np.all(ts == sv[:, None])
+        us = r.us_soa.cpu().numpy()
+        np.testing.assert_array_equal(us[0], cfg["u0"])
+        assert np.all(np.isfinite(us))
+        if bitwise:
+            idx = _sample(N)
+            o = _oracle_subset(cfg, idx, saveat=sv)
+            np.testing.assert_array_equal(us[:, :, idx], o["us"])
+            np.testing.assert_array_equal(r.naccept.cpu().numpy()[idx], o["naccept"])
+            np.testing.assert_array_equal(r.nreject.cpu().numpy()[idx], o["nreject"])
+
+
+def test_robertson_1M_invariants(deg, torch_cuda):
+    torch = torch_cuda
+    cfg, r = _solve(deg, torch, "robertson", deg.HIPDevice())
+    rc = r.retcode.cpu().numpy()
+    assert np.all(rc == 0), np.bincount(rc)
+    us = r.us_soa.cpu().numpy().astype(np.float64)  # (20, 3, N)
+    ts = r.ts.cpu().numpy()
+    assert np.all(ts == r.saveat.cpu().numpy()[:, None])
+    assert np.abs(us.sum(axis=1) - 1.0).max() < 1e-4
+    assert np.all(np.diff(us[:, 2, :], axis=0) >= -1e-6)
+    assert np.all(np.diff(us[:, 0, :], axis=0) <= 1e-6)
+    assert np.all(us >= -1e-6)
+
+
+def test_robertson_1M_bitwise_sample(deg, torch_cuda):
+    torch = torch_cuda
+    cfg, r = _solve(deg, torch, "robertson", deg.HIPDevice(contract="off", powmode="det"))
+    idx = _sample(cfg["N"], 1000, seed=2)
+    o = _oracle_subset(cfg, idx, saveat=r.saveat.cpu().numpy())
+    np.testing.assert_array_equal(r.us_soa.cpu().numpy()[:, :, idx], o["us"])
+    np.testing.assert_array_equal(r.naccept.cpu().numpy()[idx], o["naccept"])
+
+
+def test_vdp_2M(deg, torch_cuda):
+    torch = torch_cuda
+    cfg, r = _solve(deg, torch, "vdp", deg.HIPDevice())
+    rc = r.retcode.cpu().numpy()
+    assert np.all(rc == 0), np.bincount(rc)
+    ts = r.ts.cpu().numpy()
+    assert np.all(ts[1] == F32(50.0)) and np.all(ts[0] == 0.0)
+    u = r.us_soa.cpu().numpy()
+    assert np.all(np.abs(u[1, 0]) <= 2.2)
+
+
+def test_lorenz_uneven_4M_longest_first(deg, torch_cuda):
+    torch = torch_cuda
+    from diffeqgpu_amd import systems, workloads
+
+    cfg = workloads.config("lorenz_uneven")
+    f, n, np_, _ = systems.SYSTEMS["lorenz"]
+    dev = deg.HIPDevice(contract="off", powmode="det", kahan=True)
+    probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev, tfs=cfg["tfs"])
+    prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
+    print("[gpu] full size lorenz_uneven N=4000000 det kahan longest-first", flush=True)
+    order = deg.order_by_span(probs)
+    r = deg.vectorized_asolve(probs, prob, deg.GPUVern7(), dt=cfg["dt"], abstol=1e-8, reltol=1e-8, order=order)
+    torch.cuda.synchronize()
+    rc = r.retcode.cpu().numpy()
+    assert np.all(rc == 0), np.bincount(rc)
+    ts = r.ts.cpu().numpy()
+    np.testing.assert_array_equal(ts[1], cfg["tfs"])
+    idx = _sample(cfg["N"], 300, seed=3)
+    o = _oracle_subset(cfg, idx)
+    np.testing.assert_array_equal(r.us_soa.cpu().numpy()[:, :, idx], o["us"])
+    np.testing.assert_array_equal(r.naccept.cpu().numpy()[idx], o["naccept"])
