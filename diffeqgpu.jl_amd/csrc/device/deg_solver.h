@@ -610,10 +610,23 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
         if (retire) break;
 #endif
 #endif
+        long long oi = -1;
         if (need_init) {
-            // ---- init (nonstiff/types.jl:305-402, stiff/types.jl:32-177; kernels.jl:35-45, :116-126)
             need_init = false;
-            i = a.id_offset + (a.order ? (long long)a.order[gid] : gid);
+            oi = a.order ? (long long)a.order[gid] : gid;
+            if (oi < 0 || oi >= a.n_traj) {
+                // an entry of a caller's start order outside [0, n_traj): nothing to solve
+                oi = -1;
+#if DEG_PERSISTENT
+                want = true;  // the lane takes the next id instead
+#else
+                break;
+#endif
+            }
+        }
+        if (oi >= 0) {
+            // ---- init (nonstiff/types.jl:305-402, stiff/types.jl:32-177; kernels.jl:35-45, :116-126)
+            i = a.id_offset + oi;
             DEG_FOR(c) u[c] = a.u0[(long long)c * S + i];
             _Pragma("unroll") for (int j = 0; j < DEG_NP; ++j) p[j] = a.p[(long long)j * S + i];
             t = a.t0s ? a.t0s[i] : a.t0;

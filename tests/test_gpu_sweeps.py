@@ -158,3 +158,28 @@ def test_order_by_span_and_order_independence(deg, torch_cuda):
     for k in range(2):
         for key in res[0][k]:
             np.testing.assert_array_equal(res[0][k][key], res[1][k][key])
+
+
+def test_order_entries_outside_range_are_skipped(deg, torch_cuda):
+    """A caller's start order with entries outside [0, N) solves every listed trajectory exactly
+    as without an order and leaves the rest alone (the kernel's own bounds check)."""
+    torch = torch_cuda
+    from diffeqgpu_amd import systems, workloads
+
+    f, n, np_, _ = systems.SYSTEMS["lorenz"]
+    cfg = workloads.config("lorenz_adaptive", N=2000)
+    prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
+    kw = dict(dt=0.1, abstol=1e-6, reltol=1e-6, saveat=1.0)
+    for persistent in (True, False):
+        dev = deg.HIPDevice(persistent=persistent)
+        probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev)
+        ref = deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), **kw)
+        order = np.arange(2000, dtype=np.int32)[::-1].copy()
+        order[7], order[1500] = -5, 2003
+        skipped = np.array([2000 - 1 - 7, 2000 - 1 - 1500])
+        print(f"[gpu] order with out-of-range entries persistent={persistent}", flush=True)
+        got = deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), order=torch.as_tensor(order, device="cuda"), **kw)
+        torch.cuda.synchronize()
+        keep = np.setdiff1d(np.arange(2000), skipped)
+        for a, b in ((ref.us_soa, got.us_soa), (ref.ts, got.ts), (ref.naccept, got.naccept)):
+            np.testing.assert_array_equal(a.cpu().numpy()[..., keep], b.cpu().numpy()[..., keep])
