@@ -26,6 +26,7 @@ F_FASTPOW = 1 << 6
 F_STEPLOG = 1 << 7
 F_SAVEAT_GLOBAL = 1 << 8
 F_WAVES_SHIFT = 9
+F_COMPACT = 1 << 12
 SAVEAT_LDS_CAP = 4096
 
 RET_SUCCESS, RET_DTMIN, RET_MAXITERS, RET_NONFINITE = 0, 1, 2, 3

@@ -88,6 +88,9 @@ class HIPDevice:
       blocks_per_cu persistent grids: launch this many blocks per compute unit (0: as many as
                   fit). Fewer resident lanes means more trajectories per lane, which shortens
                   the tail where lanes sit idle.
+      compact     persistent adaptive grids: once the queue is empty, waves with few
+                  trajectories left move them to other waves of the workgroup (through LDS)
+                  and leave, so fewer, fuller waves finish the tail.
     """
 
     device: int = 0
@@ -101,6 +104,7 @@ class HIPDevice:
     chunk: int = 0
     waves: int = 0
     blocks_per_cu: int = 0
+    compact: bool = False
 
     def flags(self, adaptive: bool) -> int:
         f = 0
@@ -122,6 +126,8 @@ class HIPDevice:
         persistent = self.persistent if self.persistent is not None else adaptive
         if persistent:
             f |= _lib.F_PERSISTENT
+        if self.compact and adaptive and persistent:
+            f |= _lib.F_COMPACT
         if not 0 <= self.waves <= 7:
             raise ValueError("waves must be in 0..7")
         f |= self.waves << _lib.F_WAVES_SHIFT

@@ -478,6 +478,78 @@ __device__ __forceinline__ float deg_advance(float t, float dt, float& comp) {
 #endif
 
 // ------------------------------------------------------------------------------------
+// Tail compaction inside a workgroup (DEG_COMPACT). Once a wave can claim no more ids,
+// a wave with few trajectories left hands their complete state to the workgroup's LDS
+// mailbox and leaves; waves with idle lanes adopt mailbox states. One 64-bit LDS word
+// tracks the mailbox: live waves (bits 0-7), donated states (8-15), states written (16-23),
+// states claimed (24-31), states read (32-39). A donation needs another live wave that
+// has not left; a wave leaves only while the mailbox is empty, so no state is stranded.
+// No wave waits on another: every step is one bounded CAS loop by the wave's leader lane.
+// ------------------------------------------------------------------------------------
+#ifndef DEG_COMPACT
+#define DEG_COMPACT 0
+#endif
+#ifndef DEG_COMPACT_LANES
+#define DEG_COMPACT_LANES 24
+#endif
+#define DEG_MB_SLOTS 64
+#define DEG_MB_WORDS (16 + 2 * DEG_N)
+
+__device__ __forceinline__ unsigned long long deg_mb_read(unsigned long long* mb) {
+    return *(volatile unsigned long long*)mb;
+}
+
+// the leader lane of a wave with k (1..64) trajectories left reserves the mailbox for them
+__device__ __forceinline__ int deg_mb_donate(unsigned long long* mb, int k) {
+    unsigned long long w = deg_mb_read(mb);
+    for (int it = 0; it < 64; ++it) {
+        const unsigned long long live = w & 0xffull, pend = (w >> 8) & 0xffull;
+        if (pend != 0ull || live < 2ull) return 0;
+        const unsigned long long nw = (live - 1ull) | ((unsigned long long)k << 8);  // the donor leaves
+        const unsigned long long prev = atomicCAS(mb, w, nw);
+        if (prev == w) return 1;
+        w = prev;
+    }
+    return 0;
+}
+
+// the leader lane of a wave without trajectories leaves, unless donated states still wait
+__device__ __forceinline__ int deg_mb_retire(unsigned long long* mb) {
+    unsigned long long w = deg_mb_read(mb);
+    for (int it = 0; it < 64; ++it) {
+        if (((w >> 8) & 0xffull) != 0ull) return 0;
+        const unsigned long long prev = atomicCAS(mb, w, w - 1ull);
+        if (prev == w) return 1;
+        w = prev;
+    }
+    return 0;
+}
+
+// the leader lane claims up to `room` written states; returns the count, first slot in *base
+__device__ __forceinline__ int deg_mb_take(unsigned long long* mb, int room, int* base) {
+    unsigned long long w = deg_mb_read(mb);
+    for (int it = 0; it < 64; ++it) {
+        const int ready = (int)((w >> 16) & 0xffull), taken = (int)((w >> 24) & 0xffull);
+        if (ready <= taken) return 0;
+        const int m = min(room, ready - taken);
+        const unsigned long long prev = atomicCAS(mb, w, w + ((unsigned long long)m << 24));
+        if (prev == w) {
+            *base = taken;
+            return m;
+        }
+        w = prev;
+    }
+    return 0;
+}
+
+// after reading m states: the last reader frees the mailbox (keeps the live count)
+__device__ __forceinline__ void deg_mb_consumed(unsigned long long* mb, int m) {
+    const unsigned long long old = atomicAdd(mb, (unsigned long long)m << 32);
+    const unsigned long long pend = (old >> 8) & 0xffull, cons = ((old >> 32) & 0xffull) + (unsigned long long)m;
+    if (cons == pend) atomicAnd(mb, 0xffull);
+}
+
+// ------------------------------------------------------------------------------------
 // The ensemble kernel (kernels.jl:1-152), one trajectory per lane, one attempt per
 // loop iteration so that lanes stay in step and a finished lane can be refilled.
 // ------------------------------------------------------------------------------------
@@ -496,6 +568,14 @@ __device__ __forceinline__ float deg_advance(float t, float dt, float& comp) {
 
 extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_KERNEL(const deg_kargs a) {
     const long long S = a.stride;
+#if DEG_COMPACT
+    __shared__ unsigned long long mb_word;
+    __shared__ float mb_state[DEG_MB_SLOTS][DEG_MB_WORDS];
+    if (threadIdx.x == 0) mb_word = (unsigned long long)(blockDim.x / 64);  // every wave starts live
+#if DEG_SAVEAT_GLOBAL
+    __syncthreads();
+#endif
+#endif
     // saveat times staged in LDS: reading them then needs no vector-memory wait (which on
     // CDNA would also wait for every store still in flight)
 #if DEG_SAVEAT_GLOBAL
@@ -603,7 +683,113 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                 retire = want;  // every slab is exhausted
             }
         }
-#if DEG_LEARNED
+#if DEG_COMPACT
+        (void)retire;
+        {
+            // tail compaction: only once this wave can claim no more ids
+            bool wave_exit = false;
+            // (lanes that were just handed an id still have to initialise it: not this iteration)
+            if (qtries >= DEG_NQUEUE && __ballot(need_init) == 0ull) {
+                const unsigned long long hm = __ballot(have);
+                const unsigned long long wm = __ballot(want);
+                const int ldr = __ffsll((unsigned long long)__ballot(1)) - 1;
+                const int nact = __popcll(hm);
+                const unsigned long long below = (1ull << lane) - 1ull;
+                // one plain read of the mailbox word decides whether any atomic is worth trying
+                const unsigned long long mw = deg_mb_read(&mb_word);
+                const bool mb_free = ((mw >> 8) & 0xffull) == 0ull;
+                const bool mb_has = ((mw >> 16) & 0xffull) > ((mw >> 24) & 0xffull);
+                if (nact == 0 && !mb_free) {
+                    // donated states are on their way: take them below
+                } else if (nact == 0) {
+                    int ok = 0;
+                    if (lane == ldr) ok = deg_mb_retire(&mb_word);
+                    wave_exit = __shfl(ok, ldr) != 0;
+                } else if (nact <= DEG_COMPACT_LANES && mb_free && (mw & 0xffull) >= 2ull) {
+                    int ok = 0;
+                    if (lane == ldr) ok = deg_mb_donate(&mb_word, nact);
+                    if (__shfl(ok, ldr) != 0) {
+                        if (have) {
+                            float* st = mb_state[__popcll(hm & below)];
+                            st[0] = __builtin_bit_cast(float, (int)(i & 0xffffffffll));
+                            st[1] = __builtin_bit_cast(float, (int)(i >> 32));
+                            st[2] = t;
+                            st[3] = tf;
+                            st[4] = dt;
+                            st[5] = tc;
+                            st[6] = qold;
+                            st[7] = qpow;
+                            st[8] = __builtin_bit_cast(float, cur);
+                            st[9] = __builtin_bit_cast(float, nacc);
+                            st[10] = __builtin_bit_cast(float, nrej);
+                            st[11] = __builtin_bit_cast(float, (int)(attempts & 0xffffffffll));
+                            st[12] = __builtin_bit_cast(float, (int)(attempts >> 32));
+                            st[13] = __builtin_bit_cast(float, have_k ? 1 : 0);
+                            st[14] = tprev;
+                            st[15] = dtacc;
+                            DEG_FOR(c) st[16 + c] = u[c];
+#if DEG_METHOD == 0
+                            // the k1 of the next attempt: the FSAL stage after an accepted step,
+                            // the unchanged k1 after a rejected one
+                            DEG_FOR(c) st[16 + DEG_N + c] = fsal_pending ? k[7][c] : k[1][c];
+#endif
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                        if (lane == ldr) atomicAdd(&mb_word, (unsigned long long)nact << 16);  // written
+                        have = false;
+                        wave_exit = true;
+                    }
+                }
+                if (!wave_exit && wm != 0ull && mb_has) {
+                    int base = 0, m = 0;
+                    if (lane == ldr) m = deg_mb_take(&mb_word, __popcll(wm), &base);
+                    m = __shfl(m, ldr);
+                    base = __shfl(base, ldr);
+                    if (m > 0) {
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                        const int rk = __popcll(wm & below);
+                        if (want && rk < m) {
+                            const float* st = mb_state[base + rk];
+                            i = (long long)(unsigned int)__builtin_bit_cast(int, st[0]) |
+                                ((long long)__builtin_bit_cast(int, st[1]) << 32);
+                            t = st[2];
+                            tf = st[3];
+                            dt = st[4];
+                            tc = st[5];
+                            qold = st[6];
+                            qpow = st[7];
+                            cur = __builtin_bit_cast(int, st[8]);
+                            nacc = __builtin_bit_cast(int, st[9]);
+                            nrej = __builtin_bit_cast(int, st[10]);
+                            attempts = (long long)(unsigned int)__builtin_bit_cast(int, st[11]) |
+                                       ((long long)__builtin_bit_cast(int, st[12]) << 32);
+                            const int fl = __builtin_bit_cast(int, st[13]);
+                            have_k = (fl & 1) != 0;
+                            fsal_pending = false;  // k1 arrives ready to use
+                            tprev = st[14];
+                            dtacc = st[15];
+                            DEG_FOR(c) u[c] = st[16 + c];
+#if DEG_METHOD == 0
+                            DEG_FOR(c) k[1][c] = st[16 + DEG_N + c];
+#endif
+                            _Pragma("unroll") for (int j = 0; j < DEG_NP; ++j) p[j] = a.p[(long long)j * S + i];
+                            status = DEG_RET_SUCCESS;
+                            if (use_saveat) {
+                                tnext = (cur < a.nsaveat) ? svp[cur] : finf;
+                                ts_w = a.ts + (long long)cur * S + i;
+                                us_w = a.us + (long long)cur * DEG_N * S + i;
+                            }
+                            have = true;
+                            want = false;
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // reads done
+                        if (lane == ldr) deg_mb_consumed(&mb_word, m);
+                    }
+                }
+            }
+            if (wave_exit) break;  // wave-uniform
+        }
+#elif DEG_LEARNED
         // the matrix-core proposer needs every lane: the wave leaves together
         if (__ballot(!retire) == 0ull) break;
 #else

@@ -48,6 +48,9 @@ extern "C" {
 #define DEG_F_FASTPOW (1u << 6)    /* controller powers by hardware log2/exp2 */
 #define DEG_F_STEPLOG (1u << 7)    /* write one log row per attempt (training data) */
 #define DEG_F_SAVEAT_GLOBAL (1u << 8) /* saveat times read from global memory (needed above 4096 points) */
+/* tail compaction: once the queue is empty, waves with few trajectories left hand their states to
+ * another wave of the workgroup through LDS and leave (adaptive persistent kernels) */
+#define DEG_F_COMPACT (1u << 12)
 /* register cap: compile for at least w (1..7) resident waves per SIMD (bits 9..11; 0: no cap) */
 #define DEG_F_WAVES_SHIFT 9
 #define DEG_F_WAVES(w) ((uint32_t)(w) << DEG_F_WAVES_SHIFT)
