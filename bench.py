@@ -230,7 +230,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); trajectories are independent and "
-              "the multi-GPU split is a static shard of trajectory ids (see DESIGN.md)", file=sys.stderr)
+              "a multi-GPU run would give each GPU a contiguous range of trajectory ids", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -32,10 +32,12 @@ def variants():
             for dev, adaptive in devs:
                 out.append((name, src, n, np_, m, dev.flags(adaptive)))
             if name in ("lorenz", "vdp", "robertson"):
-                # register caps the autotuner tries on the benchmark systems
-                for w in (4, 5, 6, 7):
+                # register caps the autotuner tries on the benchmark systems, tail compaction
+                for w in (0, 4, 5, 6, 7):
                     for kahan in ((False, True) if m == _lib.DEG_VERN7 else (False,)):
-                        out.append((name, src, n, np_, m, HIPDevice(waves=w, kahan=kahan).flags(True)))
+                        if w:
+                            out.append((name, src, n, np_, m, HIPDevice(waves=w, kahan=kahan).flags(True)))
+                        out.append((name, src, n, np_, m, HIPDevice(waves=w, kahan=kahan, compact=True).flags(True)))
             if m != _lib.DEG_RB23:
                 # the learned proposer and the step-log writer (default adaptive flags)
                 base = HIPDevice().flags(True)
