@@ -27,6 +27,7 @@ F_STEPLOG = 1 << 7
 F_SAVEAT_GLOBAL = 1 << 8
 F_WAVES_SHIFT = 9
 F_COMPACT = 1 << 12
+F_BATCH_SAVES = 1 << 13
 SAVEAT_LDS_CAP = 4096
 
 RET_SUCCESS, RET_DTMIN, RET_MAXITERS, RET_NONFINITE = 0, 1, 2, 3

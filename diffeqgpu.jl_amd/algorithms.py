@@ -88,6 +88,8 @@ class HIPDevice:
       blocks_per_cu persistent grids: launch this many blocks per compute unit (0: as many as
                   fit). Fewer resident lanes means more trajectories per lane, which shortens
                   the tail where lanes sit idle.
+      batch_saves adaptive GPUTsit5 (n <= 4) with saveat: stash steps that cross save points in
+                  LDS and interpolate for many lanes at once (same results).
       compact     persistent adaptive grids: once the queue is empty, waves with few
                   trajectories left move them to other waves of the workgroup (through LDS)
                   and leave, so fewer, fuller waves finish the tail.
@@ -105,6 +107,7 @@ class HIPDevice:
     waves: int = 0
     blocks_per_cu: int = 0
     compact: bool = False
+    batch_saves: bool = False
 
     def flags(self, adaptive: bool) -> int:
         f = 0
@@ -128,6 +131,8 @@ class HIPDevice:
             f |= _lib.F_PERSISTENT
         if self.compact and adaptive and persistent:
             f |= _lib.F_COMPACT
+        if self.batch_saves and adaptive:
+            f |= _lib.F_BATCH_SAVES
         if not 0 <= self.waves <= 7:
             raise ValueError("waves must be in 0..7")
         f |= self.waves << _lib.F_WAVES_SHIFT

@@ -492,6 +492,17 @@ __device__ __forceinline__ float deg_advance(float t, float dt, float& comp) {
 #ifndef DEG_COMPACT_LANES
 #define DEG_COMPACT_LANES 24
 #endif
+// Batched dense output (DEG_BATCH_SAVES, adaptive Tsit5): a lane whose accepted step crosses
+// save points copies the step's dense-output data to its LDS stash; the wave interpolates
+// for all stashed lanes together once DEG_SAVE_BATCH lanes wait (or a lane needs its stash
+// again or ends), instead of running the interpolation branch in nearly every iteration.
+#ifndef DEG_BATCH_SAVES
+#define DEG_BATCH_SAVES 0
+#endif
+#define DEG_STASH (((8 * DEG_N + 2) + 3) & ~3)
+#ifndef DEG_SAVE_BATCH
+#define DEG_SAVE_BATCH 16
+#endif
 #define DEG_MB_SLOTS 64
 #define DEG_MB_WORDS (16 + 2 * DEG_N)
 
@@ -578,14 +589,25 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
 #endif
     // saveat times staged in LDS: reading them then needs no vector-memory wait (which on
     // CDNA would also wait for every store still in flight)
+    extern __shared__ float deg_dyn[];  // sized at launch: saveat times, then the save stash
 #if DEG_SAVEAT_GLOBAL
     const float* __restrict__ svp = a.saveat;  // more save points than the LDS cache holds
 #else
-    extern __shared__ float svp[];  // nsaveat floats, sized at launch
+    float* svp = deg_dyn;  // nsaveat floats
     if (a.saveat != nullptr) {
         for (int j = threadIdx.x; j < a.nsaveat; j += blockDim.x) svp[j] = a.saveat[j];
     }
     __syncthreads();
+#endif
+#if DEG_BATCH_SAVES
+    // this lane's stash of one accepted step's dense-output data (uprev, k1..k7, dt, tprev)
+#if DEG_SAVEAT_GLOBAL
+    float* const stash = deg_dyn + (long long)threadIdx.x * DEG_STASH;
+#else
+    float* const stash = deg_dyn + ((a.nsaveat + 3) & ~3) + threadIdx.x * DEG_STASH;
+#endif
+    bool stashed = false;  // the stash holds saves [plo, phi) not yet written
+    int plo = 0, phi = 0;
 #endif
 #if DEG_PERSISTENT
     long long gid = 0;
@@ -871,6 +893,36 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
         // (A) the stages and the error estimate, (B) the step controller, the saves and the
         // endpoint rules. In the learned variant the lane block is closed between A and B so
         // that the proposer runs on the matrix cores with all lanes of the wavefront.
+#if DEG_BATCH_SAVES
+        // interpolate every save point of the stashed step: rows [plo, phi)
+        auto flush_stash = [&]() {
+            float su[DEG_N], sk[DEG_NK + 1][DEG_N];
+            DEG_FOR(c) su[c] = stash[c];
+            _Pragma("unroll") for (int j = 1; j <= 7; ++j) DEG_FOR(c) sk[j][c] = stash[DEG_N * j + c];
+            const float sdt = stash[8 * DEG_N], stp = stash[8 * DEG_N + 1];
+            for (int r = plo; r < phi; ++r) {
+                const float s_r = svp[r];
+                float out[DEG_N];
+#if DEG_PARITY
+                const float th = (s_r - stp) / sdt;
+#else
+                const float th = (s_r - stp) * __builtin_amdgcn_rcpf(sdt);
+#endif
+                deg_interp(th, sdt, stp, p, su, sk, out);
+                if (r < a.nsave) {
+                    DEG_FOR(c) a.us[((long long)r * DEG_N + c) * S + i] = out[c];
+                    a.ts[(long long)r * S + i] = s_r;
+                }
+            }
+            stashed = false;
+        };
+        // batched dense output: once enough lanes of the wave hold a stash, write them all
+        // (a lane that ended while holding a stash finalises only after its stash is written)
+        if (__ballot(stashed && (status != DEG_RET_SUCCESS || !(t < tf))) != 0ull ||
+            __popcll(__ballot(stashed)) >= DEG_SAVE_BATCH) {
+            if (stashed) flush_stash();
+        }
+#endif
         bool done = false;
         bool accepted = false;
 #if DEG_ADAPTIVE
@@ -994,6 +1046,9 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
         {
 #endif
         done = !(t < tf);
+#if DEG_BATCH_SAVES
+        if (status != DEG_RET_SUCCESS) done = true;  // finalisation deferred for a stash
+#endif
         if (!done) {
             if (attempts >= a.max_attempts) {
                 status = DEG_RET_MAXITERS;
@@ -1095,10 +1150,53 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
         if (have) {
         if (attempted) control();
 #endif
+#if DEG_BATCH_SAVES
+        if (accepted && use_saveat) {
+            const bool crossing = tnext <= t;
+            const bool fin = (status != DEG_RET_SUCCESS) || !(t < tf);
+            if (crossing) {
+                if (stashed || fin) {
+                    // the stash is taken or the lane ends this iteration: interpolate this step
+                    // from the registers now (the stashed step is written below)
+                    while (tnext <= t) {
+                        float out[DEG_N];
+#if DEG_PARITY
+                        const float th = (tnext - tprev) / dtacc;
+#else
+                        const float th = (tnext - tprev) * __builtin_amdgcn_rcpf(dtacc);
+#endif
+                        deg_interp(th, dtacc, tprev, p, uprev, k, out);
+                        if (cur < a.nsave) {
+                            DEG_FOR(c) us_w[(long long)c * S] = out[c];
+                            *ts_w = tnext;
+                        }
+                        ++cur;
+                        ts_w += S;
+                        us_w += (long long)DEG_N * S;
+                        tnext = (cur < a.nsaveat) ? svp[cur] : finf;
+                    }
+                } else {
+                    DEG_FOR(c) stash[c] = uprev[c];
+                    _Pragma("unroll") for (int j = 1; j <= 7; ++j) DEG_FOR(c) stash[DEG_N * j + c] = k[j][c];
+                    stash[8 * DEG_N] = dtacc;
+                    stash[8 * DEG_N + 1] = tprev;
+                    plo = cur;
+                    while (tnext <= t) {
+                        ++cur;
+                        tnext = (cur < a.nsaveat) ? svp[cur] : finf;
+                    }
+                    phi = cur;
+                    ts_w = a.ts + (long long)cur * S + i;
+                    us_w = a.us + (long long)cur * DEG_N * S + i;
+                    stashed = true;
+                }
+            }
+        }
+#endif
         if (accepted) {
             // ---- savevalues! (integrator_utils.jl:13-50)
             if (use_saveat) {
-#if DEG_HAS_INTERP
+#if DEG_HAS_INTERP && !DEG_BATCH_SAVES
                 while (tnext <= t) {
                     float out[DEG_N];
 #if DEG_PARITY
@@ -1130,7 +1228,11 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
         }
         if (!done) done = (status != DEG_RET_SUCCESS) || !(t < tf);
 
+#if DEG_BATCH_SAVES
+        if (done && !stashed) {
+#else
         if (done) {
+#endif
             // ---- endpoint rules (kernels.jl:53-62 / :133-142) and unsaved rows
             if (use_saveat) {
                 for (int r = cur; r < a.nsave; ++r) a.ts[(long long)r * S + i] = a.t0;

@@ -38,6 +38,10 @@ def variants():
                         if w:
                             out.append((name, src, n, np_, m, HIPDevice(waves=w, kahan=kahan).flags(True)))
                         out.append((name, src, n, np_, m, HIPDevice(waves=w, kahan=kahan, compact=True).flags(True)))
+            if m == _lib.DEG_TSIT5 and n <= 4:
+                for dev in (HIPDevice(batch_saves=True), HIPDevice(batch_saves=True, contract="off", powmode="det"),
+                            HIPDevice(batch_saves=True, persistent=False)):
+                    out.append((name, src, n, np_, m, dev.flags(True)))
             if m != _lib.DEG_RB23:
                 # the learned proposer and the step-log writer (default adaptive flags)
                 base = HIPDevice().flags(True)

@@ -51,6 +51,9 @@ extern "C" {
 /* tail compaction: once the queue is empty, waves with few trajectories left hand their states to
  * another wave of the workgroup through LDS and leave (adaptive persistent kernels) */
 #define DEG_F_COMPACT (1u << 12)
+/* batched dense output (adaptive GPUTsit5, n <= 4): a step that crosses save points is stashed in
+ * LDS and a wave interpolates for many stashed lanes at once; results are unchanged */
+#define DEG_F_BATCH_SAVES (1u << 13)
 /* register cap: compile for at least w (1..7) resident waves per SIMD (bits 9..11; 0: no cap) */
 #define DEG_F_WAVES_SHIFT 9
 #define DEG_F_WAVES(w) ((uint32_t)(w) << DEG_F_WAVES_SHIFT)
