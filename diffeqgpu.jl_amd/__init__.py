@@ -6,6 +6,8 @@ suite, the benchmark and the build entry point load it as ``diffeqgpu_amd``.
 """
 from .algorithms import (EnsembleGPUKernel, EnsembleKernelAlgorithm, GPUODEAlgorithm, GPURosenbrock23, GPUTsit5,
                          GPUVern7, HIPDevice, alg_order)
+from .callbacks import (CallbackSet, ContinuousCallback, DiscreteCallback, GPUContinuousCallback, GPUDiscreteCallback,
+                        LeftRootFind, NoRootFind, RightRootFind, terminate, u_modified)
 from .ensemble import EnsembleContext, EnsembleSolution, ODESolution, solve
 from .lowerlevel import SolveResult, order_by_span, vectorized_asolve, vectorized_solve
 from .problems import (EnsembleProblem, HIPProbVector, ImmutableODEProblem, ODEFunction, ODEProblem, adapt,
@@ -21,5 +23,7 @@ __all__ = [
     "HIPDevice", "alg_order", "EnsembleContext", "EnsembleSolution", "ODESolution", "solve", "SolveResult", "order_by_span",
     "vectorized_asolve", "vectorized_solve", "EnsembleProblem", "HIPProbVector", "ImmutableODEProblem",
     "ODEFunction", "ODEProblem", "adapt", "get_backend", "make_prob_compatible", "remake", "colon", "linrange",
-    "SVector", "math", "Draw", "ParameterSweep", "autotune", "learned",
+    "SVector", "math", "Draw", "ParameterSweep", "autotune", "learned", "CallbackSet", "ContinuousCallback",
+    "DiscreteCallback", "GPUContinuousCallback", "GPUDiscreteCallback", "LeftRootFind", "NoRootFind", "RightRootFind",
+    "terminate", "u_modified",
 ]

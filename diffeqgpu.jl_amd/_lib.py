@@ -28,11 +28,13 @@ F_SAVEAT_GLOBAL = 1 << 8
 F_WAVES_SHIFT = 9
 F_COMPACT = 1 << 12
 F_BATCH_SAVES = 1 << 13
+F_TSTOPS = 1 << 14
+F_CALLBACKS = 1 << 15
 SAVEAT_LDS_CAP = 4096
 
-RET_SUCCESS, RET_DTMIN, RET_MAXITERS, RET_NONFINITE = 0, 1, 2, 3
+RET_SUCCESS, RET_DTMIN, RET_MAXITERS, RET_NONFINITE, RET_TERMINATED, RET_FAILURE = 0, 1, 2, 3, 4, 5
 RETCODE_NAMES = {RET_SUCCESS: "Success", RET_DTMIN: "DtLessThanMin", RET_MAXITERS: "MaxIters",
-                 RET_NONFINITE: "Unstable"}
+                 RET_NONFINITE: "Unstable", RET_TERMINATED: "Terminated", RET_FAILURE: "Failure"}
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -79,6 +81,9 @@ class SolveArgs(ctypes.Structure):
         ("chunk", ctypes.c_int32),
         ("mlp_every", ctypes.c_int32),
         ("order", ctypes.c_void_p),
+        ("tstops", ctypes.c_void_p),
+        ("ntstops", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
     ]
 
 

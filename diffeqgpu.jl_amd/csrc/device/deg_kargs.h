@@ -31,4 +31,7 @@ struct deg_kargs {
     int chunk;                // persistent grids: ids claimed per queue atomic
     int mlp_every;            // learned: proposer evaluated every k-th loop iteration
     const int* order;         // optional start order: k-th trajectory started = id_offset + order[k]
+    const float* tstops;      // [ntstops] stopping times shared by all trajectories, or null
+    int ntstops;
+    int pad0;
 };

@@ -31,8 +31,31 @@
 #define DEG_RET_DTMIN 1
 #define DEG_RET_MAXITERS 2
 #define DEG_RET_NONFINITE 3
+#define DEG_RET_TERMINATED 4  // terminate! from a callback's affect
+#define DEG_RET_FAILURE 5     // an event time before tprev (the reference raises an error there)
 
 #define DEG_FOR(c) _Pragma("unroll") for (int c = 0; c < DEG_N; ++c)
+
+// Stopping times and callbacks (integrators/integrator_utils.jl:52-479). DEG_TSTOPS comes
+// with the launch flags; the traced source that defines the callbacks also defines
+//   DEG_NDCB discrete callbacks:   bool  deg_dcb_cond(j, u, p, t),  int deg_dcb_affect(j, u, p, t)
+//   DEG_NCCB continuous callbacks: float deg_ccb_cond(j, u, p, t),  int deg_ccb_affect(j, neg, u, p, t)
+//   and for each continuous callback deg_ccb_mode(j) (bit 0 affect!, bit 1 affect_neg!,
+//   bits 2-3 root finding: 0 none, 1 left, 2 right), deg_ccb_abstol(j), deg_ccb_nudge(j)
+//   (repeat_nudge) and deg_ccb_dtrelax(j).
+// An affect returns DEG_AFF_* bits.
+#ifndef DEG_TSTOPS
+#define DEG_TSTOPS 0
+#endif
+#ifndef DEG_NDCB
+#define DEG_NDCB 0
+#endif
+#ifndef DEG_NCCB
+#define DEG_NCCB 0
+#endif
+#define DEG_CALLBACKS ((DEG_NDCB + DEG_NCCB) > 0)
+#define DEG_AFF_TERMINATE 1  // terminate!(integrator)
+#define DEG_AFF_KEEP_FSAL 2  // u_modified!(integrator, false)
 
 // ------------------------------------------------------------------------------------
 // Julia Float32 semantics
@@ -143,6 +166,8 @@ __device__ __forceinline__ float scaled_error2_fast(float (&e)[DEG_N], const flo
 #define DEG_GAMMA ((float)(9.0 / 10.0))
 #define DEG_QOLDINIT ((float)1.0e-4)
 #define DEG_DTMIN 1.0e-14f
+// a stopping time within 100 eps(Float32) past the end of the step is taken by it
+#define DEG_TSTOP_SLACK (100.0f * 1.1920928955078125e-7f)
 
 // ------------------------------------------------------------------------------------
 // Tsit5 (gpu_tsit5_perform_step.jl:36-49, :104-122; interpolants.jl:372-387)
@@ -460,6 +485,26 @@ __device__ __forceinline__ float deg_fast_control(float E2, bool accept, float& 
     return factor;
 }
 
+// Julia's sign(x) for Float32: -1, +1, or x itself for zeros and NaN
+__device__ __forceinline__ float jl_sign(float x) { return (x < 0.0f) ? -1.0f : ((x > 0.0f) ? 1.0f : x); }
+
+// Julia's eps(x) for Float32: the spacing 2^(exponent(x) - 23) for normal x, the smallest
+// subnormal below floatmin, NaN for Inf and NaN
+__device__ __forceinline__ float jl_eps(float x) {
+    const float ax = __builtin_fabsf(x);
+    if (!(ax <= 3.40282347e38f)) return __builtin_nanf("");
+    if (ax < 1.17549435e-38f) return 1.4e-45f;
+    const int e = (int)((__builtin_bit_cast(unsigned int, ax) >> 23) & 0xffu) - 127 - 23;
+    if (e >= -126) return __builtin_bit_cast(float, (unsigned int)(e + 127) << 23);
+    return __builtin_bit_cast(float, 1u << (e + 149));
+}
+
+// Julia's nextfloat(x) for Float32 (NaN stays NaN, Inf stays Inf)
+__device__ __forceinline__ float jl_nextfloat(float x) {
+    if (x != x || x == __builtin_inff()) return x;
+    return deg_next_up(x);
+}
+
 // t <- t + dt, optionally compensated
 __device__ __forceinline__ float deg_advance(float t, float dt, float& comp) {
 #if DEG_KAHAN
@@ -630,6 +675,12 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
     long long attempts = 0;
     bool have_k = false;
     bool fsal_pending = false;
+#if DEG_TSTOPS
+    int tsi = 0;     // tstops_idx (0-based)
+#endif
+#if DEG_NCCB > 0
+    int evlast = 1;  // event_last_time: 1-based index of the last continuous event, 0 none
+#endif
 #if DEG_LEARNED
     float zml_keep = 0.0f;  // the proposer's last correction for this lane
     int mlp_tick = 0;       // wave-uniform iteration counter modulo mlp_every
@@ -855,6 +906,12 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
             status = DEG_RET_SUCCESS;
             have_k = false;
             fsal_pending = false;
+#if DEG_TSTOPS
+            tsi = 0;
+#endif
+#if DEG_NCCB > 0
+            evlast = 1;  // nonstiff/types.jl:318, stiff/types.jl:44
+#endif
 #if DEG_LEARNED || DEG_STEPLOG
             lx_prev = -13.2877123795f;
             ldt_prev = 0.0f;
@@ -1022,12 +1079,24 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                 if ((tf - t) - dt < DEG_DTMIN) {
                     t = tf;
                     tc = 0.0f;
-                } else {
+                }
+#if DEG_TSTOPS
+                else if (tsi < a.ntstops && ((a.tstops[tsi] - t) - dt) - DEG_TSTOP_SLACK < 0.0f) {
+                    // gpu_tsit5_perform_step.jl:158-164: the step ends at the stopping time, with
+                    // the state taken from this step's dense output there
+                    t = a.tstops[tsi];
+                    tc = 0.0f;
+                    ++tsi;
+                    deg_interp((t - tprev) / dtacc, dtacc, tprev, p, uprev, k, u);
+                }
+#endif
+                else {
                     float tn = deg_advance(t, dt, tc);
 #if !DEG_KAHAN
-                    // A step below the resolution of t would repeat forever: snap to tf
-                    // when it is within two ulps, otherwise stop with DtLessThanMin.
-                    if (tn == t) {
+                    // A step below the resolution of t whose successor is no larger would
+                    // repeat forever: snap to tf when it is within two ulps, otherwise stop
+                    // with DtLessThanMin. A growing dt goes on with t unchanged.
+                    if (tn == t && !(dtnew > dt)) {
                         if (tf - t <= 2.0f * (deg_next_up(t) - t)) tn = tf;
                         else status = DEG_RET_DTMIN;
                     }
@@ -1103,9 +1172,27 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                 // ---- one fixed step (gpu_*_perform_step.jl fixed versions)
                 const float told = t;
                 const float tcomp_old = tc;
-                const float tn = deg_advance(t, dt, tc);
+                float dts = dt;  // this step's length: shorter when it ends at a stopping time
+                bool at_tstop = false;
+#if DEG_TSTOPS
+                // gpu_tsit5_perform_step.jl:18-27: the step ends at the next stopping time when that
+                // lies within dt (+ 100 eps); the integrator's dt (dense output) stays dt
+                at_tstop = tsi < a.ntstops && ((a.tstops[tsi] - t) - dt) - DEG_TSTOP_SLACK < 0.0f;
+#endif
+                float tn;
+#if DEG_TSTOPS
+                if (at_tstop) {
+                    tn = a.tstops[tsi];
+                    dts = tn - told;
+                    ++tsi;
+                    tc = 0.0f;
+                } else
+#endif
+                {
+                    tn = deg_advance(t, dt, tc);
+                }
 #if !DEG_KAHAN
-                if (tn == told) {
+                if (!at_tstop && tn == told) {
                     // the step is below the resolution of t: stop instead of looping forever
                     status = DEG_RET_DTMIN;
                     tc = tcomp_old;
@@ -1124,12 +1211,12 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                 } else {
                     DEG_FOR(c) k[1][c] = k[7][c];
                 }
-                deg_attempt(uprev, p, told, dt, k, unew);
+                deg_attempt(uprev, p, told, dts, k, unew);
 #elif DEG_METHOD == 1
                 deg_rhs(uprev, p, told, k[1]);
-                deg_attempt(uprev, p, told, dt, k, unew);
+                deg_attempt(uprev, p, told, dts, k, unew);
 #else
-                deg_rb23<false>(uprev, p, told, dt, k, unew, 0.0f, 0.0f);
+                deg_rb23<false>(uprev, p, told, dts, k, unew, 0.0f, 0.0f);
 #endif
                 DEG_FOR(c) u[c] = unew[c];
                 ++nacc;
@@ -1195,35 +1282,175 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
 #endif
         if (accepted) {
             // ---- savevalues! (integrator_utils.jl:13-50)
-            if (use_saveat) {
+            auto savevalues = [&]() {
+                if (use_saveat) {
 #if DEG_HAS_INTERP && !DEG_BATCH_SAVES
-                while (tnext <= t) {
-                    float out[DEG_N];
+                    while (tnext <= t) {
+                        float out[DEG_N];
 #if DEG_PARITY
-                    const float th = (tnext - tprev) / dtacc;
+                        const float th = (tnext - tprev) / dtacc;
 #else
-                    const float th = (tnext - tprev) * __builtin_amdgcn_rcpf(dtacc);
+                        const float th = (tnext - tprev) * __builtin_amdgcn_rcpf(dtacc);
 #endif
-                    deg_interp(th, dtacc, tprev, p, uprev, k, out);
-                    if (cur < a.nsave) {
-                        DEG_FOR(c) us_w[(long long)c * S] = out[c];
-                        *ts_w = tnext;
+                        deg_interp(th, dtacc, tprev, p, uprev, k, out);
+                        if (cur < a.nsave) {
+                            DEG_FOR(c) us_w[(long long)c * S] = out[c];
+                            *ts_w = tnext;
+                        }
+                        ++cur;
+                        ts_w += S;
+                        us_w += (long long)DEG_N * S;
+                        tnext = (cur < a.nsaveat) ? svp[cur] : finf;
                     }
-                    ++cur;
-                    ts_w += S;
-                    us_w += (long long)DEG_N * S;
-                    tnext = (cur < a.nsaveat) ? svp[cur] : finf;
+#endif
+                } else if (a.save_everystep) {
+#if !DEG_ADAPTIVE
+                    if (step_idx >= 0 && step_idx < a.nsave) {
+                        DEG_FOR(c) a.us[((long long)step_idx * DEG_N + c) * S + i] = u[c];
+                        a.ts[(long long)step_idx * S + i] = t;
+                    }
+                    ++step_idx;
+#endif
+                }
+            };
+            bool saved_in_cb = false;
+#if DEG_CALLBACKS
+            if (status == DEG_RET_SUCCESS) {
+                // ---- handle_callbacks! (integrator_utils.jl:274-325), after the accepted step
+                auto apply_affect = [&](int fl) {
+                    if (fl & DEG_AFF_TERMINATE) status = DEG_RET_TERMINATED;
+                    if (fl & DEG_AFF_KEEP_FSAL) have_k = true;  // u_modified!(integrator, false)
+                };
+                bool csaved = false;
+#if DEG_NCCB > 0
+                {
+                    float tmin = t, upc = 0.0f;
+                    bool occ = false;
+                    int idx = 1;
+                    _Pragma("unroll") for (int j = 0; j < DEG_NCCB; ++j) {
+                        // DiffEqBase.get_condition (integrator_utils.jl:460-478): the step's end
+                        // state, its start state, or the step's dense output
+                        auto cond_at = [&](float s) -> float {
+                            if (s == t) return deg_ccb_cond(j, u, p, s);
+                            if (s == tprev) return deg_ccb_cond(j, uprev, p, s);
+                            float us_[DEG_N];
+                            deg_interp((s - tprev) / dtacc, dtacc, tprev, p, uprev, k, us_);
+                            return deg_ccb_cond(j, us_, p, s);
+                        };
+                        // find_callback_time (integrator_utils.jl:386-446); last_event_error is 0
+                        float bt = tprev;
+                        float bc = deg_ccb_cond(j, uprev, p, tprev);
+                        if (evlast == j + 1 && __builtin_fabsf(bc - 0.0f) <= deg_ccb_abstol(j)) {
+                            bt = tprev + dtacc * deg_ccb_nudge(j);
+                            bc = cond_at(bt);
+                        }
+                        const float bs = jl_sign(bc);
+                        const float tsg = jl_sign(cond_at(t));
+                        const int mode = deg_ccb_mode(j);
+                        const bool oc = ((bs < 0.0f && (mode & 1)) || (bs > 0.0f && (mode & 2))) && bs * tsg <= 0.0f;
+                        const int rf = (mode >> 2) & 3;
+                        float tj = t;
+                        if (oc && rf != 0 && tsg != 0.0f) {
+                            // gpu_find_root (integrator_utils.jl:327-384): ITP, k1 = 0.2/span0, k2 = 2, n0 = 10
+                            float left = bt, right = t;
+                            float fl = cond_at(left), fr = cond_at(right);
+                            const float span0 = right - left;
+                            const float k1 = 0.2f / span0;
+                            float eps_s = span0 * 512.0f;
+                            for (int it = 0; it < 100; ++it) {
+                                const float span = right - left;
+                                const float mid = (left + right) / 2.0f;
+                                const float r = eps_s - span / 2.0f;
+                                const float xf = left + (span * fl) / (fl - fr);
+                                const float delta = jl_max((k1 * span) * span, jl_eps(xf));
+                                const float diff = mid - xf;
+                                const float xt = (delta <= __builtin_fabsf(diff)) ? xf + __builtin_copysignf(delta, diff) : mid;
+                                const float xp = (__builtin_fabsf(xt - mid) <= r) ? xt : mid - __builtin_copysignf(r, diff);
+                                const float yp = cond_at(xp);
+                                const float yps = yp * jl_sign(fr);
+                                if (yps > 0.0f) {
+                                    right = xp;
+                                    fr = yp;
+                                } else if (yps < 0.0f) {
+                                    left = xp;
+                                    fl = yp;
+                                } else {
+                                    left = xp;
+                                    right = xp;
+                                    break;
+                                }
+                                eps_s = eps_s / 2.0f;
+                                if (jl_nextfloat(left) >= right) break;
+                            }
+                            tj = (rf == 1) ? left : right;
+                        }
+                        // DiffEqBase.find_first_continuous_callback: the earliest event, the first
+                        // callback on ties
+                        if (j == 0) {
+                            tmin = tj;
+                            upc = bs;
+                            occ = oc;
+                            idx = 1;
+                        } else if (oc && (tj < tmin || !occ)) {
+                            tmin = tj;
+                            upc = bs;
+                            occ = true;
+                            idx = j + 1;
+                        }
+                    }
+                    if (occ) {
+                        evlast = idx;
+                        // apply_callback! (integrator_utils.jl:232-272). The affect applied is the
+                        // first continuous callback's, whichever callback found the event.
+                        if (tmin < tprev) {
+                            status = DEG_RET_FAILURE;  // change_t_via_interpolation! raises here
+                        } else {
+                            if (tmin != t) {
+                                // _change_t_via_interpolation! (integrator_utils.jl:174-201)
+                                deg_interp((tmin - tprev) / dtacc, dtacc, tprev, p, uprev, k, u);
+                                step_idx -= (int)__builtin_rintf((t - tmin) / dtacc);
+                                t = tmin;
+                                tc = 0.0f;
+                            }
+#if DEG_ADAPTIVE
+                            if (dt < 1.0e-12f) dt = jl_min(deg_ccb_dtrelax(0) * dtacc, __builtin_fabsf(tf - t));
+#endif
+                            savevalues();
+                            csaved = true;
+                            have_k = false;  // u_modified = true
+                            const int mode0 = deg_ccb_mode(0);
+                            if (upc < 0.0f) {
+                                if (mode0 & 1) apply_affect(deg_ccb_affect(0, 0, u, p, t));
+                                else have_k = true;
+                            } else if (upc > 0.0f) {
+                                if (mode0 & 2) apply_affect(deg_ccb_affect(0, 1, u, p, t));
+                                else have_k = true;
+                            }
+                        }
+                    } else {
+                        evlast = 0;
+                    }
                 }
 #endif
-            } else if (a.save_everystep) {
-#if !DEG_ADAPTIVE
-                if (step_idx < a.nsave) {
-                    DEG_FOR(c) a.us[((long long)step_idx * DEG_N + c) * S + i] = u[c];
-                    a.ts[(long long)step_idx * S + i] = t;
+#if DEG_NDCB > 0
+                // apply_discrete_callback! (integrator_utils.jl:70-149), in order
+                bool dsaved = false;
+                _Pragma("unroll") for (int j = 0; j < DEG_NDCB; ++j) {
+                    if (deg_dcb_cond(j, u, p, t)) {
+                        savevalues();
+                        dsaved = true;
+                        have_k = false;  // u_modified = true
+                        apply_affect(deg_dcb_affect(j, u, p, t));
+                    }
                 }
-                ++step_idx;
+                saved_in_cb = dsaved;  // the discrete result replaces the continuous one
+                (void)csaved;
+#else
+                saved_in_cb = csaved;
 #endif
             }
+#endif
+            if (!saved_in_cb) savevalues();
             fsal_pending = true;
         }
         if (!done) done = (status != DEG_RET_SUCCESS) || !(t < tf);
@@ -1239,7 +1466,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
             } else if (a.save_everystep) {
                 for (int r = step_idx; r < a.nsave; ++r) a.ts[(long long)r * S + i] = a.t0;
 #if !DEG_ADAPTIVE && DEG_HAS_INTERP
-                if (t > tf && status == DEG_RET_SUCCESS && a.nsave > 1) {
+                if (t > tf && (status == DEG_RET_SUCCESS || status == DEG_RET_TERMINATED) && a.nsave > 1) {
                     // k[1] holds k7 after the FSAL shift for Tsit5; undo it for the dense output
                     float out[DEG_N];
                     const float th = (tf - tprev) / dtacc;

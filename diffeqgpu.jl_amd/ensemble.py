@@ -17,6 +17,7 @@ import numpy as np
 
 from . import _lib
 from .algorithms import EnsembleGPUKernel, GPUODEAlgorithm
+from .callbacks import merged_callback
 from .lowerlevel import vectorized_asolve, vectorized_solve
 from .problems import EnsembleProblem, adapt, make_prob_compatible
 
@@ -41,6 +42,23 @@ class ODESolution:
 
     def __getitem__(self, i):
         return self.u[i]
+
+    def __call__(self, tval):
+        """Linear interpolation between saved points (SciMLBase's LinearInterpolation of a
+        solution built without dense output): the first saved entry at tval if there is one,
+        otherwise the line between its neighbours. No extrapolation."""
+        t = np.asarray(self.t, dtype=np.float64)
+        u = np.asarray(self.u, dtype=np.float64)
+        x = float(tval)
+        if len(t) == 0:
+            raise ValueError("empty solution")
+        if x > t[-1] or x < t[0]:
+            raise ValueError("Solution interpolation cannot extrapolate outside the saved time span.")
+        i = int(np.searchsorted(t, x, side="left"))
+        if t[i] == x:
+            return u[i]
+        th = (x - t[i - 1]) / (t[i] - t[i - 1])
+        return (1.0 - th) * u[i - 1] + th * u[i]
 
 
 @dataclass
@@ -124,9 +142,11 @@ def batch_solve(ensembleprob, alg, ensemblealg, I, adaptive, seed=None, **kwargs
 
 
 def batch_solve_up_kernel(ensembleprob, probs, alg, ensemblealg, I, adaptive, **kwargs):
-    """solve.jl:382-419: adapt to the device, run the lower-level solve, copy back."""
-    if kwargs.get("callback") is not None:
-        raise NotImplementedError("callbacks are not provided by the HIP path")
+    """solve.jl:382-419: adapt to the device, run the lower-level solve, copy back. The kernel's
+    callbacks are the problem's ``callback`` keyword plus the solve's ``callback`` when
+    ``merge_callbacks`` is set (ensemblegpuarray/callbacks.jl:61-74), converted to GPU callbacks."""
+    kwargs["callback"] = merged_callback(probs[0].kwargs.get("callback"), kwargs.get("callback"),
+                                         bool(kwargs.pop("merge_callbacks", False)))
     dev = ensemblealg.dev
     gprobs = adapt(dev, probs)
     if adaptive:

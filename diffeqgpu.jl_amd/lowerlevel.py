@@ -18,6 +18,7 @@ import numpy as np
 
 from . import _lib
 from .algorithms import GPUODEAlgorithm, GPURosenbrock23, HIPDevice
+from .callbacks import emit_callbacks, gpu_callback_set
 from .problems import HIPProbVector, ODEProblem, get_backend
 from .ranges import JlRange, colon, linrange
 
@@ -100,20 +101,32 @@ def _rep_tspan(prob) -> tuple:
     raise TypeError("prob must be an ODEProblem")
 
 
-def _check_unsupported(callback, tstops):
-    if callback is not None:
-        raise NotImplementedError("callbacks are not provided by the HIP path")
-    if tstops is not None:
-        raise NotImplementedError("tstops are not provided by the HIP path")
+def _callback_set(callback):
+    """The GPU CallbackSet for the kernel (SciML callbacks are converted as solve.jl:386-397
+    does), or None when there are no callbacks."""
+    if callback is None:
+        return None
+    cs = gpu_callback_set(callback)
+    return cs if len(cs) else None
+
+
+def _tstops(tstops) -> Optional[np.ndarray]:
+    """``tstops`` as the Float32 vector the kernel walks in order, or None when empty."""
+    if tstops is None:
+        return None
+    a = np.asarray(list(tstops) if isinstance(tstops, (range, JlRange)) else tstops, dtype=np.float64)
+    a = a.astype(np.float32).reshape(-1)
+    return a if a.size else None
 
 
 def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, abstol, reltol, saveat_np,
             nsave, save_everystep, t0_fill, timing=False, stream=None, mlp=None, steplog=None, block_size=None,
-            grid_blocks=0, outputs=None, order=None, mlp_every=1):
+            grid_blocks=0, outputs=None, order=None, mlp_every=1, callback=None, tstops_np=None):
     """Launch one solve. ``outputs`` (optional) supplies preallocated device tensors
     {"ts", "us_soa", "retcode", "naccept", "nreject", "saveat"} so that the call allocates and
     copies nothing (required inside a graph capture). ``order`` (optional, int32 device tensor)
-    is the start order of the persistent queue, e.g. from ``order_by_span``."""
+    is the start order of the persistent queue, e.g. from ``order_by_span``. ``callback`` is a
+    CallbackSet of GPU callbacks or None; ``tstops_np`` a Float32 vector or None."""
     import torch
 
     if not isinstance(alg, GPUODEAlgorithm):
@@ -126,8 +139,18 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
         flags |= _lib.F_STEPLOG
     if saveat_np is not None and len(saveat_np) > _lib.SAVEAT_LDS_CAP:
         flags |= _lib.F_SAVEAT_GLOBAL
+    if tstops_np is not None:
+        flags |= _lib.F_TSTOPS
+    if callback is not None:
+        flags |= _lib.F_CALLBACKS
+    if flags & (_lib.F_TSTOPS | _lib.F_CALLBACKS):
+        if flags & (_lib.F_LEARNED | _lib.F_STEPLOG | _lib.F_COMPACT | _lib.F_BATCH_SAVES):
+            raise ValueError("tstops and callbacks do not combine with the learned proposer, step logs, "
+                             "tail compaction or batched saves")
     with_jac = isinstance(alg, GPURosenbrock23)
     src, _flops = probs.f.hip_source(probs.n, probs.np, with_jac)
+    if callback is not None:
+        src = src + emit_callbacks(callback, probs.n, probs.np)
     ctx = _lib.context(dev.device)
     kern = ctx.kernel(src, probs.n, probs.np, alg.method, flags)
     device = probs.u0.device
@@ -211,6 +234,15 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
         a.steplog = buf.data_ptr()
         a.steplog_count = counter.data_ptr()
         a.steplog_cap = int(buf.shape[0])
+    tst = None
+    if tstops_np is not None:
+        tst = outputs.get("tstops")
+        if tst is None:
+            tst = torch.as_tensor(np.ascontiguousarray(tstops_np, dtype=np.float32)).to(device)
+        elif tst.numel() != len(tstops_np):
+            raise ValueError("preallocated tstops has the wrong length")
+        a.tstops = tst.data_ptr()
+        a.ntstops = int(tst.numel())
     kern.solve(a)
     if timing:
         ms = float(box.value)
@@ -223,32 +255,47 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
     # the result keeps the argument block and every tensor it points to alive
     return SolveResult(ts, us_soa.permute(0, 2, 1), retcode=retcode, naccept=nacc, nreject=nrej, us_soa=us_soa,
                        kernel_ms=ms, saveat=sv, kernel=kern, flags=flags, relaunch=relaunch, args=a,
-                       probs=probs)
+                       probs=probs, tstops=tst)
 
 
 def vectorized_solve(probs, prob, alg, *, dt, saveat=None, save_everystep=True, debug=False, callback=None,
                      tstops=None, timing=False, **kwargs):
-    """Fixed-step ensemble solve (lowerlevel_solve.jl:53-131). Returns ``(ts, us)``."""
-    _check_unsupported(callback, tstops)
+    """Fixed-step ensemble solve (lowerlevel_solve.jl:53-131). Returns ``(ts, us)``.
+
+    ``callback``: GPU callbacks (a callback or CallbackSet; SciML callbacks are converted).
+    ``tstops``: stopping times shared by all trajectories; each adds a row to the every-step
+    output unless it lies on the t0:dt:tf grid (lowerlevel_solve.jl:73-77)."""
     get_backend(probs)
     t0, tf = _rep_tspan(prob)
     dt32 = np.float32(dt)
     if not dt32 > 0:
         raise ValueError("dt must be positive for a fixed-step solve")
+    cbs = _callback_set(callback)
+    tst = _tstops(tstops)
     sv = normalize_saveat_solve(saveat, (t0, tf))
     if sv is None:
-        nsave = len(colon(t0, dt32, tf)) if save_everystep else 2
+        if save_everystep:
+            grid = colon(t0, dt32, tf)
+            nsave = len(grid)
+            if tst is not None:
+                on_grid = int(np.isin(grid.collect().astype(np.float32), tst).sum())
+                nsave += len(tst) - on_grid
+        else:
+            nsave = 2
     else:
         nsave = len(sv)
     return _launch(probs, alg, False, dt=dt32, abstol=0.0, reltol=0.0, saveat_np=sv, nsave=nsave,
-                   save_everystep=save_everystep and sv is None, t0_fill=t0, timing=timing, **kwargs)
+                   save_everystep=save_everystep and sv is None, t0_fill=t0, timing=timing, callback=cbs,
+                   tstops_np=tst, **kwargs)
 
 
 def vectorized_asolve(probs, prob, alg, *, dt=0.1, saveat=None, save_everystep=False, abstol=1.0e-6,
                       reltol=1.0e-3, debug=False, callback=None, tstops=None, timing=False, **kwargs):
-    """Adaptive ensemble solve (lowerlevel_solve.jl:253-346). Returns ``(ts, us)``."""
-    _check_unsupported(callback, tstops)
+    """Adaptive ensemble solve (lowerlevel_solve.jl:253-346). Returns ``(ts, us)``.
+    ``callback`` and ``tstops`` as for vectorized_solve (tstops add no output rows here)."""
     get_backend(probs)
+    cbs = _callback_set(callback)
+    tst = _tstops(tstops)
     t0, tf = _rep_tspan(prob)
     sv = normalize_saveat_asolve(saveat, (t0, tf))
     dt32 = np.float32(dt)
@@ -262,7 +309,8 @@ def vectorized_asolve(probs, prob, alg, *, dt=0.1, saveat=None, save_everystep=F
     # The adaptive integrator never saves every step (nonstiff/types.jl:340-402 passes
     # save_everystep = false); the kernel only skips the endpoint write in that case.
     return _launch(probs, alg, True, dt=dt32, abstol=abstol, reltol=reltol, saveat_np=sv, nsave=nsave,
-                   save_everystep=save_everystep and sv is None, t0_fill=t0, timing=timing, **kwargs)
+                   save_everystep=save_everystep and sv is None, t0_fill=t0, timing=timing, callback=cbs,
+                   tstops_np=tst, **kwargs)
 
 
 def order_by_span(probs, stream=None):

@@ -10,6 +10,7 @@ same association order (u and p are 0-based here, 1-based in the Julia sources):
   vdp        Van der Pol
   tdep       u' = p t u
   rot        u1' = -p u2, u2' = p u1
+  ball       u1' = u2, u2' = -p1 (test/gpu_kernel_de/gpu_ode_continuous_callbacks.jl:6-10)
 """
 from .problems import ODEFunction
 from .tracer import SVector
@@ -76,6 +77,14 @@ def rot(u, p, t):
     return SVector(-p[0] * u[1], p[0] * u[0])
 
 
+def ball(u, p, t):
+    return SVector(u[1], -p[0])
+
+
+def ball_jac(u, p, t):
+    return [[0.0, 1.0], [0.0, 0.0]]
+
+
 # (ODEFunction, n, np, oracle system id)
 SYSTEMS = {
     "lorenz": (ODEFunction(lorenz, jac=lorenz_jac, tgrad=zero_tgrad3), 3, 3, 0),
@@ -86,4 +95,5 @@ SYSTEMS = {
     "vdp": (ODEFunction(vdp, jac=vdp_jac), 2, 1, 5),
     "tdep": (ODEFunction(tdep), 1, 1, 6),
     "rot": (ODEFunction(rot), 2, 1, 7),
+    "ball": (ODEFunction(ball, jac=ball_jac), 2, 1, 8),
 }

@@ -80,12 +80,49 @@ class Node:
         return self.g.mk("pow", (self.g.lift(b), self))
 
     def __bool__(self):
-        raise TraceError("a traced value has no truth value; use diffeqgpu_amd.math.ifelse for branches")
+        raise TraceError("a traced value has no truth value: use diffeqgpu_amd.math.ifelse for branches, and "
+                         "& | ~ (not and/or/not) to combine the comparisons of a callback condition")
 
-    def _no_compare(self, other):
-        raise TraceError("comparisons of traced values are not recorded; use diffeqgpu_amd.math.ifelse")
+    # -- comparisons: recorded only while tracing a callback condition (Graph.cmp_ok) --
+    def _cmp(self, other, op, rev=False):
+        g = self.g
+        if not g.cmp_ok:
+            raise TraceError("comparisons of traced values are recorded only in callback conditions; "
+                             "use diffeqgpu_amd.math.ifelse in a right-hand side")
+        o = g.lift(other)
+        return g.mk("cmp", (o, self) if rev else (self, o), op)
 
-    __lt__ = __le__ = __gt__ = __ge__ = _no_compare
+    def __lt__(self, o): return self._cmp(o, "<")
+    def __le__(self, o): return self._cmp(o, "<=")
+    def __gt__(self, o): return self._cmp(o, ">")
+    def __ge__(self, o): return self._cmp(o, ">=")
+
+    def __eq__(self, o):
+        if self.g.cmp_ok and _is_scalar(o):
+            return self._cmp(o, "==")
+        return self is o
+
+    def __ne__(self, o):
+        if self.g.cmp_ok and _is_scalar(o):
+            return self._cmp(o, "!=")
+        return self is not o
+
+    __hash__ = object.__hash__
+
+    # -- boolean combinations of comparisons (Julia's & | ! on Bool) --
+    def _bool_bin(self, other, op):
+        g = self.g
+        return g.mk(op, (g.lift_bool(self), g.lift_bool(other)))
+
+    def __and__(self, o): return self._bool_bin(o, "and")
+    def __rand__(self, o): return self.g.mk("and", (self.g.lift_bool(o), self.g.lift_bool(self)))
+    def __or__(self, o): return self._bool_bin(o, "or")
+    def __ror__(self, o): return self.g.mk("or", (self.g.lift_bool(o), self.g.lift_bool(self)))
+    def __invert__(self): return self.g.mk("not", (self.g.lift_bool(self),))
+
+    @property
+    def is_bool(self):
+        return self.op in _BOOL_OPS
 
     def __float__(self):
         raise TraceError("cannot convert a traced value to float inside a traced RHS")
@@ -95,12 +132,29 @@ class Node:
 
 
 _UNARY = {"neg", "sin", "cos", "exp", "log", "sqrt", "tanh", "abs"}
+_BOOL_OPS = {"cmp", "and", "or", "not", "bconst"}
+
+
+def _is_scalar(x) -> bool:
+    if isinstance(x, Node):
+        return True
+    if isinstance(x, bool):
+        return False
+    if isinstance(x, (int, float)):
+        return True
+    try:
+        import numpy as np
+
+        return isinstance(x, np.generic) and not isinstance(x, np.bool_)
+    except ImportError:  # pragma: no cover
+        return False
 
 
 class Graph:
-    def __init__(self):
+    def __init__(self, cmp_ok: bool = False):
         self.nodes: List[Node] = []
         self.table: Dict[tuple, Node] = {}
+        self.cmp_ok = cmp_ok
 
     def _key(self, op, args, val):
         return (op, tuple(a.idx for a in args), val)
@@ -121,8 +175,27 @@ class Graph:
     def const(self, x: float) -> Node:
         return self.mk("const", (), float(x))
 
+    def lift_bool(self, x) -> Node:
+        """A boolean node: a traced comparison (or combination) or a Python bool constant."""
+        if isinstance(x, Node):
+            if not x.is_bool:
+                raise TraceError("& | ~ combine comparisons, not arithmetic values")
+            return x
+        if isinstance(x, bool):
+            return self.mk("bconst", (), bool(x))
+        try:
+            import numpy as np
+
+            if isinstance(x, np.bool_):
+                return self.mk("bconst", (), bool(x))
+        except ImportError:  # pragma: no cover
+            pass
+        raise TraceError(f"expected a condition, got {type(x).__name__}")
+
     def lift(self, x) -> Node:
         if isinstance(x, Node):
+            if x.is_bool:
+                raise TraceError("a comparison is not an arithmetic value")
             return x
         if isinstance(x, bool):
             raise TraceError("booleans are not arithmetic values")
@@ -166,6 +239,52 @@ class SVector(tuple):
         return super().__new__(cls, xs)
 
 
+class Vec:
+    """The state or parameter vector an affect sees: elementwise ``+ - * /`` with another vector
+    (an SVector, list, tuple or Vec of the same length) or a scalar, the way Julia's
+    ``integrator.u + @SVector[...]`` and ``@SVector[...] .* integrator.u`` combine SVectors.
+    Items may also be assigned (``integrator.u[0] = ...``)."""
+
+    def __init__(self, g: "Graph", items: List[Node]):
+        self._g = g
+        self._items = items
+
+    def __len__(self):
+        return len(self._items)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return list(self._items[i])
+        return self._items[i]
+
+    def __setitem__(self, i, v):
+        self._items[i] = self._g.lift(v)
+
+    def __iter__(self):
+        return iter(self._items)
+
+    def _zip(self, o, fn, rev=False):
+        g = self._g
+        if isinstance(o, (Vec, SymVec, list, tuple)):
+            if len(o) != len(self._items):
+                raise TraceError(f"vector lengths differ: {len(self._items)} and {len(o)}")
+            pairs = [(g.lift(a), g.lift(b)) for a, b in zip(self._items, o)]
+        else:
+            bb = g.lift(o)
+            pairs = [(g.lift(a), bb) for a in self._items]
+        return Vec(g, [fn(b, a) if rev else fn(a, b) for a, b in pairs])
+
+    def __add__(self, o): return self._zip(o, lambda a, b: a + b)
+    def __radd__(self, o): return self._zip(o, lambda a, b: a + b, True)
+    def __sub__(self, o): return self._zip(o, lambda a, b: a - b)
+    def __rsub__(self, o): return self._zip(o, lambda a, b: a - b, True)
+    def __mul__(self, o): return self._zip(o, lambda a, b: a * b)
+    def __rmul__(self, o): return self._zip(o, lambda a, b: a * b, True)
+    def __truediv__(self, o): return self._zip(o, lambda a, b: a / b)
+    def __rtruediv__(self, o): return self._zip(o, lambda a, b: a / b, True)
+    def __neg__(self): return Vec(self._g, [-a for a in self._items])
+
+
 # ------------------------------------------------------------------------------------------
 # elementary functions usable on numbers and traced values
 # ------------------------------------------------------------------------------------------
@@ -195,8 +314,17 @@ def pow(x, y):  # noqa: A001 - mirrors the math function name
     return _pymath.pow(x, y)
 
 
-def ifelse(cond_lhs, cmp: str, cond_rhs, a, b):
-    """``cond_lhs <cmp> cond_rhs ? a : b`` with cmp in <, <=, >, >=, ==, != (both branches evaluated)."""
+def ifelse(cond_lhs, cmp, cond_rhs=None, a=None, b=None):
+    """``cond_lhs <cmp> cond_rhs ? a : b`` with cmp in <, <=, >, >=, ==, != (both branches evaluated).
+    Inside a callback condition or affect, ``ifelse(cond, a, b)`` also takes a traced comparison."""
+    if b is None and a is None and cond_rhs is not None:
+        cond, a, b = cond_lhs, cmp, cond_rhs
+        if isinstance(cond, Node):
+            g = cond.g
+            return g.mk("bselect", (g.lift_bool(cond), g.lift(a), g.lift(b)))
+        if isinstance(cond, bool):
+            return a if cond else b
+        raise TraceError("ifelse(cond, a, b) needs a traced comparison or a bool")
     if cmp not in ("<", "<=", ">", ">=", "==", "!="):
         raise TraceError(f"bad comparison {cmp!r}")
     vals = [cond_lhs, cond_rhs, a, b]
@@ -386,6 +514,15 @@ def _lit(v: float) -> str:
 
 def _emit_body(g: Graph, outs: List[Node], out_name: str, indent="    ") -> Tuple[List[str], int]:
     """Emit statements computing `outs` into out_name[i]; returns lines and the number of flops."""
+    lines, names, flops = _emit_values(g, outs, indent)
+    for i, o in enumerate(outs):
+        lines.append(f"{indent}{out_name}[{i}] = {names[o.idx]};")
+    return lines, flops
+
+
+def _emit_values(g: Graph, outs: List[Node], indent="    "):
+    """Emit one const statement per node `outs` depend on (in recording order); returns the lines,
+    the C expression naming each node, and the number of flops."""
     need = set()
     stack = list(outs)
     while stack:
@@ -413,7 +550,23 @@ def _emit_body(g: Graph, outs: List[Node], out_name: str, indent="    ") -> Tupl
         if op == "t":
             names[x.idx] = "t"
             continue
+        if op == "bconst":
+            names[x.idx] = "true" if x.val else "false"
+            continue
         a = [names[y.idx] for y in x.args]
+        if op in _BOOL_OPS:
+            if op == "cmp":
+                expr = f"({a[0]} {x.val} {a[1]})"
+            elif op == "and":
+                expr = f"({a[0]} && {a[1]})"
+            elif op == "or":
+                expr = f"({a[0]} || {a[1]})"
+            else:  # not
+                expr = f"!({a[0]})"
+            v = f"v{x.idx}"
+            lines.append(f"{indent}const bool {v} = {expr};")
+            names[x.idx] = v
+            continue
         if op in ("+", "-", "*", "/"):
             expr = f"{a[0]} {op} {a[1]}"
             flops += 1
@@ -427,14 +580,14 @@ def _emit_body(g: Graph, outs: List[Node], out_name: str, indent="    ") -> Tupl
             expr = f"powf({a[0]}, {a[1]})"
         elif op == "select":
             expr = f"({a[0]} {x.val} {a[1]}) ? {a[2]} : {a[3]}"
+        elif op == "bselect":
+            expr = f"{a[0]} ? {a[1]} : {a[2]}"
         else:  # pragma: no cover
             raise TraceError(f"unknown op {op}")
         v = f"v{x.idx}"
         lines.append(f"{indent}const float {v} = {expr};")
         names[x.idx] = v
-    for i, o in enumerate(outs):
-        lines.append(f"{indent}{out_name}[{i}] = {names[o.idx]};")
-    return lines, flops
+    return lines, names, flops
 
 
 def count_flops(tr: Trace) -> int:

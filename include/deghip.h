@@ -54,6 +54,12 @@ extern "C" {
 /* batched dense output (adaptive GPUTsit5, n <= 4): a step that crosses save points is stashed in
  * LDS and a wave interpolates for many stashed lanes at once; results are unchanged */
 #define DEG_F_BATCH_SAVES (1u << 13)
+/* stopping times: the step that reaches (within 100 eps) the next of deg_solve_args.tstops ends
+ * exactly there (the `tstops` keyword, gpu_tsit5_perform_step.jl:18-27 and :158-164) */
+#define DEG_F_TSTOPS (1u << 14)
+/* the traced source defines callbacks (DEG_NDCB / DEG_NCCB and their functions, see
+ * deg_kernel_compile), applied after every accepted step (integrator_utils.jl:274-325) */
+#define DEG_F_CALLBACKS (1u << 15)
 /* register cap: compile for at least w (1..7) resident waves per SIMD (bits 9..11; 0: no cap) */
 #define DEG_F_WAVES_SHIFT 9
 #define DEG_F_WAVES(w) ((uint32_t)(w) << DEG_F_WAVES_SHIFT)
@@ -63,6 +69,9 @@ extern "C" {
 #define DEG_RET_DTMIN 1     /* the reference raises error("dt<dtmin") (gpu_tsit5_perform_step.jl:102) */
 #define DEG_RET_MAXITERS 2  /* attempt cap reached (no reference counterpart: the reference loops on) */
 #define DEG_RET_NONFINITE 3 /* final t or u not finite */
+#define DEG_RET_TERMINATED 4 /* terminate! in a callback's affect (ReturnCode.Terminated) */
+#define DEG_RET_FAILURE 5   /* an event time before the step start: the reference raises
+                               error("Current interpolant only works between tprev and t") */
 
 typedef struct deg_ctx deg_ctx;
 typedef struct deg_kernel deg_kernel;
@@ -106,6 +115,10 @@ typedef struct deg_solve_args {
                                 correction in between); 0 or 1: every attempt */
     const int32_t *order;    /* optional [n_traj] permutation: the k-th trajectory started is id_offset + order[k]
                                 (e.g. deg_order_by_span, longest first); NULL: ascending ids */
+    const float *tstops;     /* DEG_F_TSTOPS: [ntstops] ascending stopping times shared by all trajectories
+                                (device); the `tstops` keyword of vectorized_solve / vectorized_asolve */
+    int32_t ntstops;
+    int32_t reserved0;
 } deg_solve_args;
 
 /* Context: one device (launches go to the stream given per call). */
@@ -121,7 +134,20 @@ int deg_set_cache_dir(const char *dir);
  * rhs_src defines, in HIP C:
  *   __device__ void deg_rhs(const float* u, const float* p, float t, float* du);
  * and for DEG_RB23 also deg_jac (row-major J[r*n + c] = d du_r / d u_c) and deg_tgrad.
+ * With DEG_F_CALLBACKS it also defines the callbacks (GPUDiscreteCallback / GPUContinuousCallback,
+ * src/ensemblegpukernel/callbacks.jl:1-124), j counting from 0 in CallbackSet order:
+ *   #define DEG_NDCB <discrete callbacks>   #define DEG_NCCB <continuous callbacks>
+ *   bool  deg_dcb_cond(int j, const float* u, const float* p, float t);
+ *   int   deg_dcb_affect(int j, float* u, float* p, float t);
+ *   float deg_ccb_cond(int j, const float* u, const float* p, float t);
+ *   int   deg_ccb_affect(int j, int neg, float* u, float* p, float t);   neg: affect_neg!
+ *   int   deg_ccb_mode(int j);     bit 0 affect! present, bit 1 affect_neg! present,
+ *                                  bits 2-3 rootfind (0 NoRootFind, 1 LeftRootFind, 2 RightRootFind)
+ *   float deg_ccb_abstol(int j), deg_ccb_nudge(int j) (repeat_nudge), deg_ccb_dtrelax(int j);
+ * an affect updates u (and p) in place and returns DEG_AFF_* bits.
  * Replaces the compile of ode_solve_kernel / ode_asolve_kernel for one problem type. */
+#define DEG_AFF_TERMINATE 1u /* terminate!(integrator) */
+#define DEG_AFF_KEEP_FSAL 2u /* u_modified!(integrator, false) */
 int deg_kernel_compile(deg_ctx *ctx, const char *rhs_src, int n, int np, int method, uint32_t flags,
                        deg_kernel **out);
 int deg_kernel_destroy(deg_kernel *k);

@@ -15,13 +15,22 @@
  *   dense output                            integrators/nonstiff/interpolants.jl:28-190 (Vern7), :372-387
  *                                           (Tsit5), integrators/stiff/interpolants.jl (Rosenbrock23)
  *   error norm                              DiffEqBase.ODE_DEFAULT_NORM = sqrt(sum(abs2, x) / n)
+ *   tstops                                  perform_step/gpu_tsit5_perform_step.jl:18-27, :158-164
+ *   callbacks: handle_callbacks!, find_callback_time, ITP root finder, apply_callback!,
+ *   change_t_via_interpolation!, apply_discrete_callback!, terminate!
+ *                                           integrators/integrator_utils.jl:52-479
+ *   earliest continuous event               DiffEqBase.find_first_continuous_callback (external
+ *                                           dependency DiffEqBase 7.2: the earliest event time
+ *                                           wins, the first callback on ties)
  * with these stated deviations, shared with the GPU solver:
  *   - a lane that would hit the reference's in-kernel error("dt<dtmin") stops with a
  *     per-trajectory return code instead of aborting the launch;
  *   - a per-trajectory attempt cap (max_attempts) ends a lane that would loop forever;
- *   - an accepted step that cannot advance t (t + dt rounds to t) snaps t to tf when tf is
- *     within two ulps of t and otherwise ends the lane with DtLessThanMin (the reference
- *     repeats such a step forever);
+ *   - an accepted adaptive step that cannot advance t (t + dt rounds to t) and whose next dt
+ *     is no larger snaps t to tf when tf is within two ulps of t and otherwise ends the lane
+ *     with DtLessThanMin (the reference repeats such a step forever); when the next dt is
+ *     larger the lane goes on with t unchanged, as the reference does. A fixed step that
+ *     cannot advance t ends the lane with DtLessThanMin;
  *   - Rosenbrock23 forms W = I - gamma J and solves it in FP64 (LU with partial
  *     pivoting) instead of Float32 Cramer's rule (linalg/linsolve.jl:31-57);
  *   - optional compensated (Kahan) accumulation of t (flag DEG_O_KAHAN).
@@ -31,6 +40,7 @@
  * An FP64 Vern9 integrator (tableaus/verner_tableaus.jl:1198-1332, same PI controller
  * in double) provides the high-accuracy reference solutions.
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -48,7 +58,8 @@
 enum { OM_TSIT5 = 0, OM_VERN7 = 1, OM_RB23 = 2 };
 enum { DEG_O_KAHAN = 1, DEG_O_DETPOW = 2 };
 /* per-trajectory return codes (same values as the C ABI of the HIP library) */
-enum { ORET_SUCCESS = 0, ORET_DTMIN = 1, ORET_MAXITERS = 2, ORET_NONFINITE = 3 };
+enum { ORET_SUCCESS = 0, ORET_DTMIN = 1, ORET_MAXITERS = 2, ORET_NONFINITE = 3, ORET_TERMINATED = 4,
+       ORET_FAILURE = 5 };
 
 /* ------------------------------------------------------------------------------------ */
 /* Float32 tableau in the shape every explicit method here uses                         */
@@ -386,6 +397,27 @@ static void w_solve(const olu *F, int n, const float *b, float *x) {
 /* ------------------------------------------------------------------------------------ */
 /* Per-trajectory solver                                                                 */
 /* ------------------------------------------------------------------------------------ */
+/* Callback descriptors: the conditions and affects the tests use, restated in C. They play the
+ * role of the user functions of a GPUDiscreteCallback / GPUContinuousCallback
+ * (src/ensemblegpukernel/callbacks.jl:1-124). */
+enum { OCB_DISCRETE = 0, OCB_CONTINUOUS = 1 };
+enum { OCOND_T_EQ = 0,      /* t == cval                 (discrete) */
+       OCOND_U0 = 1,        /* u[0]                      (continuous) */
+       OCOND_U0_MINUS = 2,  /* u[0] - cval               (continuous) */
+       OCOND_U0_LT = 3 };   /* u[0] < cval               (discrete) */
+enum { OAFF_ADD = 1,        /* u[c] = u[c] + aval[c]            (integrator.u += SVector(aval)) */
+       OAFF_SCALE = 2,      /* u[c] = u[c] + aval[c] * u[c]     (integrator.u += aval .* integrator.u) */
+       OAFF_TERMINATE = 4,  /* terminate!(integrator) */
+       OAFF_KEEP_FSAL = 8 };/* u_modified!(integrator, false) */
+typedef struct {
+    int32_t kind, cond;
+    int32_t affect, affect_neg; /* OAFF_* bits, -1: nothing (continuous affect_neg! = nothing) */
+    int32_t rootfind;           /* 0 NoRootFind, 1 LeftRootFind, 2 RightRootFind */
+    float cval;
+    float aval[4];
+    float abstol, nudge, dtrelax;
+} ocb;
+
 typedef struct {
     const osys *S;
     int method;
@@ -397,7 +429,30 @@ typedef struct {
     int kahan, detpow;
     int64_t max_attempts;
     float *ts, *us;
+    const float *tstops;
+    int ntstops;
+    const ocb *cbs;
+    int ncb;
 } ojob;
+
+/* One trajectory's integrator state (the fields of GPUTsit5Integrator & co. that the step,
+ * the saves and the callbacks use; integrators/nonstiff/types.jl:3-38). */
+typedef struct {
+    int n;
+    float p[OMAXN];
+    float u[OMAXN], uprev[OMAXN];
+    float k[OMAXS + 1][OMAXN];
+    float rk1[OMAXN], rk2[OMAXN]; /* Rosenbrock23 interpolation slopes */
+    float t, tprev, tf, tcomp;
+    float dtstep;                 /* integrator.dt: fixed dt, or the last accepted step */
+    float dtnew;                  /* adaptive: the next step's dt */
+    int cur;                      /* 0-based saveat cursor */
+    int step_idx;                 /* 0-based row of the next save_everystep write */
+    int tsi;                      /* 0-based tstops_idx */
+    int evlast;                   /* event_last_time (1-based callback index, 0 none) */
+    int umod;                     /* u_modified */
+    int retcode;
+} ost;
 
 static inline void put_u(const ojob *J, int row, int64_t i, const float *u) {
     int n = J->S->n;
@@ -424,9 +479,10 @@ static inline float next_up(float x) {
     return r;
 }
 
-/* An accepted step that leaves t unchanged (t + dt rounds to t) would repeat forever in the
- * reference's loop. Here the lane snaps to tf when tf is within two units of t's resolution,
- * and otherwise stops with DtLessThanMin. Returns 1 when the lane must stop. */
+/* An accepted step that leaves t unchanged (t + dt rounds to t) and proposes no larger dt
+ * would repeat forever in the reference's loop. Here the lane snaps to tf when tf is within
+ * two units of t's resolution, and otherwise stops with DtLessThanMin. Returns 1 when the lane
+ * must stop. */
 static inline int no_progress(float *t, float tf) {
     float ulp = next_up(*t) - *t;
     if (tf - *t <= 2.0f * ulp) {
@@ -436,115 +492,328 @@ static inline int no_progress(float *t, float tf) {
     return 1;
 }
 
+/* integrator(x): dense output of the last step at time x, Θ = (x - tprev) / dt
+ * (nonstiff/types.jl:41-44, stiff/types.jl:921-939) */
+static void interp_at(const ojob *J, const ost *s, float x, float *out) {
+    const float th = (x - s->tprev) / s->dtstep;
+    const float d = 1.0f / (2.0f + sqrtf(2.0f));
+    if (J->method == OM_TSIT5) tsit5_interp(th, s->dtstep, s->n, s->uprev, (float(*)[OMAXN])s->k, out);
+    else if (J->method == OM_RB23) rb23_interp(th, s->dtstep, s->n, d, s->uprev, s->rk1, s->rk2, out);
+    else vern7_interp(J->S, s->p, th, s->dtstep, s->tprev, s->n, s->uprev, (float(*)[OMAXN])s->k, out);
+}
+
+/* savevalues! (integrator_utils.jl:13-50) */
+static void savevalues(const ojob *J, int64_t i, ost *s) {
+    if (J->saveat) {
+        while (s->cur < J->nsaveat && J->saveat[s->cur] <= s->t) {
+            const float st = J->saveat[s->cur];
+            float out[OMAXN];
+            interp_at(J, s, st, out);
+            if (s->cur < J->nsave) {
+                put_u(J, s->cur, i, out);
+                J->ts[(int64_t)s->cur * J->N + i] = st;
+            }
+            ++s->cur;
+        }
+    } else if (J->save_everystep && !J->adaptive) {
+        if (s->step_idx >= 0 && s->step_idx < J->nsave) {
+            put_u(J, s->step_idx, i, s->u);
+            J->ts[(int64_t)s->step_idx * J->N + i] = s->t;
+        }
+        ++s->step_idx;
+    }
+}
+
+/* ---- callbacks (integrator_utils.jl:52-479) ---- */
+static inline float jl_sign(float x) { return (x < 0.0f) ? -1.0f : ((x > 0.0f) ? 1.0f : x); }
+
+/* Julia eps(x::Float32) */
+static float jl_eps(float x) {
+    float ax = fabsf(x);
+    if (!isfinite(ax)) return NAN;
+    if (ax < FLT_MIN) return 1.4e-45f;
+    int e;
+    frexpf(ax, &e); /* ax = m 2^e, m in [0.5, 1): exponent(x) = e - 1 */
+    return ldexpf(1.0f, (e - 1) - 23);
+}
+
+/* Julia nextfloat(x::Float32) */
+static float jl_nextfloat(float x) {
+    if (isnan(x) || x == INFINITY) return x;
+    return next_up(x);
+}
+
+static float ocb_ccond(const ocb *c, const float *u, float t) {
+    (void)t;
+    switch (c->cond) {
+        case OCOND_U0: return u[0];
+        case OCOND_U0_MINUS: return u[0] - c->cval;
+        default: return NAN;
+    }
+}
+
+static int ocb_dcond(const ocb *c, const float *u, float t) {
+    switch (c->cond) {
+        case OCOND_T_EQ: return t == c->cval;
+        case OCOND_U0_LT: return u[0] < c->cval;
+        default: return 0;
+    }
+}
+
+/* applies affect bits to u; returns the bits */
+static int ocb_apply(const ocb *c, int bits, float *u, int n) {
+    if (bits & OAFF_ADD)
+        for (int q = 0; q < n && q < 4; ++q) u[q] = u[q] + c->aval[q];
+    if (bits & OAFF_SCALE)
+        for (int q = 0; q < n && q < 4; ++q) u[q] = u[q] + c->aval[q] * u[q];
+    return bits;
+}
+
+static void apply_affect_flags(ost *s, int bits) {
+    if (bits & OAFF_TERMINATE) s->retcode = ORET_TERMINATED; /* terminate! (integrator_utils.jl:52-67) */
+    if (bits & OAFF_KEEP_FSAL) s->umod = 0;                  /* u_modified!(integrator, false) */
+}
+
+/* DiffEqBase.get_condition (integrator_utils.jl:460-478) */
+static float cond_at(const ojob *J, const ost *s, const ocb *c, float x) {
+    if (x == s->t) return ocb_ccond(c, s->u, x);
+    if (x == s->tprev) return ocb_ccond(c, s->uprev, x);
+    float tmp[OMAXN];
+    interp_at(J, s, x, tmp);
+    return ocb_ccond(c, tmp, x);
+}
+
+/* gpu_find_root (integrator_utils.jl:327-384): ITP with scaled_k1 = 0.2, k2 = 2, n0 = 10 */
+static float find_root(const ojob *J, const ost *s, const ocb *c, float left, float right, int rootfind) {
+    float fl = cond_at(J, s, c, left);
+    float fr = cond_at(J, s, c, right);
+    const float span0 = right - left;
+    const float k1 = 0.2f / span0;
+    float eps_s = span0 * 512.0f;
+    for (int it = 0; it < 100; ++it) {
+        const float span = right - left;
+        const float mid = (left + right) / 2.0f;
+        const float r = eps_s - span / 2.0f;
+        const float xf = left + (span * fl) / (fl - fr);
+        const float delta = jl_max((k1 * span) * span, jl_eps(xf));
+        const float diff = mid - xf;
+        const float xt = (delta <= fabsf(diff)) ? xf + copysignf(delta, diff) : mid;
+        const float xp = (fabsf(xt - mid) <= r) ? xt : mid - copysignf(r, diff);
+        const float yp = cond_at(J, s, c, xp);
+        const float yps = yp * jl_sign(fr);
+        if (yps > 0.0f) {
+            right = xp;
+            fr = yp;
+        } else if (yps < 0.0f) {
+            left = xp;
+            fl = yp;
+        } else {
+            left = xp;
+            right = xp;
+            break;
+        }
+        eps_s = eps_s / 2.0f;
+        if (jl_nextfloat(left) >= right) break;
+    }
+    return (rootfind == 1) ? left : right;
+}
+
+/* handle_callbacks! (integrator_utils.jl:274-325). Returns saved_in_cb. */
+static int handle_callbacks(const ojob *J, int64_t i, ost *s) {
+    int csaved = 0, ndisc = 0, ncont = 0;
+    for (int q = 0; q < J->ncb; ++q) {
+        if (J->cbs[q].kind == OCB_DISCRETE) ++ndisc;
+        else ++ncont;
+    }
+    if (ncont > 0) {
+        /* DiffEqBase.find_first_continuous_callback over find_callback_time (:386-446) */
+        float tmin = s->t, upc = 0.0f;
+        int occ = 0, idx = 1, j = 0;
+        const ocb *first = NULL;
+        for (int q = 0; q < J->ncb; ++q) {
+            const ocb *c = &J->cbs[q];
+            if (c->kind != OCB_CONTINUOUS) continue;
+            if (!first) first = c;
+            float bt = s->tprev;
+            float bc = ocb_ccond(c, s->uprev, s->tprev);
+            if (s->evlast == j + 1 && fabsf(bc - 0.0f) <= c->abstol) {
+                bt = s->tprev + s->dtstep * c->nudge;
+                bc = cond_at(J, s, c, bt);
+            }
+            const float bs = jl_sign(bc);
+            const float tsg = jl_sign(cond_at(J, s, c, s->t));
+            const int oc = ((bs < 0.0f && c->affect >= 0) || (bs > 0.0f && c->affect_neg >= 0)) && bs * tsg <= 0.0f;
+            float tj = s->t;
+            if (oc && c->rootfind != 0 && tsg != 0.0f) tj = find_root(J, s, c, bt, s->t, c->rootfind);
+            if (j == 0) {
+                tmin = tj;
+                upc = bs;
+                occ = oc;
+                idx = 1;
+            } else if (oc && (tj < tmin || !occ)) {
+                tmin = tj;
+                upc = bs;
+                occ = 1;
+                idx = j + 1;
+            }
+            ++j;
+        }
+        if (occ) {
+            s->evlast = idx;
+            /* apply_callback! (:232-272) with the first continuous callback's affects */
+            if (tmin < s->tprev) {
+                s->retcode = ORET_FAILURE; /* error("Current interpolant only works between tprev and t") */
+            } else {
+                if (tmin != s->t) {
+                    /* _change_t_via_interpolation! (:174-201) */
+                    float tmp[OMAXN];
+                    interp_at(J, s, tmin, tmp);
+                    memcpy(s->u, tmp, sizeof(float) * s->n);
+                    s->step_idx -= (int)rintf((s->t - tmin) / s->dtstep);
+                    s->t = tmin;
+                    s->tcomp = 0.0f;
+                }
+                if (J->adaptive && s->dtnew < 1.0e-12f) s->dtnew = jl_min(first->dtrelax * s->dtstep, fabsf(s->tf - s->t));
+                savevalues(J, i, s);
+                csaved = 1;
+                s->umod = 1;
+                if (upc < 0.0f) {
+                    if (first->affect >= 0) apply_affect_flags(s, ocb_apply(first, first->affect, s->u, s->n));
+                    else s->umod = 0;
+                } else if (upc > 0.0f) {
+                    if (first->affect_neg >= 0) apply_affect_flags(s, ocb_apply(first, first->affect_neg, s->u, s->n));
+                    else s->umod = 0;
+                }
+            }
+        } else {
+            s->evlast = 0;
+        }
+    }
+    if (ndisc > 0) {
+        /* apply_discrete_callback! (:70-149), in order; its result replaces the continuous one */
+        int dsaved = 0;
+        for (int q = 0; q < J->ncb; ++q) {
+            const ocb *c = &J->cbs[q];
+            if (c->kind != OCB_DISCRETE) continue;
+            if (ocb_dcond(c, s->u, s->t)) {
+                savevalues(J, i, s);
+                dsaved = 1;
+                s->umod = 1;
+                apply_affect_flags(s, ocb_apply(c, c->affect, s->u, s->n));
+            }
+        }
+        return dsaved;
+    }
+    return csaved;
+}
+
+#define O_TSTOP_SLACK (100.0f * 1.1920928955078125e-7f) /* 100 eps(Float32) */
+
 static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p, float t0, float tf,
                       int32_t *nacc_out, int32_t *nrej_out) {
     const osys *S = J->S;
     const int n = S->n;
     const erk32 *T = (J->method == OM_TSIT5) ? &TAB_T5 : &TAB_V7;
-    float u[OMAXN], uprev[OMAXN], unew[OMAXN];
-    float k[OMAXS + 1][OMAXN];
-    float rk1[OMAXN], rk2[OMAXN]; /* Rosenbrock23 interpolation slopes */
-    memcpy(u, u0, sizeof(float) * n);
-    memcpy(uprev, u0, sizeof(float) * n);
-    int retcode = ORET_SUCCESS;
+    ost st;
+    ost *s = &st;
+    memset(s, 0, sizeof(st));
+    s->n = n;
+    memcpy(s->p, p, sizeof(float) * S->np);
+    float unew[OMAXN];
+    float (*k)[OMAXN] = s->k;
+    memcpy(s->u, u0, sizeof(float) * n);
+    memcpy(s->uprev, u0, sizeof(float) * n);
+    s->retcode = ORET_SUCCESS;
+    s->tf = tf;
+    s->evlast = 1; /* nonstiff/types.jl:318 */
+    s->umod = 1;   /* u_modified = true at init: the first step evaluates k1 */
     int32_t nacc = 0, nrej = 0;
     int64_t attempts = 0;
     const int use_saveat = (J->saveat != NULL);
-    int cur = 0; /* 0-based saveat cursor */
-    int step_idx = 1; /* 0-based row of the next save_everystep write */
+    s->step_idx = 1;
 
     /* kernels.jl:35-45 / :116-126: initial save */
     if (use_saveat) {
         if (J->nsaveat > 0 && t0 == J->saveat[0]) {
             put_u(J, 0, i, u0);
-            cur = 1;
+            s->cur = 1;
         }
     } else {
         J->ts[i] = t0;
         put_u(J, 0, i, u0);
     }
 
-    float t = t0, tprev = t0, tcomp = 0.0f;
+    s->t = t0;
+    s->tprev = t0;
+    s->tcomp = 0.0f;
     const float d = 1.0f / (2.0f + sqrtf(2.0f));
-    float dtstep = J->dt; /* fixed dt, or the accepted dt of the last adaptive step */
+    s->dtstep = J->dt; /* fixed dt, or the accepted dt of the last adaptive step */
 
     if (!J->adaptive) {
         /* ---------------- fixed step: kernels.jl:49-52 -------------------------------- */
         const float dt = J->dt;
-        int first = 1;
-        while (t < tf) {
-            if (attempts >= J->max_attempts) { retcode = ORET_MAXITERS; break; }
+        while (s->t < tf && s->retcode != ORET_TERMINATED) {
+            if (attempts >= J->max_attempts) { s->retcode = ORET_MAXITERS; break; }
             ++attempts;
-            memcpy(uprev, u, sizeof(float) * n);
-            tprev = t;
-            float told = t;
-            t = advance_t(t, dt, &tcomp, J->kahan);
-            if (t == told && !J->kahan) {
-                retcode = ORET_DTMIN; /* the fixed step is below the resolution of t */
-                --attempts;
-                break;
+            memcpy(s->uprev, s->u, sizeof(float) * n);
+            s->tprev = s->t;
+            const float told = s->t;
+            float dts = dt;
+            /* gpu_tsit5_perform_step.jl:18-27: a step reaching the next stopping time ends there;
+             * integrator.dt (the dense output's) stays dt */
+            if (s->tsi < J->ntstops && ((J->tstops[s->tsi] - s->t) - dt) - O_TSTOP_SLACK < 0.0f) {
+                s->t = J->tstops[s->tsi];
+                dts = s->t - told;
+                ++s->tsi;
+                s->tcomp = 0.0f;
+            } else {
+                s->t = advance_t(s->t, dt, &s->tcomp, J->kahan);
+                if (s->t == told && !J->kahan) {
+                    s->retcode = ORET_DTMIN; /* the fixed step is below the resolution of t */
+                    --attempts;
+                    break;
+                }
             }
             if (J->method == OM_RB23) {
                 /* gpu_rosenbrock23_perform_step.jl:1-70 */
                 float Jm[OMAXN * OMAXN], dT[OMAXN], F0[OMAXN], rhs[OMAXN], tmp[OMAXN], F1[OMAXN], x[OMAXN];
-                float gam = dt * d;
-                float dto2 = dt / 2.0f;
-                S->jac(uprev, p, told, Jm);
-                S->tgrad(uprev, p, told, dT);
+                float gam = dts * d;
+                float dto2 = dts / 2.0f;
+                S->jac(s->uprev, s->p, told, Jm);
+                S->tgrad(s->uprev, s->p, told, dT);
                 olu F;
                 w_factor(&F, n, gam, Jm);
-                S->f(uprev, p, told, F0);
+                S->f(s->uprev, s->p, told, F0);
                 for (int c = 0; c < n; ++c) rhs[c] = F0[c] + gam * dT[c];
-                w_solve(&F, n, rhs, rk1);
-                for (int c = 0; c < n; ++c) tmp[c] = uprev[c] + dto2 * rk1[c];
-                S->f(tmp, p, told + dto2, F1);
-                for (int c = 0; c < n; ++c) rhs[c] = F1[c] - rk1[c];
+                w_solve(&F, n, rhs, s->rk1);
+                for (int c = 0; c < n; ++c) tmp[c] = s->uprev[c] + dto2 * s->rk1[c];
+                S->f(tmp, s->p, told + dto2, F1);
+                for (int c = 0; c < n; ++c) rhs[c] = F1[c] - s->rk1[c];
                 w_solve(&F, n, rhs, x);
-                for (int c = 0; c < n; ++c) rk2[c] = x[c] + rk1[c];
-                for (int c = 0; c < n; ++c) u[c] = uprev[c] + dt * rk2[c];
+                for (int c = 0; c < n; ++c) s->rk2[c] = x[c] + s->rk1[c];
+                for (int c = 0; c < n; ++c) s->u[c] = s->uprev[c] + dts * s->rk2[c];
             } else {
                 /* gpu_tsit5_perform_step.jl:1-64, gpu_vern7_perform_step.jl:1-74 */
-                if (T->fsal && !first) {
+                if (T->fsal && !s->umod) {
                     memcpy(k[1], k[T->s], sizeof(float) * n);
                 } else {
-                    S->f(uprev, p, told, k[1]);
+                    S->f(s->uprev, s->p, told, k[1]);
                 }
-                erk_attempt(T, S, p, uprev, told, dt, k, unew);
-                memcpy(u, unew, sizeof(float) * n);
+                erk_attempt(T, S, s->p, s->uprev, told, dts, k, unew);
+                memcpy(s->u, unew, sizeof(float) * n);
             }
-            first = 0;
+            s->umod = 0;
             ++nacc;
-            /* savevalues!: integrator_utils.jl:13-50 */
-            if (use_saveat) {
-                while (cur < J->nsaveat && J->saveat[cur] <= t) {
-                    float st = J->saveat[cur];
-                    float th = (st - tprev) / dt;
-                    float out[OMAXN];
-                    if (J->method == OM_TSIT5) tsit5_interp(th, dt, n, uprev, k, out);
-                    else if (J->method == OM_RB23) rb23_interp(th, dt, n, d, uprev, rk1, rk2, out);
-                    else vern7_interp(S, p, th, dt, tprev, n, uprev, k, out);
-                    if (cur < J->nsave) {
-                        put_u(J, cur, i, out);
-                        J->ts[(int64_t)cur * J->N + i] = st;
-                    }
-                    ++cur;
-                }
-            } else if (J->save_everystep) {
-                if (step_idx < J->nsave) {
-                    put_u(J, step_idx, i, u);
-                    J->ts[(int64_t)step_idx * J->N + i] = t;
-                }
-                ++step_idx;
-            }
+            int saved_in_cb = (J->ncb > 0) ? handle_callbacks(J, i, s) : 0;
+            if (s->retcode == ORET_FAILURE) break;
+            if (!saved_in_cb) savevalues(J, i, s);
         }
         /* kernels.jl:53-62 */
-        if (t > tf && !use_saveat && J->save_everystep && retcode == ORET_SUCCESS) {
+        if (s->t > tf && !use_saveat && J->save_everystep &&
+            (s->retcode == ORET_SUCCESS || s->retcode == ORET_TERMINATED)) {
             float out[OMAXN];
-            float th = (tf - tprev) / dt;
             int row = J->nsave - 1;
-            if (J->method == OM_TSIT5) tsit5_interp(th, dt, n, uprev, k, out);
-            else if (J->method == OM_RB23) rb23_interp(th, dt, n, d, uprev, rk1, rk2, out);
-            else vern7_interp(S, p, th, dt, tprev, n, uprev, k, out);
+            interp_at(J, s, tf, out);
             put_u(J, row, i, out);
             J->ts[(int64_t)row * J->N + i] = tf;
         }
@@ -558,20 +827,21 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
         const float qoldinit = (float)1.0e-4;
         float qold = qoldinit;
         float dt = J->dt; /* integ.dtnew */
-        int have_k7 = 0;
         const float e32 = 6.0f + sqrtf(2.0f);
-        while (t < tf) {
-            memcpy(uprev, u, sizeof(float) * n);
+        while (s->t < tf && s->retcode != ORET_TERMINATED) {
+            memcpy(s->uprev, s->u, sizeof(float) * n);
+            const float t = s->t;
             float EEst = INFINITY;
             /* FSAL: gpu_tsit5_perform_step.jl:92-97 */
             if (J->method == OM_TSIT5) {
-                if (have_k7) memcpy(k[1], k[7], sizeof(float) * n);
-                else S->f(uprev, p, t, k[1]);
+                if (!s->umod) memcpy(k[1], k[7], sizeof(float) * n);
+                else S->f(s->uprev, s->p, t, k[1]);
             }
+            s->umod = 0;
             int stop = 0;
             while (EEst > 1.0f) {
-                if (dt < 1.0e-14f) { retcode = ORET_DTMIN; stop = 1; break; }
-                if (attempts >= J->max_attempts) { retcode = ORET_MAXITERS; stop = 1; break; }
+                if (dt < 1.0e-14f) { s->retcode = ORET_DTMIN; stop = 1; break; }
+                if (attempts >= J->max_attempts) { s->retcode = ORET_MAXITERS; stop = 1; break; }
                 ++attempts;
                 float q11 = 0.0f, q;
                 if (J->method == OM_RB23) {
@@ -580,35 +850,35 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                     float gam = dt * d;
                     float dto2 = dt / 2.0f;
                     float dto6 = dt / 6.0f;
-                    S->jac(uprev, p, t, Jm);
-                    S->tgrad(uprev, p, t, dT);
+                    S->jac(s->uprev, s->p, t, Jm);
+                    S->tgrad(s->uprev, s->p, t, dT);
                     olu F;
                     w_factor(&F, n, gam, Jm);
-                    S->f(uprev, p, t, F0);
+                    S->f(s->uprev, s->p, t, F0);
                     for (int c = 0; c < n; ++c) rhs[c] = F0[c] + gam * dT[c];
-                    w_solve(&F, n, rhs, rk1);
-                    for (int c = 0; c < n; ++c) tmp[c] = uprev[c] + dto2 * rk1[c];
-                    S->f(tmp, p, t + dto2, F1);
-                    for (int c = 0; c < n; ++c) rhs[c] = F1[c] - rk1[c];
+                    w_solve(&F, n, rhs, s->rk1);
+                    for (int c = 0; c < n; ++c) tmp[c] = s->uprev[c] + dto2 * s->rk1[c];
+                    S->f(tmp, s->p, t + dto2, F1);
+                    for (int c = 0; c < n; ++c) rhs[c] = F1[c] - s->rk1[c];
                     w_solve(&F, n, rhs, x);
-                    for (int c = 0; c < n; ++c) rk2[c] = x[c] + rk1[c];
-                    for (int c = 0; c < n; ++c) unew[c] = uprev[c] + dt * rk2[c];
-                    S->f(unew, p, t + dt, F2);
+                    for (int c = 0; c < n; ++c) s->rk2[c] = x[c] + s->rk1[c];
+                    for (int c = 0; c < n; ++c) unew[c] = s->uprev[c] + dt * s->rk2[c];
+                    S->f(unew, s->p, t + dt, F2);
                     for (int c = 0; c < n; ++c)
-                        rhs[c] = ((F2[c] - e32 * (rk2[c] - F1[c])) - 2.0f * (rk1[c] - F0[c])) + dt * dT[c];
+                        rhs[c] = ((F2[c] - e32 * (s->rk2[c] - F1[c])) - 2.0f * (s->rk1[c] - F0[c])) + dt * dT[c];
                     float k3[OMAXN];
                     w_solve(&F, n, rhs, k3);
                     float e[OMAXN] = {0};
-                    for (int c = 0; c < n; ++c) e[c] = dto6 * ((rk1[c] - 2.0f * rk2[c]) + k3[c]);
+                    for (int c = 0; c < n; ++c) e[c] = dto6 * ((s->rk1[c] - 2.0f * s->rk2[c]) + k3[c]);
                     for (int c = 0; c < n; ++c) {
-                        float sc = J->abstol + jl_max(fabsf(uprev[c]), fabsf(unew[c])) * J->reltol;
+                        float sc = J->abstol + jl_max(fabsf(s->uprev[c]), fabsf(unew[c])) * J->reltol;
                         e[c] = e[c] / sc;
                     }
                     EEst = rms_norm(e, n);
                 } else {
-                    if (J->method == OM_VERN7) S->f(uprev, p, t, k[1]); /* gpu_vern7_perform_step.jl:117 */
-                    erk_attempt(T, S, p, uprev, t, dt, k, unew);
-                    EEst = erk_eest(T, n, dt, k, uprev, unew, J->abstol, J->reltol);
+                    if (J->method == OM_VERN7) S->f(s->uprev, s->p, t, k[1]); /* gpu_vern7_perform_step.jl:117 */
+                    erk_attempt(T, S, s->p, s->uprev, t, dt, k, unew);
+                    EEst = erk_eest(T, n, dt, k, s->uprev, unew, J->abstol, J->reltol);
                 }
                 /* PI controller: gpu_tsit5_perform_step.jl:124-137 */
                 if (EEst == 0.0f) {
@@ -625,76 +895,75 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                     qold = jl_max(EEst, qoldinit);
                     float dtnew = dt / q;
                     dtnew = jl_min(fabsf(dtnew), fabsf((tf - t) - dt));
-                    dtstep = dt;
-                    tprev = t;
-                    memcpy(u, unew, sizeof(float) * n);
+                    s->dtstep = dt;
+                    s->tprev = t;
+                    memcpy(s->u, unew, sizeof(float) * n);
                     if ((tf - t) - dt < 1.0e-14f) {
-                        t = tf;
-                        tcomp = 0.0f;
+                        s->t = tf;
+                        s->tcomp = 0.0f;
+                    } else if (s->tsi < J->ntstops && ((J->tstops[s->tsi] - t) - dt) - O_TSTOP_SLACK < 0.0f) {
+                        /* gpu_tsit5_perform_step.jl:158-164: end at the stopping time, state from
+                         * this step's dense output */
+                        s->t = J->tstops[s->tsi];
+                        s->tcomp = 0.0f;
+                        ++s->tsi;
+                        float tmp[OMAXN];
+                        interp_at(J, s, s->t, tmp);
+                        memcpy(s->u, tmp, sizeof(float) * n);
                     } else {
-                        float tn = advance_t(t, dt, &tcomp, J->kahan);
-                        if (tn == t && !J->kahan && no_progress(&tn, tf)) {
-                            retcode = ORET_DTMIN;
+                        float tn = advance_t(t, dt, &s->tcomp, J->kahan);
+                        if (tn == t && !J->kahan && !(dtnew > dt) && no_progress(&tn, tf)) {
+                            s->retcode = ORET_DTMIN;
                             stop = 1;
                         }
-                        t = tn;
+                        s->t = tn;
                     }
                     dt = dtnew;
                     ++nacc;
-                    have_k7 = 1;
                     if (stop) break;
                 }
             }
             if (stop) break;
+            s->dtnew = dt;
+            int saved_in_cb = (J->ncb > 0) ? handle_callbacks(J, i, s) : 0;
+            dt = s->dtnew;
+            if (s->retcode == ORET_FAILURE) break;
             /* savevalues! after the accepted step */
-            if (use_saveat) {
-                while (cur < J->nsaveat && J->saveat[cur] <= t) {
-                    float st = J->saveat[cur];
-                    float th = (st - tprev) / dtstep;
-                    float out[OMAXN];
-                    if (J->method == OM_TSIT5) tsit5_interp(th, dtstep, n, uprev, k, out);
-                    else if (J->method == OM_RB23) rb23_interp(th, dtstep, n, d, uprev, rk1, rk2, out);
-                    else vern7_interp(S, p, th, dtstep, tprev, n, uprev, k, out);
-                    if (cur < J->nsave) {
-                        put_u(J, cur, i, out);
-                        J->ts[(int64_t)cur * J->N + i] = st;
-                    }
-                    ++cur;
-                }
-            }
+            if (!saved_in_cb) savevalues(J, i, s);
         }
     }
     /* kernels.jl:59-62: endpoint-only output keeps the last state */
     if (!use_saveat && !J->save_everystep) {
-        put_u(J, 1, i, u);
-        J->ts[J->N + i] = t;
+        put_u(J, 1, i, s->u);
+        J->ts[J->N + i] = s->t;
     }
-    if (retcode == ORET_SUCCESS) {
-        int fin = isfinite(t);
-        for (int c = 0; c < n; ++c) fin = fin && isfinite(u[c]);
-        if (!fin) retcode = ORET_NONFINITE;
+    if (s->retcode == ORET_SUCCESS) {
+        int fin = isfinite(s->t);
+        for (int c = 0; c < n; ++c) fin = fin && isfinite(s->u[c]);
+        if (!fin) s->retcode = ORET_NONFINITE;
     }
     *nacc_out = nacc;
     *nrej_out = nrej;
-    (void)dtstep;
-    return retcode;
+    return s->retcode;
 }
 
 /* ------------------------------------------------------------------------------------ */
 /* Exported: Float32 restatement over an ensemble (SoA layout, same as the HIP solver)   */
 /* ------------------------------------------------------------------------------------ */
 /* u0[c*N + i], p[j*N + i]; outputs ts[row*N + i], us[(row*n + c)*N + i].
- * t0s/tfs may be NULL (then the shared t0/tf apply). saveat may be NULL.
+ * t0s/tfs may be NULL (then the shared t0/tf apply). saveat may be NULL. tstops (ntstops) may
+ * be NULL; cbs holds ncb callback descriptors (discrete and continuous, each kind in order).
  * Returns 0, or a negative value for an unsupported combination. */
-int deg_oracle_solve(int system, int method, int adaptive, int64_t N,
-                     const float *u0, const float *p, const float *t0s, const float *tfs,
-                     float t0, float tf, float dt, float abstol, float reltol,
-                     const float *saveat, int nsaveat, int nsave, int save_everystep,
-                     int flags, int64_t max_attempts,
-                     float *ts, float *us, int32_t *retcode, int32_t *naccept, int32_t *nreject,
-                     int nthreads) {
+int deg_oracle_solve_cb(int system, int method, int adaptive, int64_t N,
+                        const float *u0, const float *p, const float *t0s, const float *tfs,
+                        float t0, float tf, float dt, float abstol, float reltol,
+                        const float *saveat, int nsaveat, int nsave, int save_everystep,
+                        int flags, int64_t max_attempts,
+                        float *ts, float *us, int32_t *retcode, int32_t *naccept, int32_t *nreject,
+                        int nthreads, const float *tstops, int ntstops, const ocb *cbs, int ncb) {
     if (system < 0 || system >= OSYS_COUNT) return -1;
     if (method < OM_TSIT5 || method > OM_RB23) return -1;
+    if (ntstops < 0 || ncb < 0 || (ntstops > 0 && !tstops) || (ncb > 0 && !cbs)) return -1;
     init_tabs();
     const osys *S = &OSYSTEMS[system];
     ojob J;
@@ -714,6 +983,10 @@ int deg_oracle_solve(int system, int method, int adaptive, int64_t N,
     J.max_attempts = max_attempts > 0 ? max_attempts : (int64_t)1 << 62;
     J.ts = ts;
     J.us = us;
+    J.tstops = tstops;
+    J.ntstops = tstops ? ntstops : 0;
+    J.cbs = cbs;
+    J.ncb = cbs ? ncb : 0;
     int bad = 0;
     if (nthreads <= 0) nthreads = 1;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads) reduction(| : bad)
@@ -731,6 +1004,18 @@ int deg_oracle_solve(int system, int method, int adaptive, int64_t N,
         if (nreject) nreject[i] = nr;
     }
     return bad ? -2 : 0;
+}
+
+int deg_oracle_solve(int system, int method, int adaptive, int64_t N,
+                     const float *u0, const float *p, const float *t0s, const float *tfs,
+                     float t0, float tf, float dt, float abstol, float reltol,
+                     const float *saveat, int nsaveat, int nsave, int save_everystep,
+                     int flags, int64_t max_attempts,
+                     float *ts, float *us, int32_t *retcode, int32_t *naccept, int32_t *nreject,
+                     int nthreads) {
+    return deg_oracle_solve_cb(system, method, adaptive, N, u0, p, t0s, tfs, t0, tf, dt, abstol, reltol, saveat,
+                               nsaveat, nsave, save_everystep, flags, max_attempts, ts, us, retcode, naccept,
+                               nreject, nthreads, NULL, 0, NULL, 0);
 }
 
 /* ------------------------------------------------------------------------------------ */

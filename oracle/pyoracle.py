@@ -14,9 +14,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "libdeg_oracle.so")
 
-SYSTEM_IDS = {"lorenz": 0, "decay": 1, "growth": 2, "quad": 3, "robertson": 4, "vdp": 5, "tdep": 6, "rot": 7}
+SYSTEM_IDS = {"lorenz": 0, "decay": 1, "growth": 2, "quad": 3, "robertson": 4, "vdp": 5, "tdep": 6, "rot": 7,
+              "ball": 8}
 SYSTEM_DIMS = {"lorenz": (3, 3), "decay": (1, 1), "growth": (1, 1), "quad": (1, 1), "robertson": (3, 3),
-               "vdp": (2, 1), "tdep": (1, 1), "rot": (2, 1)}
+               "vdp": (2, 1), "tdep": (1, 1), "rot": (2, 1), "ball": (2, 1)}
 METHOD_IDS = {"tsit5": 0, "vern7": 1, "rb23": 2}
 KAHAN, DETPOW = 1, 2
 
@@ -25,6 +26,51 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 _lp = ctypes.POINTER(ctypes.c_int64)
 _lib = None
+
+# callback descriptors (deg_oracle.c: ocb)
+DISCRETE, CONTINUOUS = 0, 1
+COND_T_EQ, COND_U0, COND_U0_MINUS, COND_U0_LT = 0, 1, 2, 3
+AFF_ADD, AFF_SCALE, AFF_TERMINATE, AFF_KEEP_FSAL = 1, 2, 4, 8
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+class OCB(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("cond", ctypes.c_int32), ("affect", ctypes.c_int32),
+                ("affect_neg", ctypes.c_int32), ("rootfind", ctypes.c_int32), ("cval", ctypes.c_float),
+                ("aval", ctypes.c_float * 4), ("abstol", ctypes.c_float), ("nudge", ctypes.c_float),
+                ("dtrelax", ctypes.c_float)]
+
+
+def discrete(cond, cval=0.0, affect=0, aval=()):
+    """A discrete callback descriptor: condition id (COND_T_EQ / COND_U0_LT) with its constant, and
+    affect bits (AFF_*) with their constants."""
+    return dict(kind=DISCRETE, cond=cond, cval=cval, affect=affect, affect_neg=-1, aval=aval)
+
+
+def continuous(cond, cval=0.0, affect=0, affect_neg="same", aval=(), rootfind=1, abstol=10 * EPS32,
+               nudge=0.01, dtrelax=1.0):
+    """A continuous callback descriptor (GPUContinuousCallback defaults). affect / affect_neg: AFF_*
+    bits or None for `nothing`; affect_neg "same" copies affect."""
+    an = affect if affect_neg == "same" else affect_neg
+    return dict(kind=CONTINUOUS, cond=cond, cval=cval, affect=-1 if affect is None else affect,
+                affect_neg=-1 if an is None else an, aval=aval, rootfind=rootfind, abstol=abstol, nudge=nudge,
+                dtrelax=dtrelax)
+
+
+def _ocb_array(cbs):
+    arr = (OCB * max(1, len(cbs)))()
+    for q, c in enumerate(cbs):
+        o = arr[q]
+        o.kind, o.cond, o.affect, o.affect_neg = c["kind"], c["cond"], c["affect"], c.get("affect_neg", -1)
+        o.rootfind = c.get("rootfind", 0)
+        o.cval = float(np.float32(c.get("cval", 0.0)))
+        av = list(c.get("aval", ())) + [0.0] * 4
+        for j in range(4):
+            o.aval[j] = float(np.float32(av[j]))
+        o.abstol = float(np.float32(c.get("abstol", 0.0)))
+        o.nudge = float(np.float32(c.get("nudge", 0.01)))
+        o.dtrelax = float(np.float32(c.get("dtrelax", 1.0)))
+    return arr
 
 
 def build():
@@ -42,6 +88,9 @@ def lib():
                                        _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                        _fp, _fp, _ip, _ip, _ip, ctypes.c_int]
         L.deg_oracle_solve.restype = ctypes.c_int
+        L.deg_oracle_solve_cb.argtypes = L.deg_oracle_solve.argtypes + [_fp, ctypes.c_int, ctypes.c_void_p,
+                                                                         ctypes.c_int]
+        L.deg_oracle_solve_cb.restype = ctypes.c_int
         L.deg_oracle_vern9.argtypes = [ctypes.c_int, ctypes.c_int64, _fp, _fp, _fp, _fp, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
                                        ctypes.c_int, _dp, _ip, _lp, ctypes.c_int]
@@ -67,10 +116,11 @@ def _f32(a):
 
 def solve(system: str, method: str, adaptive: bool, u0, p, tspan, dt, *, abstol=1e-6, reltol=1e-3, saveat=None,
           nsave=None, save_everystep=False, t0s=None, tfs=None, kahan=False, detpow=False, max_attempts=0,
-          nthreads=None):
+          nthreads=None, tstops=None, callbacks=()):
     """Float32 restatement over an ensemble. u0 (n, N), p (np, N) float32 SoA. Returns dict with
     ts (nsave, N), us (nsave, n, N), retcode, naccept, nreject. Rows the reference leaves
-    uninitialised are NaN here."""
+    uninitialised are NaN here. tstops: stopping times; callbacks: descriptors from discrete() /
+    continuous()."""
     n, np_ = SYSTEM_DIMS[system]
     u0 = _f32(u0)
     p = _f32(p) if np_ > 0 else np.zeros((0, u0.shape[1]), np.float32)
@@ -85,12 +135,16 @@ def solve(system: str, method: str, adaptive: bool, u0, p, tspan, dt, *, abstol=
     nr = np.zeros(N, np.int32)
     flags = (KAHAN if kahan else 0) | (DETPOW if detpow else 0)
     t0s, tfs = _f32(t0s), _f32(tfs)
-    r = lib().deg_oracle_solve(SYSTEM_IDS[system], METHOD_IDS[method], int(adaptive), N, _p(u0, _fp), _p(p, _fp),
-                               _p(t0s, _fp), _p(tfs, _fp), float(np.float32(tspan[0])), float(np.float32(tspan[1])),
-                               float(np.float32(dt)), float(np.float32(abstol)), float(np.float32(reltol)),
-                               _p(sv, _fp), 0 if sv is None else len(sv), int(nsave), int(bool(save_everystep)), flags,
-                               int(max_attempts), _p(ts, _fp), _p(us, _fp), _p(rc, _ip), _p(na, _ip), _p(nr, _ip),
-                               int(nthreads or os.cpu_count() or 1))
+    tst = None if tstops is None else np.ascontiguousarray(np.asarray(tstops, np.float64).astype(np.float32).reshape(-1))
+    cbs = list(callbacks or ())
+    arr = _ocb_array(cbs)
+    r = lib().deg_oracle_solve_cb(SYSTEM_IDS[system], METHOD_IDS[method], int(adaptive), N, _p(u0, _fp),
+                                  _p(p, _fp), _p(t0s, _fp), _p(tfs, _fp), float(np.float32(tspan[0])),
+                                  float(np.float32(tspan[1])), float(np.float32(dt)), float(np.float32(abstol)),
+                                  float(np.float32(reltol)), _p(sv, _fp), 0 if sv is None else len(sv), int(nsave),
+                                  int(bool(save_everystep)), flags, int(max_attempts), _p(ts, _fp), _p(us, _fp),
+                                  _p(rc, _ip), _p(na, _ip), _p(nr, _ip), int(nthreads or os.cpu_count() or 1),
+                                  _p(tst, _fp), 0 if tst is None else len(tst), ctypes.addressof(arr), len(cbs))
     if r != 0:
         raise RuntimeError(f"oracle solve failed ({r})")
     return dict(ts=ts, us=us, retcode=rc, naccept=na, nreject=nr)

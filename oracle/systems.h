@@ -17,7 +17,7 @@
 #define DEG_ORACLE_SYSTEMS_H
 
 enum { OSYS_LORENZ = 0, OSYS_DECAY = 1, OSYS_GROWTH = 2, OSYS_QUAD = 3, OSYS_ROBERTSON = 4,
-       OSYS_VDP = 5, OSYS_TDEP = 6, OSYS_ROT = 7, OSYS_COUNT = 8 };
+       OSYS_VDP = 5, OSYS_TDEP = 6, OSYS_ROT = 7, OSYS_BALL = 8, OSYS_COUNT = 9 };
 
 typedef void (*orhs_fn)(const float *u, const float *p, float t, float *du);
 typedef void (*ojac_fn)(const float *u, const float *p, float t, float *J);
@@ -134,6 +134,18 @@ static void rot_jac(const float *u, const float *p, float t, float *J) {
     J[2] = p[0];  J[3] = 0.0f;
 }
 
+/* falling ball: u1' = u2, u2' = -p1 (test/gpu_kernel_de/gpu_ode_continuous_callbacks.jl:6-10) */
+static void ball_f(const float *u, const float *p, float t, float *du) {
+    (void)t;
+    du[0] = u[1];
+    du[1] = -p[0];
+}
+static void ball_jac(const float *u, const float *p, float t, float *J) {
+    (void)u; (void)p; (void)t;
+    J[0] = 0.0f;  J[1] = 1.0f;
+    J[2] = 0.0f;  J[3] = 0.0f;
+}
+
 static const osys OSYSTEMS[OSYS_COUNT] = {
     {3, 3, lorenz_f, lorenz_jac, zero_tgrad3},
     {1, 1, decay_f, decay_jac, zero_tgrad1},
@@ -143,6 +155,7 @@ static const osys OSYSTEMS[OSYS_COUNT] = {
     {2, 1, vdp_f, vdp_jac, zero_tgrad2},
     {1, 1, tdep_f, tdep_jac, tdep_tgrad},
     {2, 1, rot_f, rot_jac, zero_tgrad2},
+    {2, 1, ball_f, ball_jac, zero_tgrad2},
 };
 
 /* The same right-hand sides in double precision, for the FP64 oracle. */
@@ -183,9 +196,14 @@ static void rot_fd(const double *u, const double *p, double t, double *du) {
     du[0] = -p[0] * u[1];
     du[1] = p[0] * u[0];
 }
+static void ball_fd(const double *u, const double *p, double t, double *du) {
+    (void)t;
+    du[0] = u[1];
+    du[1] = -p[0];
+}
 typedef void (*orhs_fn_d)(const double *u, const double *p, double t, double *du);
 static const orhs_fn_d OSYSTEMS_D[OSYS_COUNT] = {
-    lorenz_fd, decay_fd, growth_fd, quad_fd, robertson_fd, vdp_fd, tdep_fd, rot_fd,
+    lorenz_fd, decay_fd, growth_fd, quad_fd, robertson_fd, vdp_fd, tdep_fd, rot_fd, ball_fd,
 };
 
 #endif

@@ -42,7 +42,8 @@ def test_solve_args_offsets_match_header(deg):
     off = {f[0]: getattr(_lib.SolveArgs, f[0]).offset for f in _lib.SolveArgs._fields_}
     assert off["n_traj"] == 8 and off["u0"] == 32 and off["t0"] == 64 and off["saveat"] == 88
     assert off["ts"] == 112 and off["stream"] == 160 and off["steplog_cap"] == 208 and off["chunk"] == 216
-    assert off["order"] == 224 and ctypes.sizeof(_lib.SolveArgs) == 232
+    assert off["order"] == 224 and off["tstops"] == 232 and off["ntstops"] == 240
+    assert ctypes.sizeof(_lib.SolveArgs) == 248
 
 
 @pytest.mark.parametrize("method", [0, 1, 2])
