@@ -39,6 +39,13 @@ class GPUVern7(GPUODEAlgorithm):
     order = 7
 
 
+class GPUVern9(GPUODEAlgorithm):
+    """Verner 9(8), not FSAL (gpukernel_algorithms.jl:83); dense output with ten extra stages."""
+
+    method = _lib.DEG_VERN9
+    order = 9
+
+
 class GPURosenbrock23(GPUODEAlgorithm):
     """Rosenbrock 2(3) (gpukernel_algorithms.jl:112, :214-220). ``autodiff=True`` derives the
     Jacobian by forward-mode differentiation of the traced RHS (unless the problem's

@@ -5,13 +5,13 @@
 // Reference behaviour restated here (DiffEqGPU src/ensemblegpukernel/):
 //   kernels.jl:1-72 / :74-152          per-trajectory loop, initial save, endpoint rules
 //   perform_step/gpu_tsit5_perform_step.jl:1-175, gpu_vern7_perform_step.jl:1-210,
-//   perform_step/gpu_rosenbrock23_perform_step.jl:1-201   one step / one attempt
+//   perform_step/gpu_rosenbrock23_perform_step.jl:1-201, gpu_vern9_perform_step.jl:1-319   one step / one attempt
 //   integrators/integrator_utils.jl:1-50   PI constants, savevalues!
-//   integrators/nonstiff/interpolants.jl:372-387, stiff/interpolants.jl   dense output
+//   integrators/nonstiff/interpolants.jl:28-370, :372-387, stiff/interpolants.jl   dense output
 //
 // Configuration macros supplied by the host before this file:
 //   DEG_N, DEG_NP                state and parameter counts
-//   DEG_METHOD                   0 Tsit5, 1 Vern7, 2 Rosenbrock23
+//   DEG_METHOD                   0 Tsit5, 1 Vern7, 2 Rosenbrock23, 3 Vern9
 //   DEG_ADAPTIVE                 0 fixed step, 1 adaptive (PI controller)
 //   DEG_DETPOW                   controller powers via det_pow (bitwise-reproducible on CPU)
 //   DEG_FASTPOW                  controller powers via hardware log2/exp2
@@ -156,6 +156,8 @@ __device__ __forceinline__ float scaled_error2_fast(float (&e)[DEG_N], const flo
 #define DEG_ORDER 5
 #elif DEG_METHOD == 1
 #define DEG_ORDER 7
+#elif DEG_METHOD == 3
+#define DEG_ORDER 9
 #else
 #define DEG_ORDER 2
 #endif
@@ -317,6 +319,122 @@ __device__ __noinline__ void deg_interp(float th, float dt, float tprev, const f
                                  k14[c] * q14 + k15[c] * q15 + k16[c] * q16);
         out[c] = (y0[c] + th * step_sum) + thm * corr;
     }
+}
+#endif
+
+// ------------------------------------------------------------------------------------
+// Vern9 (gpu_vern9_perform_step.jl:36-105, :166-240); not FSAL: k1 recomputed per attempt
+// ------------------------------------------------------------------------------------
+#if DEG_METHOD == 3
+#define DEG_NK 16
+__device__ __forceinline__ void deg_attempt(const float (&uprev)[DEG_N], const float* p, float t, float dt,
+                                            float (&k)[DEG_NK + 1][DEG_N], float (&unew)[DEG_N]) {
+    float tmp[DEG_N];
+    const float a = dt * V9_a0201;
+    DEG_FOR(c) tmp[c] = uprev[c] + a * k[1][c];
+    deg_rhs(tmp, p, t + V9_c1 * dt, k[2]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a0301 * k[1][c] + V9_a0302 * k[2][c]);
+    deg_rhs(tmp, p, t + V9_c2 * dt, k[3]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a0401 * k[1][c] + V9_a0403 * k[3][c]);
+    deg_rhs(tmp, p, t + V9_c3 * dt, k[4]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a0501 * k[1][c] + V9_a0503 * k[3][c] + V9_a0504 * k[4][c]);
+    deg_rhs(tmp, p, t + V9_c4 * dt, k[5]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a0601 * k[1][c] + V9_a0604 * k[4][c] + V9_a0605 * k[5][c]);
+    deg_rhs(tmp, p, t + V9_c5 * dt, k[6]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a0701 * k[1][c] + V9_a0704 * k[4][c] + V9_a0705 * k[5][c] + V9_a0706 * k[6][c]);
+    deg_rhs(tmp, p, t + V9_c6 * dt, k[7]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a0801 * k[1][c] + V9_a0806 * k[6][c] + V9_a0807 * k[7][c]);
+    deg_rhs(tmp, p, t + V9_c7 * dt, k[8]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a0901 * k[1][c] + V9_a0906 * k[6][c] + V9_a0907 * k[7][c] + V9_a0908 * k[8][c]);
+    deg_rhs(tmp, p, t + V9_c8 * dt, k[9]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a1001 * k[1][c] + V9_a1006 * k[6][c] + V9_a1007 * k[7][c] + V9_a1008 * k[8][c] + V9_a1009 * k[9][c]);
+    deg_rhs(tmp, p, t + V9_c9 * dt, k[10]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a1101 * k[1][c] + V9_a1106 * k[6][c] + V9_a1107 * k[7][c] + V9_a1108 * k[8][c] + V9_a1109 * k[9][c] + V9_a1110 * k[10][c]);
+    deg_rhs(tmp, p, t + V9_c10 * dt, k[11]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a1201 * k[1][c] + V9_a1206 * k[6][c] + V9_a1207 * k[7][c] + V9_a1208 * k[8][c] + V9_a1209 * k[9][c] + V9_a1210 * k[10][c] + V9_a1211 * k[11][c]);
+    deg_rhs(tmp, p, t + V9_c11 * dt, k[12]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a1301 * k[1][c] + V9_a1306 * k[6][c] + V9_a1307 * k[7][c] + V9_a1308 * k[8][c] + V9_a1309 * k[9][c] + V9_a1310 * k[10][c] + V9_a1311 * k[11][c] + V9_a1312 * k[12][c]);
+    deg_rhs(tmp, p, t + V9_c12 * dt, k[13]);
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a1401 * k[1][c] + V9_a1406 * k[6][c] + V9_a1407 * k[7][c] + V9_a1408 * k[8][c] + V9_a1409 * k[9][c] + V9_a1410 * k[10][c] + V9_a1411 * k[11][c] + V9_a1412 * k[12][c] + V9_a1413 * k[13][c]);
+    deg_rhs(tmp, p, t + V9_c13 * dt, k[14]);
+    float g16[DEG_N];
+    DEG_FOR(c) tmp[c] = uprev[c] + dt * (V9_a1501 * k[1][c] + V9_a1506 * k[6][c] + V9_a1507 * k[7][c] + V9_a1508 * k[8][c] + V9_a1509 * k[9][c] + V9_a1510 * k[10][c] + V9_a1511 * k[11][c] + V9_a1512 * k[12][c] + V9_a1513 * k[13][c] + V9_a1514 * k[14][c]);
+    DEG_FOR(c) g16[c] = uprev[c] + dt * (V9_a1601 * k[1][c] + V9_a1606 * k[6][c] + V9_a1607 * k[7][c] + V9_a1608 * k[8][c] + V9_a1609 * k[9][c] + V9_a1610 * k[10][c] + V9_a1611 * k[11][c] + V9_a1612 * k[12][c] + V9_a1613 * k[13][c]);
+    deg_rhs(tmp, p, t + dt, k[15]);
+    deg_rhs(g16, p, t + dt, k[16]);
+    DEG_FOR(c) unew[c] = uprev[c] + dt * (V9_b1 * k[1][c] + V9_b8 * k[8][c] + V9_b9 * k[9][c] + V9_b10 * k[10][c] + V9_b11 * k[11][c] + V9_b12 * k[12][c] + V9_b13 * k[13][c] + V9_b14 * k[14][c] + V9_b15 * k[15][c]);
+}
+__device__ __forceinline__ float deg_error(float dt, const float (&k)[DEG_NK + 1][DEG_N], const float (&uprev)[DEG_N],
+                                           const float (&unew)[DEG_N], float abstol, float reltol) {
+    float e[DEG_N];
+    DEG_FOR(c) e[c] = dt * (V9_btilde1 * k[1][c] + V9_btilde8 * k[8][c] + V9_btilde9 * k[9][c] + V9_btilde10 * k[10][c] + V9_btilde11 * k[11][c] + V9_btilde12 * k[12][c] + V9_btilde13 * k[13][c] + V9_btilde14 * k[14][c] + V9_btilde15 * k[15][c] + V9_btilde16 * k[16][c]);
+#if DEG_FASTPOW
+    return scaled_error2_fast(e, uprev, unew, abstol, reltol);
+#else
+    return scaled_error(e, uprev, unew, abstol, reltol);
+#endif
+}
+// b_i(th) by fused Horner steps: th * H(r_i1..r_i9) for stage 1, th^2 * H(r_i2..r_i9) otherwise
+#define V9B1(th) ((th) * V9H8(th, V9_r011, V9_r012, V9_r013, V9_r014, V9_r015, V9_r016, V9_r017, V9_r018, V9_r019))
+#define V9B(i, th) ((th) * ((th) * V9H7(th, V9_r##i##2, V9_r##i##3, V9_r##i##4, V9_r##i##5, V9_r##i##6, V9_r##i##7, V9_r##i##8, V9_r##i##9)))
+__device__ __forceinline__ float V9H7(float th, float r2, float r3, float r4, float r5, float r6, float r7, float r8, float r9) {
+    float h = __builtin_fmaf(th, r9, r8);
+    h = __builtin_fmaf(th, h, r7);
+    h = __builtin_fmaf(th, h, r6);
+    h = __builtin_fmaf(th, h, r5);
+    h = __builtin_fmaf(th, h, r4);
+    h = __builtin_fmaf(th, h, r3);
+    return __builtin_fmaf(th, h, r2);
+}
+__device__ __forceinline__ float V9H8(float th, float r1, float r2, float r3, float r4, float r5, float r6, float r7, float r8, float r9) {
+    return __builtin_fmaf(th, V9H7(th, r2, r3, r4, r5, r6, r7, r8, r9), r1);
+}
+// Vern9 dense output (interpolants.jl:192-370): extra stages 17..26 from uprev and the step's
+// stages 1, 8..15, then y0 + dt * (sum of b_i(th) k_i). Scalar muladds (stage times, Horner
+// steps) are fused; vector sums are unfused left folds.
+__device__ __noinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                        const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
+    float tmp[DEG_N], x[10][DEG_N];  // extra stages 17..26
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V9_a1701 * k[1][c] + V9_a1708 * k[8][c] + V9_a1709 * k[9][c] + V9_a1710 * k[10][c] + V9_a1711 * k[11][c] + V9_a1712 * k[12][c] + V9_a1713 * k[13][c] + V9_a1714 * k[14][c] + V9_a1715 * k[15][c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V9_c17, dt, tprev), x[0]);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V9_a1801 * k[1][c] + V9_a1808 * k[8][c] + V9_a1809 * k[9][c] + V9_a1810 * k[10][c] + V9_a1811 * k[11][c] + V9_a1812 * k[12][c] + V9_a1813 * k[13][c] + V9_a1814 * k[14][c] + V9_a1815 * k[15][c] + V9_a1817 * x[0][c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V9_c18, dt, tprev), x[1]);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V9_a1901 * k[1][c] + V9_a1908 * k[8][c] + V9_a1909 * k[9][c] + V9_a1910 * k[10][c] + V9_a1911 * k[11][c] + V9_a1912 * k[12][c] + V9_a1913 * k[13][c] + V9_a1914 * k[14][c] + V9_a1915 * k[15][c] + V9_a1917 * x[0][c] + V9_a1918 * x[1][c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V9_c19, dt, tprev), x[2]);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V9_a2001 * k[1][c] + V9_a2008 * k[8][c] + V9_a2009 * k[9][c] + V9_a2010 * k[10][c] + V9_a2011 * k[11][c] + V9_a2012 * k[12][c] + V9_a2013 * k[13][c] + V9_a2014 * k[14][c] + V9_a2015 * k[15][c] + V9_a2017 * x[0][c] + V9_a2018 * x[1][c] + V9_a2019 * x[2][c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V9_c20, dt, tprev), x[3]);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V9_a2101 * k[1][c] + V9_a2108 * k[8][c] + V9_a2109 * k[9][c] + V9_a2110 * k[10][c] + V9_a2111 * k[11][c] + V9_a2112 * k[12][c] + V9_a2113 * k[13][c] + V9_a2114 * k[14][c] + V9_a2115 * k[15][c] + V9_a2117 * x[0][c] + V9_a2118 * x[1][c] + V9_a2119 * x[2][c] + V9_a2120 * x[3][c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V9_c21, dt, tprev), x[4]);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V9_a2201 * k[1][c] + V9_a2208 * k[8][c] + V9_a2209 * k[9][c] + V9_a2210 * k[10][c] + V9_a2211 * k[11][c] + V9_a2212 * k[12][c] + V9_a2213 * k[13][c] + V9_a2214 * k[14][c] + V9_a2215 * k[15][c] + V9_a2217 * x[0][c] + V9_a2218 * x[1][c] + V9_a2219 * x[2][c] + V9_a2220 * x[3][c] + V9_a2221 * x[4][c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V9_c22, dt, tprev), x[5]);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V9_a2301 * k[1][c] + V9_a2308 * k[8][c] + V9_a2309 * k[9][c] + V9_a2310 * k[10][c] + V9_a2311 * k[11][c] + V9_a2312 * k[12][c] + V9_a2313 * k[13][c] + V9_a2314 * k[14][c] + V9_a2315 * k[15][c] + V9_a2317 * x[0][c] + V9_a2318 * x[1][c] + V9_a2319 * x[2][c] + V9_a2320 * x[3][c] + V9_a2321 * x[4][c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V9_c23, dt, tprev), x[6]);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V9_a2401 * k[1][c] + V9_a2408 * k[8][c] + V9_a2409 * k[9][c] + V9_a2410 * k[10][c] + V9_a2411 * k[11][c] + V9_a2412 * k[12][c] + V9_a2413 * k[13][c] + V9_a2414 * k[14][c] + V9_a2415 * k[15][c] + V9_a2417 * x[0][c] + V9_a2418 * x[1][c] + V9_a2419 * x[2][c] + V9_a2420 * x[3][c] + V9_a2421 * x[4][c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V9_c24, dt, tprev), x[7]);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V9_a2501 * k[1][c] + V9_a2508 * k[8][c] + V9_a2509 * k[9][c] + V9_a2510 * k[10][c] + V9_a2511 * k[11][c] + V9_a2512 * k[12][c] + V9_a2513 * k[13][c] + V9_a2514 * k[14][c] + V9_a2515 * k[15][c] + V9_a2517 * x[0][c] + V9_a2518 * x[1][c] + V9_a2519 * x[2][c] + V9_a2520 * x[3][c] + V9_a2521 * x[4][c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V9_c25, dt, tprev), x[8]);
+    DEG_FOR(c) tmp[c] = y0[c] + dt * (V9_a2601 * k[1][c] + V9_a2608 * k[8][c] + V9_a2609 * k[9][c] + V9_a2610 * k[10][c] + V9_a2611 * k[11][c] + V9_a2612 * k[12][c] + V9_a2613 * k[13][c] + V9_a2614 * k[14][c] + V9_a2615 * k[15][c] + V9_a2617 * x[0][c] + V9_a2618 * x[1][c] + V9_a2619 * x[2][c] + V9_a2620 * x[3][c] + V9_a2621 * x[4][c]);
+    deg_rhs(tmp, p, __builtin_fmaf(V9_c26, dt, tprev), x[9]);
+    const float b1 = V9B1(th);
+    const float b8 = V9B(08, th);
+    const float b9 = V9B(09, th);
+    const float b10 = V9B(10, th);
+    const float b11 = V9B(11, th);
+    const float b12 = V9B(12, th);
+    const float b13 = V9B(13, th);
+    const float b14 = V9B(14, th);
+    const float b15 = V9B(15, th);
+    const float b17 = V9B(17, th);
+    const float b18 = V9B(18, th);
+    const float b19 = V9B(19, th);
+    const float b20 = V9B(20, th);
+    const float b21 = V9B(21, th);
+    const float b22 = V9B(22, th);
+    const float b23 = V9B(23, th);
+    const float b24 = V9B(24, th);
+    const float b25 = V9B(25, th);
+    const float b26 = V9B(26, th);
+    DEG_FOR(c) out[c] = y0[c] + dt * (k[1][c] * b1 + k[8][c] * b8 + k[9][c] * b9 + k[10][c] * b10 + k[11][c] * b11 + k[12][c] * b12 + k[13][c] * b13 + k[14][c] * b14 + k[15][c] * b15 + x[0][c] * b17 + x[1][c] * b18 + x[2][c] * b19 + x[3][c] * b20 + x[4][c] * b21 + x[5][c] * b22 + x[6][c] * b23 + x[7][c] * b24 + x[8][c] * b25 + x[9][c] * b26);
 }
 #endif
 
@@ -1141,7 +1259,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                 fsal_pending = false;
                 deg_attempt(u, p, t, dt, k, unew);
                 EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
-#elif DEG_METHOD == 1
+#elif DEG_METHOD == 1 || DEG_METHOD == 3
                 deg_rhs(u, p, t, k[1]);
                 deg_attempt(u, p, t, dt, k, unew);
                 EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
@@ -1212,7 +1330,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                     DEG_FOR(c) k[1][c] = k[7][c];
                 }
                 deg_attempt(uprev, p, told, dts, k, unew);
-#elif DEG_METHOD == 1
+#elif DEG_METHOD == 1 || DEG_METHOD == 3
                 deg_rhs(uprev, p, told, k[1]);
                 deg_attempt(uprev, p, told, dts, k, unew);
 #else

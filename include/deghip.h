@@ -33,10 +33,11 @@ extern "C" {
 #define DEG_E_UNSUPPORTED (-4)
 #define DEG_E_NOMEM (-5)
 
-/* methods: GPUTsit5, GPUVern7, GPURosenbrock23 (src/ensemblegpukernel/gpukernel_algorithms.jl:30,57,112) */
+/* methods: GPUTsit5, GPUVern7, GPURosenbrock23, GPUVern9 (src/ensemblegpukernel/gpukernel_algorithms.jl:30,57,112,83) */
 #define DEG_TSIT5 0
 #define DEG_VERN7 1
 #define DEG_RB23 2
+#define DEG_VERN9 3
 
 /* compile flags (one code object per (rhs, n, np, method, flags)) */
 #define DEG_F_ADAPTIVE (1u << 0)   /* PI-controlled adaptive stepping (vectorized_asolve) */

@@ -10,9 +10,11 @@
  *   Tsit5 fixed / adaptive step             perform_step/gpu_tsit5_perform_step.jl:1-64 / :68-175
  *   Vern7 fixed / adaptive step             perform_step/gpu_vern7_perform_step.jl:1-74 / :78-210
  *   Rosenbrock23 fixed / adaptive step      perform_step/gpu_rosenbrock23_perform_step.jl:1-70 / :74-201
+ *   Vern9 fixed / adaptive step             perform_step/gpu_vern9_perform_step.jl:1-124 / :128-319
  *   PI controller constants                 integrators/integrator_utils.jl:1-11, alg_utils.jl:5-8
  *   saveat / save_everystep writes          integrators/integrator_utils.jl:13-50
- *   dense output                            integrators/nonstiff/interpolants.jl:28-190 (Vern7), :372-387
+ *   dense output                            integrators/nonstiff/interpolants.jl:28-190 (Vern7), :192-370
+ *                                           (Vern9), :372-387
  *                                           (Tsit5), integrators/stiff/interpolants.jl (Rosenbrock23)
  *   error norm                              DiffEqBase.ODE_DEFAULT_NORM = sqrt(sum(abs2, x) / n)
  *   tstops                                  perform_step/gpu_tsit5_perform_step.jl:18-27, :158-164
@@ -55,7 +57,7 @@
 #define OMAXN 16
 #define OMAXS 17
 
-enum { OM_TSIT5 = 0, OM_VERN7 = 1, OM_RB23 = 2 };
+enum { OM_TSIT5 = 0, OM_VERN7 = 1, OM_RB23 = 2, OM_VERN9 = 3 };
 enum { DEG_O_KAHAN = 1, DEG_O_DETPOW = 2 };
 /* per-trajectory return codes (same values as the C ABI of the HIP library) */
 enum { ORET_SUCCESS = 0, ORET_DTMIN = 1, ORET_MAXITERS = 2, ORET_NONFINITE = 3, ORET_TERMINATED = 4,
@@ -94,8 +96,9 @@ static void erk32_load(erk32 *T, int s, int fsal, const double *c, const double 
     }
 }
 
-static erk32 TAB_T5, TAB_V7;
+static erk32 TAB_T5, TAB_V7, TAB_V9;
 static void init_v7x(void);
+static void init_v9x(void);
 static float T5R[8][5];
 static int tabs_ready = 0;
 
@@ -103,9 +106,11 @@ static void init_tabs(void) {
     if (tabs_ready) return;
     erk32_load(&TAB_T5, OT5_S, 1, OT5_C, &OT5_A[0][0], &OT5_AM[0][0], OT5_B, OT5_BM, OT5_BT, OT5_BTM);
     erk32_load(&TAB_V7, OV7_S, 0, OV7_C, &OV7_A[0][0], &OV7_AM[0][0], OV7_B, OV7_BM, OV7_BT, OV7_BTM);
+    erk32_load(&TAB_V9, OV9_S, 0, OV9_C, &OV9_A[0][0], &OV9_AM[0][0], OV9_B, OV9_BM, OV9_BT, OV9_BTM);
     for (int i = 0; i < 8; ++i)
         for (int k = 0; k < 5; ++k) T5R[i][k] = (float)OT5_R[i][k];
     init_v7x();
+    init_v9x();
     tabs_ready = 1;
 }
 
@@ -337,6 +342,62 @@ static void vern7_interp(const osys *S, const float *p, float th, float dt, floa
     }
 }
 
+/* Vern9 dense output (interpolants.jl:192-370): extra stages 17..26 from uprev and the step's
+ * stages 1, 8..15, then y0 + dt * (sum of b_i(th) k_i) with b_1 = th*H(r_11..r_19) and
+ * b_i = th*(th*H(r_i2..r_i9)), H a fused Horner evaluation (@evalpoly). Stage times are fused
+ * muladds; vector sums are unfused left folds. */
+static float V9X_A[27][27], V9X_C[27], V9R[27][10];
+static unsigned char V9X_AM[27][27];
+static const int V9_BSTAGES[19] = {1, 8, 9, 10, 11, 12, 13, 14, 15, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26};
+
+static void init_v9x(void) {
+    for (int i = 0; i < 27; ++i) {
+        V9X_C[i] = (float)OV9X_C[i];
+        for (int j = 0; j < 27; ++j) {
+            V9X_A[i][j] = (float)OV9X_A[i][j];
+            V9X_AM[i][j] = OV9X_AM[i][j];
+        }
+        for (int k = 0; k < 10; ++k) V9R[i][k] = (float)OV9R[i][k];
+    }
+}
+
+static float vern9_b(int i, float th) {
+    const float *r = V9R[i];
+    float h = fmaf(th, r[9], r[8]);
+    for (int k = 7; k >= 2; --k) h = fmaf(th, h, r[k]);
+    if (i == 1) return th * fmaf(th, h, r[1]);
+    return th * (th * h);
+}
+
+static void vern9_interp(const osys *S, const float *p, float th, float dt, float tprev, int n, const float *y0,
+                         float k[][OMAXN], float *out) {
+    float kx[27][OMAXN];
+    for (int s = 1; s <= 16; ++s)
+        for (int c = 0; c < n; ++c) kx[s][c] = k[s][c];
+    float tmp[OMAXN];
+    for (int i = 17; i <= 26; ++i) {
+        for (int c = 0; c < n; ++c) {
+            float acc = 0.0f;
+            int first = 1;
+            for (int j = 1; j < i; ++j) {
+                if (!V9X_AM[i][j]) continue;
+                float term = V9X_A[i][j] * kx[j][c];
+                acc = first ? term : acc + term;
+                first = 0;
+            }
+            tmp[c] = y0[c] + dt * acc;
+        }
+        S->f(tmp, p, fmaf(V9X_C[i], dt, tprev), kx[i]);
+    }
+    float b[27];
+    for (int m = 0; m < 19; ++m) b[V9_BSTAGES[m]] = vern9_b(V9_BSTAGES[m], th);
+    for (int c = 0; c < n; ++c) {
+        float acc = kx[1][c] * b[1];
+        for (int m = 1; m < 19; ++m) acc = acc + kx[V9_BSTAGES[m]][c] * b[V9_BSTAGES[m]];
+        out[c] = y0[c] + dt * acc;
+    }
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* FP64 LU with partial pivoting for W = I - gamma*J (Rosenbrock23)                     */
 /* ------------------------------------------------------------------------------------ */
@@ -499,6 +560,7 @@ static void interp_at(const ojob *J, const ost *s, float x, float *out) {
     const float d = 1.0f / (2.0f + sqrtf(2.0f));
     if (J->method == OM_TSIT5) tsit5_interp(th, s->dtstep, s->n, s->uprev, (float(*)[OMAXN])s->k, out);
     else if (J->method == OM_RB23) rb23_interp(th, s->dtstep, s->n, d, s->uprev, s->rk1, s->rk2, out);
+    else if (J->method == OM_VERN9) vern9_interp(J->S, s->p, th, s->dtstep, s->tprev, s->n, s->uprev, (float(*)[OMAXN])s->k, out);
     else vern7_interp(J->S, s->p, th, s->dtstep, s->tprev, s->n, s->uprev, (float(*)[OMAXN])s->k, out);
 }
 
@@ -713,7 +775,7 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                       int32_t *nacc_out, int32_t *nrej_out) {
     const osys *S = J->S;
     const int n = S->n;
-    const erk32 *T = (J->method == OM_TSIT5) ? &TAB_T5 : &TAB_V7;
+    const erk32 *T = (J->method == OM_TSIT5) ? &TAB_T5 : (J->method == OM_VERN9 ? &TAB_V9 : &TAB_V7);
     ost st;
     ost *s = &st;
     memset(s, 0, sizeof(st));
@@ -819,7 +881,7 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
         }
     } else {
         /* ---------------- adaptive: kernels.jl:128-131 -------------------------------- */
-        const int order = (J->method == OM_TSIT5) ? 5 : (J->method == OM_VERN7 ? 7 : 2);
+        const int order = (J->method == OM_TSIT5) ? 5 : (J->method == OM_VERN7 ? 7 : (J->method == OM_VERN9 ? 9 : 2));
         /* integrator_utils.jl:1-11 */
         const float beta1 = (float)(7.0 / (10.0 * order));
         const float beta2 = (float)(2.0 / (5.0 * order));
@@ -876,7 +938,8 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                     }
                     EEst = rms_norm(e, n);
                 } else {
-                    if (J->method == OM_VERN7) S->f(s->uprev, s->p, t, k[1]); /* gpu_vern7_perform_step.jl:117 */
+                    /* gpu_vern7_perform_step.jl:117, gpu_vern9_perform_step.jl:170: k1 in every attempt */
+                    if (J->method == OM_VERN7 || J->method == OM_VERN9) S->f(s->uprev, s->p, t, k[1]);
                     erk_attempt(T, S, s->p, s->uprev, t, dt, k, unew);
                     EEst = erk_eest(T, n, dt, k, s->uprev, unew, J->abstol, J->reltol);
                 }
@@ -962,7 +1025,7 @@ int deg_oracle_solve_cb(int system, int method, int adaptive, int64_t N,
                         float *ts, float *us, int32_t *retcode, int32_t *naccept, int32_t *nreject,
                         int nthreads, const float *tstops, int ntstops, const ocb *cbs, int ncb) {
     if (system < 0 || system >= OSYS_COUNT) return -1;
-    if (method < OM_TSIT5 || method > OM_RB23) return -1;
+    if (method < OM_TSIT5 || method > OM_VERN9) return -1;
     if (ntstops < 0 || ncb < 0 || (ntstops > 0 && !tstops) || (ncb > 0 && !cbs)) return -1;
     init_tabs();
     const osys *S = &OSYSTEMS[system];
@@ -1168,3 +1231,18 @@ void deg_oracle_tsit5_btheta(float th, float *out7) {
     for (int i = 2; i <= 7; ++i) out7[i - 1] = th * (th * fmaf(th, fmaf(th, T5R[i][4], T5R[i][3]), T5R[i][2]));
 }
 double deg_oracle_tsit5_r(int i, int k) { return OT5_R[i][k]; }
+
+/* Vern9 dense output: extra-stage nodes C[27], coefficients A[27*27] (rows 17..26) and the
+ * polynomial coefficients R[27*10], in double; and the Float32 b_i(th) the oracle evaluates. */
+void deg_oracle_vern9_dense(double *C, double *A, double *R) {
+    for (int i = 0; i < 27; ++i) {
+        C[i] = OV9X_C[i];
+        for (int j = 0; j < 27; ++j) A[i * 27 + j] = OV9X_A[i][j];
+        for (int k = 0; k < 10; ++k) R[i * 10 + k] = OV9R[i][k];
+    }
+}
+void deg_oracle_vern9_btheta(float th, float *out27) {
+    init_tabs();
+    for (int i = 0; i < 27; ++i) out27[i] = 0.0f;
+    for (int m = 0; m < 19; ++m) out27[V9_BSTAGES[m]] = vern9_b(V9_BSTAGES[m], th);
+}

@@ -18,7 +18,7 @@ SYSTEM_IDS = {"lorenz": 0, "decay": 1, "growth": 2, "quad": 3, "robertson": 4, "
               "ball": 8}
 SYSTEM_DIMS = {"lorenz": (3, 3), "decay": (1, 1), "growth": (1, 1), "quad": (1, 1), "robertson": (3, 3),
                "vdp": (2, 1), "tdep": (1, 1), "rot": (2, 1), "ball": (2, 1)}
-METHOD_IDS = {"tsit5": 0, "vern7": 1, "rb23": 2}
+METHOD_IDS = {"tsit5": 0, "vern7": 1, "rb23": 2, "vern9": 3}
 KAHAN, DETPOW = 1, 2
 
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -102,6 +102,8 @@ def lib():
         L.deg_oracle_tsit5_btheta.argtypes = [ctypes.c_float, _fp]
         L.deg_oracle_tsit5_r.argtypes = [ctypes.c_int, ctypes.c_int]
         L.deg_oracle_tsit5_r.restype = ctypes.c_double
+        L.deg_oracle_vern9_dense.argtypes = [_dp, _dp, _dp]
+        L.deg_oracle_vern9_btheta.argtypes = [ctypes.c_float, _fp]
         _lib = L
     return _lib
 
@@ -200,3 +202,20 @@ def tsit5_r():
         for k in range(5):
             R[i, k] = lib().deg_oracle_tsit5_r(i, k)
     return R
+
+
+def vern9_dense():
+    """Vern9 dense output tables: nodes C (27), coefficients A (27x27, rows 17..26), polynomial
+    coefficients R (27x10), stage indices 1-based."""
+    C = np.zeros(27)
+    A = np.zeros(27 * 27)
+    R = np.zeros(27 * 10)
+    lib().deg_oracle_vern9_dense(_p(C, _dp), _p(A, _dp), _p(R, _dp))
+    return C, A.reshape(27, 27), R.reshape(27, 10)
+
+
+def vern9_btheta(theta: float) -> np.ndarray:
+    """The Float32 weights b_i(theta) (index 1..26, 0 unused) the oracle's Vern9 dense output uses."""
+    out = np.zeros(27, np.float32)
+    lib().deg_oracle_vern9_btheta(float(np.float32(theta)), _p(out, _fp))
+    return out

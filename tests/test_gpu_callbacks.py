@@ -104,7 +104,7 @@ def _run(deg, torch, system, method, adaptive, names, *, dt, N=192, seed=0, save
     dev = deg.HIPDevice(contract="off", powmode="det", persistent=persistent, max_attempts=MAX_ATTEMPTS)
     probs = deg.HIPProbVector.from_arrays(f, u0, p, tspan, dev)
     prob = deg.ODEProblem(f, u0[:, 0], tspan, p[:, 0])
-    alg = {"tsit5": deg.GPUTsit5, "vern7": deg.GPUVern7, "rb23": deg.GPURosenbrock23}[method]()
+    alg = {"tsit5": deg.GPUTsit5, "vern7": deg.GPUVern7, "rb23": deg.GPURosenbrock23, "vern9": deg.GPUVern9}[method]()
     kw = dict(dt=dt, callback=cbs, tstops=tstops)
     if saveat is not None:
         kw["saveat"] = saveat
@@ -135,7 +135,7 @@ def _assert_same(got, o, rows_written=None):
     np.testing.assert_array_equal(got["us"][mask], o["us"][mask], err_msg="us")
 
 
-@pytest.mark.parametrize("method", ["tsit5", "vern7", "rb23"])
+@pytest.mark.parametrize("method", ["tsit5", "vern7", "rb23", "vern9"])
 @pytest.mark.parametrize("persistent", [False, True])
 def test_discrete_fixed_step(deg, torch_cuda, method, persistent):
     for names, tst, dt, kw in ((["jump24"], [2.4], 1.0, {}),
@@ -148,7 +148,7 @@ def test_discrete_fixed_step(deg, torch_cuda, method, persistent):
         _assert_same(got, o)
 
 
-@pytest.mark.parametrize("method", ["tsit5", "vern7", "rb23"])
+@pytest.mark.parametrize("method", ["tsit5", "vern7", "rb23", "vern9"])
 @pytest.mark.parametrize("persistent", [False, True])
 def test_discrete_adaptive(deg, torch_cuda, method, persistent):
     for names, tst, kw in ((["jump24"], [4.0], {}),
@@ -218,7 +218,7 @@ def _decay_exact(t, jumps):
     return np.array(out)
 
 
-@pytest.mark.parametrize("alg_name", ["GPUTsit5", "GPUVern7"])
+@pytest.mark.parametrize("alg_name", ["GPUTsit5", "GPUVern7", "GPUVern9"])
 def test_reference_discrete_callbacks(deg, torch_cuda, alg_name):
     """test/gpu_kernel_de/gpu_ode_discrete_callbacks.jl, u' = -u, u(0) = 10 on (0, 10): the
     bounds it asserts against Vern9, here against the closed form."""

@@ -18,7 +18,7 @@ import pyoracle
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
-METHODS = {"tsit5": "GPUTsit5", "vern7": "GPUVern7", "rb23": "GPURosenbrock23"}
+METHODS = {"tsit5": "GPUTsit5", "vern7": "GPUVern7", "rb23": "GPURosenbrock23", "vern9": "GPUVern9"}
 
 
 @pytest.fixture(scope="module")
@@ -85,6 +85,7 @@ FIXED_CASES = [
     ("lorenz", "tsit5", (0.0, 10.0), 0.1), ("lorenz", "vern7", (0.0, 10.0), 0.1), ("lorenz", "rb23", (0.0, 2.0), 0.01),
     ("vdp", "tsit5", (0.0, 5.0), 0.05), ("vdp", "vern7", (0.0, 5.0), 0.05), ("vdp", "rb23", (0.0, 5.0), 0.01),
     ("tdep", "tsit5", (0.0, 2.0), 0.015625), ("rot", "vern7", (0.0, 8.0), 0.5), ("robertson", "rb23", (0.0, 1.0), 0.001),
+    ("lorenz", "vern9", (0.0, 10.0), 0.1), ("vdp", "vern9", (0.0, 5.0), 0.05), ("tdep", "vern9", (0.0, 2.0), 0.0625),
 ]
 
 
@@ -121,7 +122,8 @@ ADAPTIVE_CASES = [
     ("lorenz", "rb23", (0.0, 3.0), 0.01, 1e-5, 1e-5), ("vdp", "tsit5", (0.0, 10.0), 0.01, 1e-6, 1e-6),
     ("vdp", "vern7", (0.0, 10.0), 0.01, 1e-7, 1e-7), ("vdp", "rb23", (0.0, 10.0), 0.01, 1e-6, 1e-4),
     ("robertson", "rb23", (0.0, 1e5), 1e-6, 1e-8, 1e-4), ("quad", "rb23", (0.0, 10.0), 0.01, 1e-6, 1e-3),
-    ("tdep", "tsit5", (0.0, 3.0), 0.1, 1e-7, 1e-7),
+    ("tdep", "tsit5", (0.0, 3.0), 0.1, 1e-7, 1e-7), ("lorenz", "vern9", (0.0, 10.0), 0.1, 1e-8, 1e-8),
+    ("vdp", "vern9", (0.0, 10.0), 0.01, 1e-7, 1e-7),
 ]
 
 
@@ -180,7 +182,7 @@ def test_persistent_equals_plain_grid(deg, torch_cuda):
 # ---------------------------------------------------------------------------------------
 @pytest.mark.parametrize("system,method,tspan,tol", [("vdp", "tsit5", (0.0, 10.0), 1e-6), ("decay", "vern7", (0.0, 5.0), 1e-8),
                                                      ("rot", "tsit5", (0.0, 20.0), 1e-7), ("quad", "rb23", (0.0, 10.0), 1e-5),
-                                                     ("lorenz", "vern7", (0.0, 1.0), 1e-8)])
+                                                     ("lorenz", "vern7", (0.0, 1.0), 1e-8), ("decay", "vern9", (0.0, 5.0), 1e-8)])
 def test_adaptive_powf_fp64_error(deg, torch_cuda, system, method, tspan, tol):
     """Exact operation order with the device powf instead of the oracle's powf: the step
     sequences may drift apart by rounding of the controller, so the bar is the FP64-oracle

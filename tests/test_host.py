@@ -46,7 +46,7 @@ def test_solve_args_offsets_match_header(deg):
     assert ctypes.sizeof(_lib.SolveArgs) == 248
 
 
-@pytest.mark.parametrize("method", [0, 1, 2])
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
 @pytest.mark.parametrize("flags", [0, 2, 1, 1 | 2 | 4, 1 | 16, 1 | 64, 1 | 8 | 16])
 def test_jit_compiles_every_method(deg, method, flags):
     from diffeqgpu_amd import _lib, systems

@@ -92,6 +92,55 @@ def test_tsit5_dense_output():
     assert np.all(pyoracle.tsit5_btheta(0.0) == 0.0)
 
 
+def test_vern9_dense_output():
+    """The Vern9 dense output (ten extra stages, degree-9 weights) is a continuous extension of
+    order 9: b(1) = b with zero weight on the extra stages, b(0) = 0, the extra stages' nodes are
+    their row sums, and sum_i b_i(th) Phi_i(t) = th^|t| / gamma(t) for every tree up to order 9
+    at interior th over the 26-stage extended tableau."""
+    c, A, b, bt = pyoracle.tableau("vern9")
+    C, AX, R = pyoracle.vern9_dense()
+    Ae = np.zeros((26, 26))
+    Ae[:16, :16] = A
+    Ae[16:, :] = AX[17:27, 1:27]
+    np.testing.assert_allclose(Ae[16:].sum(axis=1), C[17:27], atol=1e-14)
+    stages = [1] + list(range(8, 16)) + list(range(17, 27))
+
+    def bth(th):
+        w = np.zeros(26)
+        for i in stages:
+            w[i - 1] = sum(R[i, k] * th ** k for k in range(10))
+        return w
+
+    np.testing.assert_allclose(bth(1.0)[:16], b, atol=1e-11)
+    assert np.abs(bth(1.0)[16:]).max() < 1e-11 and np.all(bth(0.0) == 0.0)
+    for th in (0.3, 0.7):
+        w = bth(th)
+        for q in range(1, 10):
+            for t in B.trees(q):
+                assert abs(w @ B.phi(Ae, t) - th ** q / B.gamma(t)) < 1e-12, (th, q)
+    # a coefficient off in its last digits breaks the order-9 conditions
+    R2 = R.copy()
+    R2[12, 5] *= 1 + 1e-9
+    w = np.zeros(26)
+    for i in stages:
+        w[i - 1] = sum(R2[i, k] * 0.5 ** k for k in range(10))
+    assert max(abs(w @ B.phi(Ae, t) - 0.5 ** 5 / B.gamma(t)) for t in B.trees(5)) > 1e-9
+    # the oracle evaluates the same polynomials in Float32 (fused Horner): within the Horner
+    # rounding bound 2 * 9 * eps32 * sum_k |r_ik| th^k
+    eps = float(np.finfo(np.float32).eps)
+    for th in (0.0, 0.25, 0.5, 1.0):
+        got = pyoracle.vern9_btheta(th)[1:]
+        bound = np.array([18 * eps * sum(abs(R[i, k]) * th ** k for k in range(10)) for i in range(1, 27)])
+        assert np.all(np.abs(got - bth(th)) <= bound + 1e-12), th
+    # a saveat point at a step end (th = 1) reproduces the step's solution up to the Float32
+    # cancellation of the degree-9 weights (coefficients up to 6e3: about 1e-3 relative here)
+    sv = np.array([0.0, 0.05, 0.1], F32)
+    r = pyoracle.solve("lorenz", "vern9", False, LZ_U0, LZ_P, (0.0, 0.1), 0.05, saveat=sv)
+    r2 = pyoracle.solve("lorenz", "vern9", False, LZ_U0, LZ_P, (0.0, 0.1), 0.05, nsave=3, save_everystep=True)
+    assert np.all(np.isfinite(r2["us"]))
+    np.testing.assert_allclose(r["us"][1:], r2["us"][1:], rtol=2e-3)
+
+
 # ---------------------------------------------------------------------------------------
 # deterministic pow of the controller
 # ---------------------------------------------------------------------------------------
@@ -143,7 +192,7 @@ def _endpoint(system, method, u0, p, tspan, dt, adaptive=False, **kw):
 
 
 @pytest.mark.parametrize("method,order,system,dts", [("tsit5", 5, "rot", (0.5, 0.25)), ("vern7", 7, "quad", (1.0, 0.5)),
-                                                    ("rb23", 2, "rot", (0.0625, 0.03125))])
+                                                    ("rb23", 2, "rot", (0.0625, 0.03125)), ("vern9", 9, "rot", (2.0, 1.0))])
 def test_convergence_order(method, order, system, dts):
     """Empirical order under step halving on closed-form problems over (0, 8) with binary-exact
     steps: rotation u1' = -u2, u2' = u1 and u' = -u^2, u(0) = 1 (1 / (1 + t))."""
@@ -161,14 +210,14 @@ def test_convergence_order(method, order, system, dts):
     assert abs(rate - order) < 1.0, (errs, rate)
 
 
-@pytest.mark.parametrize("method", ["tsit5", "vern7", "rb23"])
+@pytest.mark.parametrize("method", ["tsit5", "vern7", "rb23", "vern9"])
 def test_nonautonomous_closed_form(method):
     # u' = p t u, u(0) = 1 -> exp(p t^2 / 2); exercises the stage times t + c_i dt
     p0, T = 0.5, 2.0
     u, r = _endpoint("tdep", method, [1.0], [p0], (0.0, T), 0.015625)
     assert r["ts"][1, 0] == T
     exact = math.exp(p0 * T * T / 2)
-    tol = {"tsit5": 2e-6, "vern7": 2e-6, "rb23": 2e-4}[method]
+    tol = {"tsit5": 2e-6, "vern7": 2e-6, "rb23": 2e-4, "vern9": 2e-6}[method]
     assert abs(u[0] - exact) / exact < tol
 
 
