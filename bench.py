@@ -168,9 +168,9 @@ def jit_compile_seconds(deg, f, n, np_, alg, dev):
     import tempfile
 
     from diffeqgpu_amd import _lib
-    from diffeqgpu_amd.algorithms import GPURosenbrock23
+    from diffeqgpu_amd.algorithms import GPUODEImplicitAlgorithm
 
-    src, _ = f.hip_source(n, np_, isinstance(alg, GPURosenbrock23))
+    src, _ = f.hip_source(n, np_, isinstance(alg, GPUODEImplicitAlgorithm))
     L = _lib.lib()
     with tempfile.TemporaryDirectory() as tmp:
         L.deg_set_cache_dir(tmp.encode())

@@ -46,19 +46,37 @@ class GPUVern9(GPUODEAlgorithm):
     order = 9
 
 
-class GPURosenbrock23(GPUODEAlgorithm):
-    """Rosenbrock 2(3) (gpukernel_algorithms.jl:112, :214-220). ``autodiff=True`` derives the
-    Jacobian by forward-mode differentiation of the traced RHS (unless the problem's
-    ODEFunction supplies ``jac``/``tgrad``); ``autodiff=False`` is not provided (the reference
-    then uses finite differences, alg_utils.jl:17-27)."""
+class GPUODEImplicitAlgorithm(GPUODEAlgorithm):
+    """The methods that use the Jacobian (gpukernel_algorithms.jl: ``GPUODEImplicitAlgorithm{AD}``).
+    ``autodiff=True`` derives the Jacobian by forward-mode differentiation of the traced RHS
+    (unless the problem's ODEFunction supplies ``jac``/``tgrad``); ``autodiff=False`` is not
+    provided (the reference then uses finite differences, alg_utils.jl:17-27)."""
+
+    def __init__(self, autodiff: bool = True):
+        if not autodiff:
+            raise NotImplementedError(f"{type(self).__name__}(autodiff=False) (finite-difference Jacobian) is not provided")
+        self.autodiff = True
+
+
+class GPURosenbrock23(GPUODEImplicitAlgorithm):
+    """Rosenbrock 2(3) (gpukernel_algorithms.jl:112, :214-220)."""
 
     method = _lib.DEG_RB23
     order = 2
 
-    def __init__(self, autodiff: bool = True):
-        if not autodiff:
-            raise NotImplementedError("GPURosenbrock23(autodiff=False) (finite-difference Jacobian) is not provided")
-        self.autodiff = True
+
+class GPURodas4(GPUODEImplicitAlgorithm):
+    """Rodas4, 6 stages, order 4 with a stiffly accurate embedded error (gpukernel_algorithms.jl:137)."""
+
+    method = _lib.DEG_RODAS4
+    order = 4
+
+
+class GPURodas5P(GPUODEImplicitAlgorithm):
+    """Rodas5P, 8 stages, order 5 (gpukernel_algorithms.jl:162)."""
+
+    method = _lib.DEG_RODAS5P
+    order = 5
 
 
 def alg_order(alg: GPUODEAlgorithm) -> int:

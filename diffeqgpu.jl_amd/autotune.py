@@ -28,7 +28,7 @@ import math
 import os
 
 from . import _lib
-from .algorithms import GPURosenbrock23, HIPDevice
+from .algorithms import GPUODEImplicitAlgorithm, HIPDevice
 
 CACHE_PATH = os.path.join(_lib.CACHE_DIR, "autotune.json")
 SHIPPED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_mi355x.json")
@@ -44,7 +44,7 @@ def blocks_per_cu_candidates(block: int):
 
 
 def _key(probs, alg, adaptive, nsave_hint) -> str:
-    src, _ = probs.f.hip_source(probs.n, probs.np, isinstance(alg, GPURosenbrock23))
+    src, _ = probs.f.hip_source(probs.n, probs.np, isinstance(alg, GPUODEImplicitAlgorithm))
     h = hashlib.sha1(src.encode()).hexdigest()[:16]
     base = dataclasses.replace(probs.dev, block_size=None, refill=0, chunk=0, waves=0, blocks_per_cu=0)
     nb = int(round(math.log2(max(1, probs.N))))

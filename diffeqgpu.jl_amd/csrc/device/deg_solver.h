@@ -11,7 +11,7 @@
 //
 // Configuration macros supplied by the host before this file:
 //   DEG_N, DEG_NP                state and parameter counts
-//   DEG_METHOD                   0 Tsit5, 1 Vern7, 2 Rosenbrock23, 3 Vern9
+//   DEG_METHOD                   0 Tsit5, 1 Vern7, 2 Rosenbrock23, 3 Vern9, 4 Rodas4, 5 Rodas5P
 //   DEG_ADAPTIVE                 0 fixed step, 1 adaptive (PI controller)
 //   DEG_DETPOW                   controller powers via det_pow (bitwise-reproducible on CPU)
 //   DEG_FASTPOW                  controller powers via hardware log2/exp2
@@ -24,7 +24,7 @@
 //                                division in the dense-output abscissa)
 //   DEG_KERNEL                   kernel symbol name
 //   DEG_BLOCK_MAX                launch-bounds block size
-// and the functions deg_rhs(u, p, t, du) (+ deg_jac, deg_tgrad for Rosenbrock23).
+// and the functions deg_rhs(u, p, t, du) (+ deg_jac, deg_tgrad for Rosenbrock23, Rodas4, Rodas5P).
 #pragma once
 
 #define DEG_RET_SUCCESS 0
@@ -158,6 +158,10 @@ __device__ __forceinline__ float scaled_error2_fast(float (&e)[DEG_N], const flo
 #define DEG_ORDER 7
 #elif DEG_METHOD == 3
 #define DEG_ORDER 9
+#elif DEG_METHOD == 4
+#define DEG_ORDER 4
+#elif DEG_METHOD == 5
+#define DEG_ORDER 5
 #else
 #define DEG_ORDER 2
 #endif
@@ -208,9 +212,11 @@ __device__ __forceinline__ float deg_error(float dt, const float (&k)[DEG_NK + 1
 }
 // y0 + dt * (b1(th) k1 + ... + b7(th) k7) with b_i evaluated by fused Horner steps
 __device__ __forceinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                           const float (&y1)[DEG_N],
                                            const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
     (void)tprev;
     (void)p;
+    (void)y1;
     const float b1 = th * __builtin_fmaf(th, __builtin_fmaf(th, __builtin_fmaf(th, T5_r14, T5_r13), T5_r12), T5_r11);
     const float b2 = th * (th * __builtin_fmaf(th, __builtin_fmaf(th, T5_r24, T5_r23), T5_r22));
     const float b3 = th * (th * __builtin_fmaf(th, __builtin_fmaf(th, T5_r34, T5_r33), T5_r32));
@@ -284,7 +290,9 @@ __device__ __forceinline__ float v7q(float th, float r2, float r3, float r4, flo
 }
 #define V7Q(i) v7q(th, V7_r##i##2, V7_r##i##3, V7_r##i##4, V7_r##i##5, V7_r##i##6, V7_r##i##7)
 __device__ __noinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                           const float (&y1)[DEG_N],
                                         const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
+    (void)y1;
     float tmp[DEG_N], k11[DEG_N], k12[DEG_N], k13[DEG_N], k14[DEG_N], k15[DEG_N], k16[DEG_N];
     DEG_FOR(c) tmp[c] = y0[c] + dt * (V7_a1101 * k[1][c] + V7_a1104 * k[4][c] + V7_a1105 * k[5][c] + V7_a1106 * k[6][c] +
                                       V7_a1107 * k[7][c] + V7_a1108 * k[8][c] + V7_a1109 * k[9][c]);
@@ -393,7 +401,9 @@ __device__ __forceinline__ float V9H8(float th, float r1, float r2, float r3, fl
 // stages 1, 8..15, then y0 + dt * (sum of b_i(th) k_i). Scalar muladds (stage times, Horner
 // steps) are fused; vector sums are unfused left folds.
 __device__ __noinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                           const float (&y1)[DEG_N],
                                         const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
+    (void)y1;
     float tmp[DEG_N], x[10][DEG_N];  // extra stages 17..26
     DEG_FOR(c) tmp[c] = y0[c] + dt * (V9_a1701 * k[1][c] + V9_a1708 * k[8][c] + V9_a1709 * k[9][c] + V9_a1710 * k[10][c] + V9_a1711 * k[11][c] + V9_a1712 * k[12][c] + V9_a1713 * k[13][c] + V9_a1714 * k[14][c] + V9_a1715 * k[15][c]);
     deg_rhs(tmp, p, __builtin_fmaf(V9_c17, dt, tprev), x[0]);
@@ -441,16 +451,10 @@ __device__ __noinline__ void deg_interp(float th, float dt, float tprev, const f
 // ------------------------------------------------------------------------------------
 // Rosenbrock23 (gpu_rosenbrock23_perform_step.jl:33-60, :109-153) with an FP64 W = I - gamma J
 // ------------------------------------------------------------------------------------
-#if DEG_METHOD == 2
-#define DEG_NK 2
+#if DEG_METHOD == 2 || DEG_METHOD == 4 || DEG_METHOD == 5
 // LU with partial pivoting; row swaps are applied with compile-time indices so W stays
 // in VGPRs.
-__device__ __forceinline__ void w_factor(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N], float gam,
-                                         const float (&J)[DEG_N * DEG_N]) {
-    const double g = (double)gam;
-    DEG_FOR(r) {
-        _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (r == c ? 1.0 : 0.0) - g * (double)J[r * DEG_N + c];
-    }
+__device__ __forceinline__ void w_lu(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N]) {
     _Pragma("unroll") for (int kk = 0; kk < DEG_N; ++kk) {
         int pk = kk;
         double mx = __builtin_fabs(W[kk][kk]);
@@ -503,6 +507,19 @@ __device__ __forceinline__ void w_solve(const double (&W)[DEG_N][DEG_N], const i
     }
     DEG_FOR(i) x[i] = (float)y[i];
 }
+#endif
+
+#if DEG_METHOD == 2
+#define DEG_NK 2
+// W = I - gamma J
+__device__ __forceinline__ void w_factor(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N], float gam,
+                                         const float (&J)[DEG_N * DEG_N]) {
+    const double g = (double)gam;
+    DEG_FOR(r) {
+        _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (r == c ? 1.0 : 0.0) - g * (double)J[r * DEG_N + c];
+    }
+    w_lu(W, piv);
+}
 #define RB23_D (1.0f / (2.0f + __builtin_sqrtf(2.0f)))
 #define RB23_E32 (6.0f + __builtin_sqrtf(2.0f))
 // One Rosenbrock23 step from (uprev, t) with step dt; k[1], k[2] receive the slopes used by
@@ -545,14 +562,149 @@ __device__ __forceinline__ float deg_rb23(const float (&uprev)[DEG_N], const flo
 }
 // y0 + dt * (c1 k1 + c2 k2), c1 = th(1-th)/(1-2d), c2 = th(th-2d)/(1-2d)
 __device__ __forceinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                           const float (&y1)[DEG_N],
                                            const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
     (void)tprev;
     (void)p;
+    (void)y1;
     const float d = RB23_D;
     const float den = __builtin_fmaf(-2.0f, d, 1.0f);
     const float c1 = (th * (1.0f - th)) / den;
     const float c2 = (th * __builtin_fmaf(-2.0f, d, th)) / den;
     DEG_FOR(c) out[c] = y0[c] + dt * (c1 * k[1][c] + c2 * k[2][c]);
+}
+#endif
+
+// ------------------------------------------------------------------------------------
+// Rodas4 / Rodas5P (gpu_rodas4_perform_step.jl:1-277, gpu_rodas5P_perform_step.jl:1-351) with
+// an FP64 W = J - I / (dt gamma); dense output y0 (1-th) + th (y1 + (1-th)(k1 + th k2 [+ th k3]))
+// from the step's end state y1 (stiff/interpolants.jl:1-23)
+// ------------------------------------------------------------------------------------
+#if DEG_METHOD == 4 || DEG_METHOD == 5
+#if DEG_METHOD == 4
+#define DEG_NK 2
+#define RO(x) RO4_##x
+#else
+#define DEG_NK 3
+#define RO(x) RO5_##x
+#endif
+// One Rodas step from (uprev, t) with step dt. k[1..DEG_NK] receive the dense-output
+// combinations (h rows); returns EEst from the last stage when want_error is set.
+template <bool want_error>
+__device__ __forceinline__ float deg_rodas(const float (&uprev)[DEG_N], const float* p, float t, float dt,
+                                           float (&k)[DEG_NK + 1][DEG_N], float (&unew)[DEG_N], float abstol,
+                                           float reltol) {
+    float Jm[DEG_N * DEG_N], dT[DEG_N], du[DEG_N], lt[DEG_N], u[DEG_N];
+    deg_jac(uprev, p, t, Jm);
+    deg_tgrad(uprev, p, t, dT);
+    const float dtC21 = RO(C21) / dt, dtC31 = RO(C31) / dt, dtC32 = RO(C32) / dt;
+    const float dtC41 = RO(C41) / dt, dtC42 = RO(C42) / dt, dtC43 = RO(C43) / dt;
+    const float dtC51 = RO(C51) / dt, dtC52 = RO(C52) / dt, dtC53 = RO(C53) / dt, dtC54 = RO(C54) / dt;
+    const float dtC61 = RO(C61) / dt, dtC62 = RO(C62) / dt, dtC63 = RO(C63) / dt, dtC64 = RO(C64) / dt;
+    const float dtC65 = RO(C65) / dt;
+    const float dtd1 = dt * RO(d1), dtd2 = dt * RO(d2), dtd3 = dt * RO(d3), dtd4 = dt * RO(d4);
+    const float dtgamma = dt * RO(gamma);
+    double W[DEG_N][DEG_N];
+    int piv[DEG_N];
+    {
+        const double ig = (double)(1.0f / dtgamma);
+        DEG_FOR(r) {
+            _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (double)Jm[r * DEG_N + c] - (r == c ? ig : 0.0);
+        }
+        w_lu(W, piv);
+    }
+    float K1[DEG_N], K2[DEG_N], K3[DEG_N], K4[DEG_N], K5[DEG_N], K6[DEG_N];
+    deg_rhs(uprev, p, t, du);
+    // stage 1
+    DEG_FOR(c) lt[c] = -(du[c] + dtd1 * dT[c]);
+    w_solve(W, piv, lt, K1);
+    DEG_FOR(c) u[c] = uprev[c] + RO(a21) * K1[c];
+    deg_rhs(u, p, t + RO(c2) * dt, du);
+    // stage 2
+    DEG_FOR(c) lt[c] = -((du[c] + dtd2 * dT[c]) + dtC21 * K1[c]);
+    w_solve(W, piv, lt, K2);
+    DEG_FOR(c) u[c] = (uprev[c] + RO(a31) * K1[c]) + RO(a32) * K2[c];
+    deg_rhs(u, p, t + RO(c3) * dt, du);
+    // stage 3
+    DEG_FOR(c) lt[c] = -((du[c] + dtd3 * dT[c]) + (dtC31 * K1[c] + dtC32 * K2[c]));
+    w_solve(W, piv, lt, K3);
+    DEG_FOR(c) u[c] = ((uprev[c] + RO(a41) * K1[c]) + RO(a42) * K2[c]) + RO(a43) * K3[c];
+    deg_rhs(u, p, t + RO(c4) * dt, du);
+    // stage 4
+    DEG_FOR(c) lt[c] = -((du[c] + dtd4 * dT[c]) + ((dtC41 * K1[c] + dtC42 * K2[c]) + dtC43 * K3[c]));
+    w_solve(W, piv, lt, K4);
+    DEG_FOR(c) u[c] = (((uprev[c] + RO(a51) * K1[c]) + RO(a52) * K2[c]) + RO(a53) * K3[c]) + RO(a54) * K4[c];
+#if DEG_METHOD == 4
+    deg_rhs(u, p, t + dt, du);
+    // stage 5 (its C terms in the reference's order 2, 4, 1, 3)
+    DEG_FOR(c) lt[c] = -(du[c] + (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]));
+    w_solve(W, piv, lt, K5);
+    DEG_FOR(c) u[c] = u[c] + K5[c];
+    deg_rhs(u, p, t + dt, du);
+    // stage 6 (order 1, 2, 5, 4, 3)
+    DEG_FOR(c) lt[c] = -(du[c] + ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC65 * K5[c]) + dtC64 * K4[c]) + dtC63 * K3[c]));
+    w_solve(W, piv, lt, K6);
+    DEG_FOR(c) unew[c] = u[c] + K6[c];
+    DEG_FOR(c) {
+        k[1][c] = (((RO(h21) * K1[c] + RO(h22) * K2[c]) + RO(h23) * K3[c]) + RO(h24) * K4[c]) + RO(h25) * K5[c];
+        k[2][c] = (((RO(h31) * K1[c] + RO(h32) * K2[c]) + RO(h33) * K3[c]) + RO(h34) * K4[c]) + RO(h35) * K5[c];
+    }
+    float* const Kerr = K6;
+#else
+    const float dtd5 = dt * RO(d5);
+    const float dtC71 = RO(C71) / dt, dtC72 = RO(C72) / dt, dtC73 = RO(C73) / dt, dtC74 = RO(C74) / dt;
+    const float dtC75 = RO(C75) / dt, dtC76 = RO(C76) / dt;
+    const float dtC81 = RO(C81) / dt, dtC82 = RO(C82) / dt, dtC83 = RO(C83) / dt, dtC84 = RO(C84) / dt;
+    const float dtC85 = RO(C85) / dt, dtC86 = RO(C86) / dt, dtC87 = RO(C87) / dt;
+    float K7[DEG_N], K8[DEG_N];
+    deg_rhs(u, p, t + RO(c5) * dt, du);
+    // stage 5 (its C terms in the reference's order 2, 4, 1, 3)
+    DEG_FOR(c) lt[c] = -((du[c] + dtd5 * dT[c]) + (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]));
+    w_solve(W, piv, lt, K5);
+    DEG_FOR(c) u[c] = ((((uprev[c] + RO(a61) * K1[c]) + RO(a62) * K2[c]) + RO(a63) * K3[c]) + RO(a64) * K4[c]) + RO(a65) * K5[c];
+    deg_rhs(u, p, t + dt, du);
+    // stage 6
+    DEG_FOR(c) lt[c] = -(du[c] + ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC63 * K3[c]) + dtC64 * K4[c]) + dtC65 * K5[c]));
+    w_solve(W, piv, lt, K6);
+    DEG_FOR(c) u[c] = u[c] + K6[c];
+    deg_rhs(u, p, t + dt, du);
+    // stage 7
+    DEG_FOR(c) lt[c] = -(du[c] + (((((dtC71 * K1[c] + dtC72 * K2[c]) + dtC73 * K3[c]) + dtC74 * K4[c]) + dtC75 * K5[c]) + dtC76 * K6[c]));
+    w_solve(W, piv, lt, K7);
+    DEG_FOR(c) u[c] = u[c] + K7[c];
+    deg_rhs(u, p, t + dt, du);
+    // stage 8
+    DEG_FOR(c) lt[c] = -(du[c] + ((((((dtC81 * K1[c] + dtC82 * K2[c]) + dtC83 * K3[c]) + dtC84 * K4[c]) + dtC85 * K5[c]) + dtC86 * K6[c]) + dtC87 * K7[c]));
+    w_solve(W, piv, lt, K8);
+    DEG_FOR(c) unew[c] = u[c] + K8[c];
+    DEG_FOR(c) {
+        k[1][c] = ((((((RO(h21) * K1[c] + RO(h22) * K2[c]) + RO(h23) * K3[c]) + RO(h24) * K4[c]) + RO(h25) * K5[c]) + RO(h26) * K6[c]) + RO(h27) * K7[c]) + RO(h28) * K8[c];
+        k[2][c] = ((((((RO(h31) * K1[c] + RO(h32) * K2[c]) + RO(h33) * K3[c]) + RO(h34) * K4[c]) + RO(h35) * K5[c]) + RO(h36) * K6[c]) + RO(h37) * K7[c]) + RO(h38) * K8[c];
+        k[3][c] = ((((((RO(h41) * K1[c] + RO(h42) * K2[c]) + RO(h43) * K3[c]) + RO(h44) * K4[c]) + RO(h45) * K5[c]) + RO(h46) * K6[c]) + RO(h47) * K7[c]) + RO(h48) * K8[c];
+    }
+    float* const Kerr = K8;
+#endif
+    if (!want_error) return 0.0f;
+    float e[DEG_N];
+    DEG_FOR(c) e[c] = Kerr[c];
+#if DEG_FASTPOW
+    return scaled_error2_fast(e, uprev, unew, abstol, reltol);
+#else
+    return scaled_error(e, uprev, unew, abstol, reltol);
+#endif
+}
+__device__ __forceinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                           const float (&y1)[DEG_N],
+                                           const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
+    (void)dt;
+    (void)tprev;
+    (void)p;
+    const float th1 = 1.0f - th;
+#if DEG_METHOD == 4
+    DEG_FOR(c) out[c] = th1 * y0[c] + th * (y1[c] + th1 * (k[1][c] + th * k[2][c]));
+#else
+    DEG_FOR(c) out[c] = th1 * y0[c] + th * (y1[c] + th1 * (k[1][c] + th * (k[2][c] + th * k[3][c])));
+#endif
 }
 #endif
 
@@ -1083,7 +1235,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
 #else
                 const float th = (s_r - stp) * __builtin_amdgcn_rcpf(sdt);
 #endif
-                deg_interp(th, sdt, stp, p, su, sk, out);
+                deg_interp(th, sdt, stp, p, su, su, sk, out);
                 if (r < a.nsave) {
                     DEG_FOR(c) a.us[((long long)r * DEG_N + c) * S + i] = out[c];
                     a.ts[(long long)r * S + i] = s_r;
@@ -1205,7 +1357,9 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                     t = a.tstops[tsi];
                     tc = 0.0f;
                     ++tsi;
-                    deg_interp((t - tprev) / dtacc, dtacc, tprev, p, uprev, k, u);
+                    float ut[DEG_N];
+                    deg_interp((t - tprev) / dtacc, dtacc, tprev, p, uprev, u, k, ut);
+                    DEG_FOR(c) u[c] = ut[c];
                 }
 #endif
                 else {
@@ -1263,8 +1417,10 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                 deg_rhs(u, p, t, k[1]);
                 deg_attempt(u, p, t, dt, k, unew);
                 EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
-#else
+#elif DEG_METHOD == 2
                 EEst = deg_rb23<true>(u, p, t, dt, k, unew, a.abstol, a.reltol);
+#else
+                EEst = deg_rodas<true>(u, p, t, dt, k, unew, a.abstol, a.reltol);
 #endif
                 attempted = true;
 #if DEG_LEARNED || DEG_STEPLOG
@@ -1333,8 +1489,10 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
 #elif DEG_METHOD == 1 || DEG_METHOD == 3
                 deg_rhs(uprev, p, told, k[1]);
                 deg_attempt(uprev, p, told, dts, k, unew);
-#else
+#elif DEG_METHOD == 2
                 deg_rb23<false>(uprev, p, told, dts, k, unew, 0.0f, 0.0f);
+#else
+                deg_rodas<false>(uprev, p, told, dts, k, unew, 0.0f, 0.0f);
 #endif
                 DEG_FOR(c) u[c] = unew[c];
                 ++nacc;
@@ -1370,7 +1528,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
 #else
                         const float th = (tnext - tprev) * __builtin_amdgcn_rcpf(dtacc);
 #endif
-                        deg_interp(th, dtacc, tprev, p, uprev, k, out);
+                        deg_interp(th, dtacc, tprev, p, uprev, u, k, out);
                         if (cur < a.nsave) {
                             DEG_FOR(c) us_w[(long long)c * S] = out[c];
                             *ts_w = tnext;
@@ -1410,7 +1568,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
 #else
                         const float th = (tnext - tprev) * __builtin_amdgcn_rcpf(dtacc);
 #endif
-                        deg_interp(th, dtacc, tprev, p, uprev, k, out);
+                        deg_interp(th, dtacc, tprev, p, uprev, u, k, out);
                         if (cur < a.nsave) {
                             DEG_FOR(c) us_w[(long long)c * S] = out[c];
                             *ts_w = tnext;
@@ -1452,7 +1610,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                             if (s == t) return deg_ccb_cond(j, u, p, s);
                             if (s == tprev) return deg_ccb_cond(j, uprev, p, s);
                             float us_[DEG_N];
-                            deg_interp((s - tprev) / dtacc, dtacc, tprev, p, uprev, k, us_);
+                            deg_interp((s - tprev) / dtacc, dtacc, tprev, p, uprev, u, k, us_);
                             return deg_ccb_cond(j, us_, p, s);
                         };
                         // find_callback_time (integrator_utils.jl:386-446); last_event_error is 0
@@ -1525,7 +1683,9 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                         } else {
                             if (tmin != t) {
                                 // _change_t_via_interpolation! (integrator_utils.jl:174-201)
-                                deg_interp((tmin - tprev) / dtacc, dtacc, tprev, p, uprev, k, u);
+                                float ut[DEG_N];
+                                deg_interp((tmin - tprev) / dtacc, dtacc, tprev, p, uprev, u, k, ut);
+                                DEG_FOR(c) u[c] = ut[c];
                                 step_idx -= (int)__builtin_rintf((t - tmin) / dtacc);
                                 t = tmin;
                                 tc = 0.0f;
@@ -1588,7 +1748,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                     // k[1] holds k7 after the FSAL shift for Tsit5; undo it for the dense output
                     float out[DEG_N];
                     const float th = (tf - tprev) / dtacc;
-                    deg_interp(th, dtacc, tprev, p, uprev, k, out);
+                    deg_interp(th, dtacc, tprev, p, uprev, u, k, out);
                     const long long r = a.nsave - 1;
                     DEG_FOR(c) a.us[(r * DEG_N + c) * S + i] = out[c];
                     a.ts[r * S + i] = tf;

@@ -89,7 +89,7 @@ def solve(ensembleprob: EnsembleProblem, alg: GPUODEAlgorithm, ensemblealg: Ense
     if not isinstance(ensemblealg, EnsembleGPUKernel):
         raise TypeError("only EnsembleGPUKernel is provided")
     if not isinstance(alg, GPUODEAlgorithm):
-        raise TypeError("alg must be GPUTsit5(), GPUVern7(), GPUVern9() or GPURosenbrock23()")
+        raise TypeError("alg must be GPUTsit5(), GPUVern7(), GPUVern9(), GPURosenbrock23(), GPURodas4() or GPURodas5P()")
     if trajectories < 1:
         raise ValueError("trajectories must be >= 1")
     batch_size = trajectories if batch_size is None else int(batch_size)

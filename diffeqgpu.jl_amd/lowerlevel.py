@@ -17,7 +17,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from .algorithms import GPUODEAlgorithm, GPURosenbrock23, HIPDevice
+from .algorithms import GPUODEAlgorithm, GPUODEImplicitAlgorithm, HIPDevice
 from .callbacks import emit_callbacks, gpu_callback_set
 from .problems import HIPProbVector, ODEProblem, get_backend
 from .ranges import JlRange, colon, linrange
@@ -147,7 +147,7 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
         if flags & (_lib.F_LEARNED | _lib.F_STEPLOG | _lib.F_COMPACT | _lib.F_BATCH_SAVES):
             raise ValueError("tstops and callbacks do not combine with the learned proposer, step logs, "
                              "tail compaction or batched saves")
-    with_jac = isinstance(alg, GPURosenbrock23)
+    with_jac = isinstance(alg, GPUODEImplicitAlgorithm)
     src, _flops = probs.f.hip_source(probs.n, probs.np, with_jac)
     if callback is not None:
         src = src + emit_callbacks(callback, probs.n, probs.np)

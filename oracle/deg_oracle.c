@@ -11,6 +11,7 @@
  *   Vern7 fixed / adaptive step             perform_step/gpu_vern7_perform_step.jl:1-74 / :78-210
  *   Rosenbrock23 fixed / adaptive step      perform_step/gpu_rosenbrock23_perform_step.jl:1-70 / :74-201
  *   Vern9 fixed / adaptive step             perform_step/gpu_vern9_perform_step.jl:1-124 / :128-319
+ *   Rodas4 / Rodas5P fixed / adaptive step  perform_step/gpu_rodas4_perform_step.jl, gpu_rodas5P_perform_step.jl
  *   PI controller constants                 integrators/integrator_utils.jl:1-11, alg_utils.jl:5-8
  *   saveat / save_everystep writes          integrators/integrator_utils.jl:13-50
  *   dense output                            integrators/nonstiff/interpolants.jl:28-190 (Vern7), :192-370
@@ -33,8 +34,8 @@
  *     with DtLessThanMin (the reference repeats such a step forever); when the next dt is
  *     larger the lane goes on with t unchanged, as the reference does. A fixed step that
  *     cannot advance t ends the lane with DtLessThanMin;
- *   - Rosenbrock23 forms W = I - gamma J and solves it in FP64 (LU with partial
- *     pivoting) instead of Float32 Cramer's rule (linalg/linsolve.jl:31-57);
+ *   - Rosenbrock23 forms W = I - gamma J, Rodas4/Rodas5P W = J - I / (dt gamma), and they solve
+ *     it in FP64 (LU with partial pivoting) instead of Float32 Cramer's rule (linalg/linsolve.jl:31-57);
  *   - optional compensated (Kahan) accumulation of t (flag DEG_O_KAHAN).
  * Dense-output polynomials use fused multiply-add Horner steps (the reference's
  * @muladd/@evalpoly), vector combinations are left folds without fusion.
@@ -57,7 +58,7 @@
 #define OMAXN 16
 #define OMAXS 17
 
-enum { OM_TSIT5 = 0, OM_VERN7 = 1, OM_RB23 = 2, OM_VERN9 = 3 };
+enum { OM_TSIT5 = 0, OM_VERN7 = 1, OM_RB23 = 2, OM_VERN9 = 3, OM_RODAS4 = 4, OM_RODAS5P = 5 };
 enum { DEG_O_KAHAN = 1, DEG_O_DETPOW = 2 };
 /* per-trajectory return codes (same values as the C ABI of the HIP library) */
 enum { ORET_SUCCESS = 0, ORET_DTMIN = 1, ORET_MAXITERS = 2, ORET_NONFINITE = 3, ORET_TERMINATED = 4,
@@ -406,12 +407,8 @@ typedef struct {
     int perm[OMAXN];
 } olu;
 
-static void w_factor(olu *F, int n, float gam, const float *J) {
-    double g = (double)gam;
-    for (int r = 0; r < n; ++r) {
-        for (int c = 0; c < n; ++c) F->lu[r][c] = (r == c ? 1.0 : 0.0) - g * (double)J[r * n + c];
-        F->perm[r] = r;
-    }
+static void w_lu(olu *F, int n) {
+    for (int r = 0; r < n; ++r) F->perm[r] = r;
     for (int k = 0; k < n; ++k) {
         int piv = k;
         double mx = fabs(F->lu[k][k]);
@@ -440,6 +437,21 @@ static void w_factor(olu *F, int n, float gam, const float *J) {
     }
 }
 
+/* Rosenbrock23: W = I - gamma J */
+static void w_factor(olu *F, int n, float gam, const float *J) {
+    double g = (double)gam;
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) F->lu[r][c] = (r == c ? 1.0 : 0.0) - g * (double)J[r * n + c];
+    w_lu(F, n);
+}
+
+/* Rodas: W = J - I / (dt gamma), the reciprocal taken in Float32 as inv(dtgamma) */
+static void w_factor_rodas(olu *F, int n, float invg, const float *J) {
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) F->lu[r][c] = (double)J[r * n + c] - (r == c ? (double)invg : 0.0);
+    w_lu(F, n);
+}
+
 static void w_solve(const olu *F, int n, const float *b, float *x) {
     double y[OMAXN];
     for (int i = 0; i < n; ++i) {
@@ -453,6 +465,144 @@ static void w_solve(const olu *F, int n, const float *b, float *x) {
         y[i] = v / F->lu[i][i];
     }
     for (int i = 0; i < n; ++i) x[i] = (float)y[i];
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Rodas4 / Rodas5P (gpu_rodas4_perform_step.jl:1-277, gpu_rodas5P_perform_step.jl:1-351)    */
+/* ------------------------------------------------------------------------------------ */
+/* One step from (uprev, t) with step dt: unew, the dense-output combinations k[1..2] (Rodas4)
+ * or k[1..3] (Rodas5P), and the last stage (the error estimate) in kerr. Vector sums keep the
+ * reference's association, including its term order in stages 5 and 6. */
+#define R4(x) ((float)ORO4_##x)
+#define R5(x) ((float)ORO5_##x)
+static void rodas4_step(const osys *S, const float *p, const float *uprev, float t, float dt, float k[][OMAXN],
+                        float *unew, float *kerr) {
+    const int n = S->n;
+    float Jm[OMAXN * OMAXN], dT[OMAXN], du[OMAXN], lt[OMAXN] = {0}, u[OMAXN];
+    float K1[OMAXN], K2[OMAXN], K3[OMAXN], K4[OMAXN], K5[OMAXN], K6[OMAXN];
+    S->jac(uprev, p, t, Jm);
+    S->tgrad(uprev, p, t, dT);
+    const float dtC21 = R4(C21) / dt, dtC31 = R4(C31) / dt, dtC32 = R4(C32) / dt;
+    const float dtC41 = R4(C41) / dt, dtC42 = R4(C42) / dt, dtC43 = R4(C43) / dt;
+    const float dtC51 = R4(C51) / dt, dtC52 = R4(C52) / dt, dtC53 = R4(C53) / dt, dtC54 = R4(C54) / dt;
+    const float dtC61 = R4(C61) / dt, dtC62 = R4(C62) / dt, dtC63 = R4(C63) / dt, dtC64 = R4(C64) / dt;
+    const float dtC65 = R4(C65) / dt;
+    const float dtd1 = dt * R4(d1), dtd2 = dt * R4(d2), dtd3 = dt * R4(d3), dtd4 = dt * R4(d4);
+    const float dtgamma = dt * R4(gamma);
+    olu F;
+    w_factor_rodas(&F, n, 1.0f / dtgamma, Jm);
+    S->f(uprev, p, t, du);
+    for (int c = 0; c < n; ++c) lt[c] = -(du[c] + dtd1 * dT[c]);
+    w_solve(&F, n, lt, K1);
+    for (int c = 0; c < n; ++c) u[c] = uprev[c] + R4(a21) * K1[c];
+    S->f(u, p, t + R4(c2) * dt, du);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd2 * dT[c]) + dtC21 * K1[c]);
+    w_solve(&F, n, lt, K2);
+    for (int c = 0; c < n; ++c) u[c] = (uprev[c] + R4(a31) * K1[c]) + R4(a32) * K2[c];
+    S->f(u, p, t + R4(c3) * dt, du);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd3 * dT[c]) + (dtC31 * K1[c] + dtC32 * K2[c]));
+    w_solve(&F, n, lt, K3);
+    for (int c = 0; c < n; ++c) u[c] = ((uprev[c] + R4(a41) * K1[c]) + R4(a42) * K2[c]) + R4(a43) * K3[c];
+    S->f(u, p, t + R4(c4) * dt, du);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd4 * dT[c]) + ((dtC41 * K1[c] + dtC42 * K2[c]) + dtC43 * K3[c]));
+    w_solve(&F, n, lt, K4);
+    for (int c = 0; c < n; ++c)
+        u[c] = (((uprev[c] + R4(a51) * K1[c]) + R4(a52) * K2[c]) + R4(a53) * K3[c]) + R4(a54) * K4[c];
+    S->f(u, p, t + dt, du);
+    for (int c = 0; c < n; ++c) lt[c] = -(du[c] + (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]));
+    w_solve(&F, n, lt, K5);
+    for (int c = 0; c < n; ++c) u[c] = u[c] + K5[c];
+    S->f(u, p, t + dt, du);
+    for (int c = 0; c < n; ++c)
+        lt[c] = -(du[c] + ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC65 * K5[c]) + dtC64 * K4[c]) + dtC63 * K3[c]));
+    w_solve(&F, n, lt, K6);
+    for (int c = 0; c < n; ++c) {
+        unew[c] = u[c] + K6[c];
+        kerr[c] = K6[c];
+        k[1][c] = (((R4(h21) * K1[c] + R4(h22) * K2[c]) + R4(h23) * K3[c]) + R4(h24) * K4[c]) + R4(h25) * K5[c];
+        k[2][c] = (((R4(h31) * K1[c] + R4(h32) * K2[c]) + R4(h33) * K3[c]) + R4(h34) * K4[c]) + R4(h35) * K5[c];
+    }
+}
+
+static void rodas5p_step(const osys *S, const float *p, const float *uprev, float t, float dt, float k[][OMAXN],
+                         float *unew, float *kerr) {
+    const int n = S->n;
+    float Jm[OMAXN * OMAXN], dT[OMAXN], du[OMAXN], lt[OMAXN] = {0}, u[OMAXN];
+    float K1[OMAXN], K2[OMAXN], K3[OMAXN], K4[OMAXN], K5[OMAXN], K6[OMAXN], K7[OMAXN], K8[OMAXN];
+    S->jac(uprev, p, t, Jm);
+    S->tgrad(uprev, p, t, dT);
+    const float dtC21 = R5(C21) / dt, dtC31 = R5(C31) / dt, dtC32 = R5(C32) / dt;
+    const float dtC41 = R5(C41) / dt, dtC42 = R5(C42) / dt, dtC43 = R5(C43) / dt;
+    const float dtC51 = R5(C51) / dt, dtC52 = R5(C52) / dt, dtC53 = R5(C53) / dt, dtC54 = R5(C54) / dt;
+    const float dtC61 = R5(C61) / dt, dtC62 = R5(C62) / dt, dtC63 = R5(C63) / dt, dtC64 = R5(C64) / dt;
+    const float dtC65 = R5(C65) / dt;
+    const float dtC71 = R5(C71) / dt, dtC72 = R5(C72) / dt, dtC73 = R5(C73) / dt, dtC74 = R5(C74) / dt;
+    const float dtC75 = R5(C75) / dt, dtC76 = R5(C76) / dt;
+    const float dtC81 = R5(C81) / dt, dtC82 = R5(C82) / dt, dtC83 = R5(C83) / dt, dtC84 = R5(C84) / dt;
+    const float dtC85 = R5(C85) / dt, dtC86 = R5(C86) / dt, dtC87 = R5(C87) / dt;
+    const float dtd1 = dt * R5(d1), dtd2 = dt * R5(d2), dtd3 = dt * R5(d3), dtd4 = dt * R5(d4), dtd5 = dt * R5(d5);
+    const float dtgamma = dt * R5(gamma);
+    olu F;
+    w_factor_rodas(&F, n, 1.0f / dtgamma, Jm);
+    S->f(uprev, p, t, du);
+    for (int c = 0; c < n; ++c) lt[c] = -(du[c] + dtd1 * dT[c]);
+    w_solve(&F, n, lt, K1);
+    for (int c = 0; c < n; ++c) u[c] = uprev[c] + R5(a21) * K1[c];
+    S->f(u, p, t + R5(c2) * dt, du);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd2 * dT[c]) + dtC21 * K1[c]);
+    w_solve(&F, n, lt, K2);
+    for (int c = 0; c < n; ++c) u[c] = (uprev[c] + R5(a31) * K1[c]) + R5(a32) * K2[c];
+    S->f(u, p, t + R5(c3) * dt, du);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd3 * dT[c]) + (dtC31 * K1[c] + dtC32 * K2[c]));
+    w_solve(&F, n, lt, K3);
+    for (int c = 0; c < n; ++c) u[c] = ((uprev[c] + R5(a41) * K1[c]) + R5(a42) * K2[c]) + R5(a43) * K3[c];
+    S->f(u, p, t + R5(c4) * dt, du);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd4 * dT[c]) + ((dtC41 * K1[c] + dtC42 * K2[c]) + dtC43 * K3[c]));
+    w_solve(&F, n, lt, K4);
+    for (int c = 0; c < n; ++c)
+        u[c] = (((uprev[c] + R5(a51) * K1[c]) + R5(a52) * K2[c]) + R5(a53) * K3[c]) + R5(a54) * K4[c];
+    S->f(u, p, t + R5(c5) * dt, du);
+    for (int c = 0; c < n; ++c)
+        lt[c] = -((du[c] + dtd5 * dT[c]) + (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]));
+    w_solve(&F, n, lt, K5);
+    for (int c = 0; c < n; ++c)
+        u[c] = ((((uprev[c] + R5(a61) * K1[c]) + R5(a62) * K2[c]) + R5(a63) * K3[c]) + R5(a64) * K4[c]) + R5(a65) * K5[c];
+    S->f(u, p, t + dt, du);
+    for (int c = 0; c < n; ++c)
+        lt[c] = -(du[c] + ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC63 * K3[c]) + dtC64 * K4[c]) + dtC65 * K5[c]));
+    w_solve(&F, n, lt, K6);
+    for (int c = 0; c < n; ++c) u[c] = u[c] + K6[c];
+    S->f(u, p, t + dt, du);
+    for (int c = 0; c < n; ++c)
+        lt[c] = -(du[c] + (((((dtC71 * K1[c] + dtC72 * K2[c]) + dtC73 * K3[c]) + dtC74 * K4[c]) + dtC75 * K5[c]) +
+                           dtC76 * K6[c]));
+    w_solve(&F, n, lt, K7);
+    for (int c = 0; c < n; ++c) u[c] = u[c] + K7[c];
+    S->f(u, p, t + dt, du);
+    for (int c = 0; c < n; ++c)
+        lt[c] = -(du[c] + ((((((dtC81 * K1[c] + dtC82 * K2[c]) + dtC83 * K3[c]) + dtC84 * K4[c]) + dtC85 * K5[c]) +
+                            dtC86 * K6[c]) + dtC87 * K7[c]));
+    w_solve(&F, n, lt, K8);
+    for (int c = 0; c < n; ++c) {
+        unew[c] = u[c] + K8[c];
+        kerr[c] = K8[c];
+        k[1][c] = ((((((R5(h21) * K1[c] + R5(h22) * K2[c]) + R5(h23) * K3[c]) + R5(h24) * K4[c]) + R5(h25) * K5[c]) +
+                    R5(h26) * K6[c]) + R5(h27) * K7[c]) + R5(h28) * K8[c];
+        k[2][c] = ((((((R5(h31) * K1[c] + R5(h32) * K2[c]) + R5(h33) * K3[c]) + R5(h34) * K4[c]) + R5(h35) * K5[c]) +
+                    R5(h36) * K6[c]) + R5(h37) * K7[c]) + R5(h38) * K8[c];
+        k[3][c] = ((((((R5(h41) * K1[c] + R5(h42) * K2[c]) + R5(h43) * K3[c]) + R5(h44) * K4[c]) + R5(h45) * K5[c]) +
+                    R5(h46) * K6[c]) + R5(h47) * K7[c]) + R5(h48) * K8[c];
+    }
+}
+
+/* Rodas dense output (stiff/interpolants.jl:1-23): (1-th) y0 + th (y1 + (1-th)(k1 + th k2 [+ th k3]))
+ * with y1 the integrator's current u */
+static void rodas_interp(int method, float th, int n, const float *y0, const float *y1, float k[][OMAXN], float *out) {
+    const float th1 = 1.0f - th;
+    for (int c = 0; c < n; ++c) {
+        if (method == OM_RODAS4) out[c] = th1 * y0[c] + th * (y1[c] + th1 * (k[1][c] + th * k[2][c]));
+        else out[c] = th1 * y0[c] + th * (y1[c] + th1 * (k[1][c] + th * (k[2][c] + th * k[3][c])));
+    }
 }
 
 /* ------------------------------------------------------------------------------------ */
@@ -558,7 +708,8 @@ static inline int no_progress(float *t, float tf) {
 static void interp_at(const ojob *J, const ost *s, float x, float *out) {
     const float th = (x - s->tprev) / s->dtstep;
     const float d = 1.0f / (2.0f + sqrtf(2.0f));
-    if (J->method == OM_TSIT5) tsit5_interp(th, s->dtstep, s->n, s->uprev, (float(*)[OMAXN])s->k, out);
+    if (J->method == OM_RODAS4 || J->method == OM_RODAS5P) rodas_interp(J->method, th, s->n, s->uprev, s->u, (float(*)[OMAXN])s->k, out);
+    else if (J->method == OM_TSIT5) tsit5_interp(th, s->dtstep, s->n, s->uprev, (float(*)[OMAXN])s->k, out);
     else if (J->method == OM_RB23) rb23_interp(th, s->dtstep, s->n, d, s->uprev, s->rk1, s->rk2, out);
     else if (J->method == OM_VERN9) vern9_interp(J->S, s->p, th, s->dtstep, s->tprev, s->n, s->uprev, (float(*)[OMAXN])s->k, out);
     else vern7_interp(J->S, s->p, th, s->dtstep, s->tprev, s->n, s->uprev, (float(*)[OMAXN])s->k, out);
@@ -836,7 +987,11 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                     break;
                 }
             }
-            if (J->method == OM_RB23) {
+            if (J->method == OM_RODAS4 || J->method == OM_RODAS5P) {
+                float kerr[OMAXN];
+                if (J->method == OM_RODAS4) rodas4_step(S, s->p, s->uprev, told, dts, k, s->u, kerr);
+                else rodas5p_step(S, s->p, s->uprev, told, dts, k, s->u, kerr);
+            } else if (J->method == OM_RB23) {
                 /* gpu_rosenbrock23_perform_step.jl:1-70 */
                 float Jm[OMAXN * OMAXN], dT[OMAXN], F0[OMAXN], rhs[OMAXN], tmp[OMAXN], F1[OMAXN], x[OMAXN];
                 float gam = dts * d;
@@ -881,7 +1036,8 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
         }
     } else {
         /* ---------------- adaptive: kernels.jl:128-131 -------------------------------- */
-        const int order = (J->method == OM_TSIT5) ? 5 : (J->method == OM_VERN7 ? 7 : (J->method == OM_VERN9 ? 9 : 2));
+        static const int orders[6] = {5, 7, 2, 9, 4, 5};
+        const int order = orders[J->method];
         /* integrator_utils.jl:1-11 */
         const float beta1 = (float)(7.0 / (10.0 * order));
         const float beta2 = (float)(2.0 / (5.0 * order));
@@ -906,7 +1062,17 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                 if (attempts >= J->max_attempts) { s->retcode = ORET_MAXITERS; stop = 1; break; }
                 ++attempts;
                 float q11 = 0.0f, q;
-                if (J->method == OM_RB23) {
+                if (J->method == OM_RODAS4 || J->method == OM_RODAS5P) {
+                    /* gpu_rodas4_perform_step.jl:158-238, gpu_rodas5P_perform_step.jl:186-296 */
+                    float e[OMAXN];
+                    if (J->method == OM_RODAS4) rodas4_step(S, s->p, s->uprev, t, dt, k, unew, e);
+                    else rodas5p_step(S, s->p, s->uprev, t, dt, k, unew, e);
+                    for (int c = 0; c < n; ++c) {
+                        float sc = J->abstol + jl_max(fabsf(s->uprev[c]), fabsf(unew[c])) * J->reltol;
+                        e[c] = e[c] / sc;
+                    }
+                    EEst = rms_norm(e, n);
+                } else if (J->method == OM_RB23) {
                     /* gpu_rosenbrock23_perform_step.jl:106-153 */
                     float Jm[OMAXN * OMAXN], dT[OMAXN], F0[OMAXN], F1[OMAXN], F2[OMAXN], rhs[OMAXN], tmp[OMAXN], x[OMAXN];
                     float gam = dt * d;
@@ -1025,7 +1191,7 @@ int deg_oracle_solve_cb(int system, int method, int adaptive, int64_t N,
                         float *ts, float *us, int32_t *retcode, int32_t *naccept, int32_t *nreject,
                         int nthreads, const float *tstops, int ntstops, const ocb *cbs, int ncb) {
     if (system < 0 || system >= OSYS_COUNT) return -1;
-    if (method < OM_TSIT5 || method > OM_VERN9) return -1;
+    if (method < OM_TSIT5 || method > OM_RODAS5P) return -1;
     if (ntstops < 0 || ncb < 0 || (ntstops > 0 && !tstops) || (ncb > 0 && !cbs)) return -1;
     init_tabs();
     const osys *S = &OSYSTEMS[system];

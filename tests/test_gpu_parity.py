@@ -18,7 +18,8 @@ import pyoracle
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
-METHODS = {"tsit5": "GPUTsit5", "vern7": "GPUVern7", "rb23": "GPURosenbrock23", "vern9": "GPUVern9"}
+METHODS = {"tsit5": "GPUTsit5", "vern7": "GPUVern7", "rb23": "GPURosenbrock23", "vern9": "GPUVern9",
+           "rodas4": "GPURodas4", "rodas5p": "GPURodas5P"}
 
 
 @pytest.fixture(scope="module")
@@ -86,6 +87,8 @@ FIXED_CASES = [
     ("vdp", "tsit5", (0.0, 5.0), 0.05), ("vdp", "vern7", (0.0, 5.0), 0.05), ("vdp", "rb23", (0.0, 5.0), 0.01),
     ("tdep", "tsit5", (0.0, 2.0), 0.015625), ("rot", "vern7", (0.0, 8.0), 0.5), ("robertson", "rb23", (0.0, 1.0), 0.001),
     ("lorenz", "vern9", (0.0, 10.0), 0.1), ("vdp", "vern9", (0.0, 5.0), 0.05), ("tdep", "vern9", (0.0, 2.0), 0.0625),
+    ("lorenz", "rodas4", (0.0, 2.0), 0.01), ("vdp", "rodas4", (0.0, 5.0), 0.01), ("tdep", "rodas4", (0.0, 2.0), 0.0625),
+    ("robertson", "rodas5p", (0.0, 1.0), 0.001), ("vdp", "rodas5p", (0.0, 5.0), 0.01), ("tdep", "rodas5p", (0.0, 2.0), 0.0625),
 ]
 
 
@@ -123,7 +126,9 @@ ADAPTIVE_CASES = [
     ("vdp", "vern7", (0.0, 10.0), 0.01, 1e-7, 1e-7), ("vdp", "rb23", (0.0, 10.0), 0.01, 1e-6, 1e-4),
     ("robertson", "rb23", (0.0, 1e5), 1e-6, 1e-8, 1e-4), ("quad", "rb23", (0.0, 10.0), 0.01, 1e-6, 1e-3),
     ("tdep", "tsit5", (0.0, 3.0), 0.1, 1e-7, 1e-7), ("lorenz", "vern9", (0.0, 10.0), 0.1, 1e-8, 1e-8),
-    ("vdp", "vern9", (0.0, 10.0), 0.01, 1e-7, 1e-7),
+    ("vdp", "vern9", (0.0, 10.0), 0.01, 1e-7, 1e-7), ("robertson", "rodas4", (0.0, 1e5), 1e-6, 1e-8, 1e-4),
+    ("vdp", "rodas4", (0.0, 10.0), 0.01, 1e-6, 1e-4), ("robertson", "rodas5p", (0.0, 1e5), 1e-6, 1e-8, 1e-4),
+    ("lorenz", "rodas5p", (0.0, 3.0), 0.01, 1e-5, 1e-5),
 ]
 
 

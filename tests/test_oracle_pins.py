@@ -142,6 +142,91 @@ def test_vern9_dense_output():
 
 
 # ---------------------------------------------------------------------------------------
+# Rodas4 / Rodas5P
+# ---------------------------------------------------------------------------------------
+def _rodas_tab(tag):
+    import os
+    import re
+
+    src = open(os.path.join(os.path.dirname(pyoracle.__file__), "tableaus.h")).read()
+    return {m.group(1): float(m.group(2)) for m in re.finditer(rf"#define {tag}_(\w+) (\S+)", src)}
+
+
+def _rodas_step64(T, stages, f, jac, tgrad, u0, t, dt):
+    """One Rosenbrock step in FP64 in the generic form: W = J - I/(dt gamma); stage i solves
+    W K_i = -(f(U_i, t + c_i dt) + d_i dt f_t + sum_j C_ij / dt K_j) with U_i = uprev + sum_j a_ij K_j
+    while a_i exists, then U_i = U_(i-1) + K_(i-1) at t + dt; the solution is U_s + K_s."""
+    n = len(u0)
+    W = jac(u0, t) - np.eye(n) / (dt * T["gamma"])
+    dT = tgrad(u0, t)
+    K = []
+    U = u0
+    for i in range(1, stages + 1):
+        if i > 1:
+            if f"a{i}1" in T:
+                U = u0 + sum(T[f"a{i}{j}"] * K[j - 1] for j in range(1, i))
+            else:
+                U = U + K[-1]
+        ti = t + T.get(f"c{i}", 0.0 if i == 1 else 1.0) * dt
+        rhs = f(U, ti) + T.get(f"d{i}", 0.0) * dt * dT + sum(T.get(f"C{i}{j}", 0.0) / dt * K[j - 1] for j in range(1, i))
+        K.append(np.linalg.solve(W, -rhs))
+    return U + K[-1]
+
+
+@pytest.mark.parametrize("tag,stages,order", [("ORO4", 6, 4), ("ORO5", 8, 5)])
+def test_rodas_tableaus_convergence(tag, stages, order):
+    """The Rodas coefficients (a, C, gamma, c, d) give the method's order in FP64 on a rotation
+    and on the non-autonomous u' = p t u (exercises c_i and d_i); a perturbed C fails it."""
+    T = _rodas_tab(tag)
+    problems = [
+        (lambda u, t: np.array([-u[1], u[0]]), lambda u, t: np.array([[0.0, -1.0], [1.0, 0.0]]),
+         lambda u, t: np.zeros(2), np.array([1.0, 0.0]), 8.0, np.array([math.cos(8.0), math.sin(8.0)])),
+        (lambda u, t: 0.5 * t * u, lambda u, t: np.array([[0.5 * t]]), lambda u, t: 0.5 * u, np.array([1.0]), 2.0,
+         np.array([math.exp(0.25 * 4.0)])),
+    ]
+
+    def err(Tab, prob, dt):
+        f, jac, tg, u, tf, ex = prob
+        t = 0.0
+        for _ in range(int(round(tf / dt))):
+            u = _rodas_step64(Tab, stages, f, jac, tg, u, t, dt)
+            t += dt
+        return np.abs(u - ex).max()
+
+    for prob in problems:
+        e1, e2 = err(T, prob, 0.125), err(T, prob, 0.0625)
+        assert math.log2(e1 / e2) > order - 0.5, (e1, e2)
+    T2 = dict(T)
+    T2["C43"] *= 1.001
+    e1, e2 = err(T2, problems[1], 0.125), err(T2, problems[1], 0.0625)
+    assert math.log2(e1 / e2) < order - 0.5
+
+
+@pytest.mark.parametrize("method,tag,stages", [("rodas4", "ORO4", 6), ("rodas5p", "ORO5", 8)])
+def test_rodas_oracle_step_vs_fp64(method, tag, stages):
+    """The Float32 oracle's Rodas step agrees with the FP64 generic form to Float32 rounding
+    (u' = -u^2 from 1 and u' = p t u from 1 at t0 = 0.5)."""
+    T = _rodas_tab(tag)
+    for system, u0, p, t0, f, jac, tg in (
+            ("quad", 1.0, 1.0, 0.0, lambda u, t: -u * u, lambda u, t: np.array([[-2.0 * u[0]]]), lambda u, t: np.zeros(1)),
+            ("tdep", 1.0, 0.5, 0.5, lambda u, t: 0.5 * t * u, lambda u, t: np.array([[0.5 * t]]), lambda u, t: 0.5 * u)):
+        for dt in (0.25, 0.05):
+            want = _rodas_step64(T, stages, f, jac, tg, np.array([u0]), t0, dt)
+            r = pyoracle.solve(system, method, False, np.array([[u0]], F32), np.array([[p]], F32), (t0, t0 + dt), dt)
+            assert abs(float(r["us"][1, 0, 0]) - want[0]) < 2e-6, (system, dt)
+
+
+def test_rodas_dense_output_endpoints():
+    """(1-th) y0 + th (y1 + (1-th)(...)) is y0 at th = 0 and y1 at th = 1 exactly in real
+    arithmetic: saveat at a step end reproduces the every-step value."""
+    for method in ("rodas4", "rodas5p"):
+        sv = np.array([0.0, 0.05, 0.1], F32)
+        r = pyoracle.solve("lorenz", method, False, LZ_U0, LZ_P, (0.0, 0.1), 0.05, saveat=sv)
+        r2 = pyoracle.solve("lorenz", method, False, LZ_U0, LZ_P, (0.0, 0.1), 0.05, nsave=3, save_everystep=True)
+        np.testing.assert_allclose(r["us"][1:], r2["us"][1:], rtol=1e-6, atol=1e-6)
+
+
+# ---------------------------------------------------------------------------------------
 # deterministic pow of the controller
 # ---------------------------------------------------------------------------------------
 def test_det_pow_accuracy():
@@ -436,9 +521,10 @@ def _golden(name):
     return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name))
 
 
-def test_robertson_fixture_vs_oracle():
+@pytest.mark.parametrize("method", ["rb23", "rodas4", "rodas5p"])
+def test_robertson_fixture_vs_oracle(method):
     cfg, gold = _workload("robertson"), _golden("robertson_radau.npz")
-    r = pyoracle.solve("robertson", "rb23", True, cfg["u0"], cfg["p"], cfg["tspan"], cfg["dt"], abstol=1e-8,
+    r = pyoracle.solve("robertson", method, True, cfg["u0"], cfg["p"], cfg["tspan"], cfg["dt"], abstol=1e-8,
                        reltol=1e-4, saveat=np.asarray(cfg["saveat"], F32))
     assert np.all(r["retcode"] == 0)
     err = np.abs(r["us"].astype(np.float64) - gold["u"]).max(axis=(0, 1))
