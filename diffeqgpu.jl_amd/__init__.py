@@ -1,11 +1,13 @@
 """MI355X-native drop-in for DiffEqGPU's EnsembleGPUKernel path (GPUTsit5, GPUVern7, GPUVern9,
-GPURosenbrock23, GPURodas4, GPURodas5P), over the C ABI of libdeghip.so (include/deghip.h).
+GPURosenbrock23, GPURodas4, GPURodas5P, GPUKvaerno3, GPUKvaerno5), over the C ABI of libdeghip.so
+(include/deghip.h).
 
 Import under any module name (the directory name is not a Python identifier); the test
 suite, the benchmark and the build entry point load it as ``diffeqgpu_amd``.
 """
 from .algorithms import (EnsembleGPUKernel, EnsembleKernelAlgorithm, GPUODEAlgorithm, GPUODEImplicitAlgorithm,
-                         GPURodas4, GPURodas5P, GPURosenbrock23, GPUTsit5, GPUVern7, GPUVern9, HIPDevice, alg_order)
+                         GPUKvaerno3, GPUKvaerno5, GPURodas4, GPURodas5P, GPURosenbrock23, GPUTsit5, GPUVern7,
+                         GPUVern9, HIPDevice, alg_order)
 from .callbacks import (CallbackSet, ContinuousCallback, DiscreteCallback, GPUContinuousCallback, GPUDiscreteCallback,
                         LeftRootFind, NoRootFind, RightRootFind, terminate, u_modified)
 from .ensemble import EnsembleContext, EnsembleSolution, ODESolution, solve
@@ -20,7 +22,7 @@ from . import tracer as math  # sin, cos, exp, log, sqrt, tanh, fabs, pow, ifels
 
 __all__ = [
     "EnsembleGPUKernel", "EnsembleKernelAlgorithm", "GPUODEAlgorithm", "GPURosenbrock23", "GPUTsit5", "GPUVern7",
-    "GPUVern9", "GPUODEImplicitAlgorithm", "GPURodas4", "GPURodas5P",
+    "GPUVern9", "GPUODEImplicitAlgorithm", "GPURodas4", "GPURodas5P", "GPUKvaerno3", "GPUKvaerno5",
     "HIPDevice", "alg_order", "EnsembleContext", "EnsembleSolution", "ODESolution", "solve", "SolveResult", "order_by_span",
     "vectorized_asolve", "vectorized_solve", "EnsembleProblem", "HIPProbVector", "ImmutableODEProblem",
     "ODEFunction", "ODEProblem", "adapt", "get_backend", "make_prob_compatible", "remake", "colon", "linrange",

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(PKG_DIR, "_build", "libdeghip.so")
 CACHE_DIR = os.path.join(PKG_DIR, "_jitcache")
 
 DEG_OK = 0
-DEG_TSIT5, DEG_VERN7, DEG_RB23, DEG_VERN9, DEG_RODAS4, DEG_RODAS5P = 0, 1, 2, 3, 4, 5
+DEG_TSIT5, DEG_VERN7, DEG_RB23, DEG_VERN9, DEG_RODAS4, DEG_RODAS5P, DEG_KVAERNO3, DEG_KVAERNO5 = range(8)
 F_ADAPTIVE = 1 << 0
 F_PARITY = 1 << 1
 F_DETPOW = 1 << 2

@@ -79,6 +79,20 @@ class GPURodas5P(GPUODEImplicitAlgorithm):
     order = 5
 
 
+class GPUKvaerno3(GPUODEImplicitAlgorithm):
+    """Kvaerno3, ESDIRK order 3 with Newton-solved stages (gpukernel_algorithms.jl:187)."""
+
+    method = _lib.DEG_KVAERNO3
+    order = 3
+
+
+class GPUKvaerno5(GPUODEImplicitAlgorithm):
+    """Kvaerno5, ESDIRK order 5 with Newton-solved stages (gpukernel_algorithms.jl:212)."""
+
+    method = _lib.DEG_KVAERNO5
+    order = 5
+
+
 def alg_order(alg: GPUODEAlgorithm) -> int:
     """alg_utils.jl:1-13."""
     if not isinstance(alg, GPUODEAlgorithm) or alg.order == 0:

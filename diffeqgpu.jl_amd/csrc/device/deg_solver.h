@@ -11,7 +11,8 @@
 //
 // Configuration macros supplied by the host before this file:
 //   DEG_N, DEG_NP                state and parameter counts
-//   DEG_METHOD                   0 Tsit5, 1 Vern7, 2 Rosenbrock23, 3 Vern9, 4 Rodas4, 5 Rodas5P
+//   DEG_METHOD                   0 Tsit5, 1 Vern7, 2 Rosenbrock23, 3 Vern9, 4 Rodas4, 5 Rodas5P,
+//                                6 Kvaerno3, 7 Kvaerno5
 //   DEG_ADAPTIVE                 0 fixed step, 1 adaptive (PI controller)
 //   DEG_DETPOW                   controller powers via det_pow (bitwise-reproducible on CPU)
 //   DEG_FASTPOW                  controller powers via hardware log2/exp2
@@ -24,7 +25,8 @@
 //                                division in the dense-output abscissa)
 //   DEG_KERNEL                   kernel symbol name
 //   DEG_BLOCK_MAX                launch-bounds block size
-// and the functions deg_rhs(u, p, t, du) (+ deg_jac, deg_tgrad for Rosenbrock23, Rodas4, Rodas5P).
+// and the functions deg_rhs(u, p, t, du) (+ deg_jac, deg_tgrad for Rosenbrock23, Rodas4, Rodas5P;
+// deg_jac for Kvaerno3, Kvaerno5).
 #pragma once
 
 #define DEG_RET_SUCCESS 0
@@ -161,6 +163,10 @@ __device__ __forceinline__ float scaled_error2_fast(float (&e)[DEG_N], const flo
 #elif DEG_METHOD == 4
 #define DEG_ORDER 4
 #elif DEG_METHOD == 5
+#define DEG_ORDER 5
+#elif DEG_METHOD == 6
+#define DEG_ORDER 3
+#elif DEG_METHOD == 7
 #define DEG_ORDER 5
 #else
 #define DEG_ORDER 2
@@ -451,7 +457,7 @@ __device__ __noinline__ void deg_interp(float th, float dt, float tprev, const f
 // ------------------------------------------------------------------------------------
 // Rosenbrock23 (gpu_rosenbrock23_perform_step.jl:33-60, :109-153) with an FP64 W = I - gamma J
 // ------------------------------------------------------------------------------------
-#if DEG_METHOD == 2 || DEG_METHOD == 4 || DEG_METHOD == 5
+#if DEG_METHOD == 2 || DEG_METHOD == 4 || DEG_METHOD == 5 || DEG_METHOD == 6 || DEG_METHOD == 7
 // LU with partial pivoting; row swaps are applied with compile-time indices so W stays
 // in VGPRs.
 __device__ __forceinline__ void w_lu(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N]) {
@@ -705,6 +711,135 @@ __device__ __forceinline__ void deg_interp(float th, float dt, float tprev, cons
 #else
     DEG_FOR(c) out[c] = th1 * y0[c] + th * (y1[c] + th1 * (k[1][c] + th * (k[2][c] + th * k[3][c])));
 #endif
+}
+#endif
+
+// ------------------------------------------------------------------------------------
+// Kvaerno3 / Kvaerno5 (gpu_kvaerno3_perform_step.jl:1-232, gpu_kvaerno5_perform_step.jl:1-300):
+// ESDIRK stages solved by Newton's method (nlsolve/utils.jl:1-27, nlsolve/type.jl:129-185) with
+// W = -I + gamma dt J re-evaluated each iteration and factored in FP64; Hermite dense output
+// (nonstiff/interpolants.jl:1-20) from uprev, u, k1 and k2 = z_last / dt
+// ------------------------------------------------------------------------------------
+#if DEG_METHOD == 6 || DEG_METHOD == 7
+#define DEG_NK 2
+#if DEG_METHOD == 6
+#define KV(x) KV3_##x
+#else
+#define KV(x) KV5_##x
+#endif
+#define DEG_KV_NEWTON_MAX 30
+#define DEG_KV_NEWTON_TOL (100.0f * 1.1920928955078125e-7f)  // 100 eps(Float32)
+// W = -I + (gamma dt) J(x, tc), LU-factored in FP64
+__device__ __forceinline__ void kv_w(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N], float gdt,
+                                     const float (&x)[DEG_N], const float* p, float tc) {
+    float Jm[DEG_N * DEG_N];
+    deg_jac(x, p, tc, Jm);
+    const double g = (double)gdt;
+    DEG_FOR(r) {
+        _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (r == c ? -1.0 : 0.0) + g * (double)Jm[r * DEG_N + c];
+    }
+    w_lu(W, piv);
+}
+// Newton on z for the stage x = tmp + gamma z at tc = tnl + c dtnl: z -= W \ (dtnl f(x) - z), at
+// most 30 iterations, until ||dtnl f(tmp + gamma z) - z|| < 100 eps(Float32)
+__device__ __forceinline__ void kv_newton(float (&z)[DEG_N], const float (&tmp)[DEG_N], const float* p, float tnl,
+                                          float dtnl, float c) {
+    const float tc = tnl + c * dtnl;
+    const float gdt = KV(gamma) * dtnl;
+    for (int it = 0; it < DEG_KV_NEWTON_MAX; ++it) {
+        float x[DEG_N], fe[DEG_N], r[DEG_N], dz[DEG_N];
+        DEG_FOR(q) x[q] = tmp[q] + KV(gamma) * z[q];
+        double W[DEG_N][DEG_N];
+        int piv[DEG_N];
+        kv_w(W, piv, gdt, x, p, tc);
+        deg_rhs(x, p, tc, fe);
+        DEG_FOR(q) r[q] = dtnl * fe[q] - z[q];
+        w_solve(W, piv, r, dz);
+        DEG_FOR(q) z[q] = z[q] - dz[q];
+        DEG_FOR(q) x[q] = tmp[q] + KV(gamma) * z[q];
+        deg_rhs(x, p, tc, fe);
+        DEG_FOR(q) r[q] = dtnl * fe[q] - z[q];
+        if (rms_norm(r) < DEG_KV_NEWTON_TOL) break;
+    }
+}
+// One Kvaerno step from (uprev, t) with step dt and first stage slope k1 (z1 = dt k1); the
+// implicit stages use the nonlinear solver's time tnl and step dtnl. zl receives the last stage.
+// Returns EEst (W(u, t + dt) \ (btilde . z), scaled) when want_error is set.
+template <bool want_error>
+__device__ __forceinline__ float deg_kvaerno(const float (&uprev)[DEG_N], const float* p, float t, float dt, float tnl,
+                                             float dtnl, const float (&k1)[DEG_N], float (&zl)[DEG_N],
+                                             float (&unew)[DEG_N], float abstol, float reltol) {
+    float z1[DEG_N], z2[DEG_N], z3[DEG_N], tmp[DEG_N], ev[DEG_N];
+    DEG_FOR(q) z1[q] = dt * k1[q];
+    DEG_FOR(q) {
+        z2[q] = z1[q];
+        tmp[q] = uprev[q] + KV(gamma) * z1[q];
+    }
+    kv_newton(z2, tmp, p, tnl, dtnl, KV(gamma));
+    DEG_FOR(q) {
+        z3[q] = KV(alpha31) * z1[q] + KV(alpha32) * z2[q];
+        tmp[q] = (uprev[q] + KV(a31) * z1[q]) + KV(a32) * z2[q];
+    }
+    kv_newton(z3, tmp, p, tnl, dtnl, KV(c3));
+#if DEG_METHOD == 6
+    DEG_FOR(q) {
+        zl[q] = (KV(a31) * z1[q] + KV(a32) * z2[q]) + KV(gamma) * z3[q];
+        tmp[q] = ((uprev[q] + KV(a41) * z1[q]) + KV(a42) * z2[q]) + KV(a43) * z3[q];
+    }
+    kv_newton(zl, tmp, p, tnl, dtnl, 1.0f);
+    DEG_FOR(q) unew[q] = tmp[q] + KV(gamma) * zl[q];
+    if (!want_error) return 0.0f;
+    DEG_FOR(q) ev[q] = ((KV(btilde1) * z1[q] + KV(btilde2) * z2[q]) + KV(btilde3) * z3[q]) + KV(btilde4) * zl[q];
+#else
+    float z4[DEG_N], z5[DEG_N], z6[DEG_N];
+    DEG_FOR(q) {
+        z4[q] = (KV(alpha41) * z1[q] + KV(alpha42) * z2[q]) + KV(alpha43) * z3[q];
+        tmp[q] = ((uprev[q] + KV(a41) * z1[q]) + KV(a42) * z2[q]) + KV(a43) * z3[q];
+    }
+    kv_newton(z4, tmp, p, tnl, dtnl, KV(c4));
+    DEG_FOR(q) {
+        z5[q] = (KV(alpha51) * z1[q] + KV(alpha52) * z2[q]) + KV(alpha53) * z3[q];
+        tmp[q] = (((uprev[q] + KV(a51) * z1[q]) + KV(a52) * z2[q]) + KV(a53) * z3[q]) + KV(a54) * z4[q];
+    }
+    kv_newton(z5, tmp, p, tnl, dtnl, KV(c5));
+    DEG_FOR(q) {
+        z6[q] = (KV(alpha61) * z1[q] + KV(alpha62) * z2[q]) + KV(alpha63) * z3[q];
+        tmp[q] = (((uprev[q] + KV(a61) * z1[q]) + KV(a63) * z3[q]) + KV(a64) * z4[q]) + KV(a65) * z5[q];
+    }
+    kv_newton(z6, tmp, p, tnl, dtnl, KV(c6));
+    DEG_FOR(q) {
+        zl[q] = (((KV(a61) * z1[q] + KV(a63) * z3[q]) + KV(a64) * z4[q]) + KV(a65) * z5[q]) + KV(gamma) * z6[q];
+        tmp[q] = ((((uprev[q] + KV(a71) * z1[q]) + KV(a73) * z3[q]) + KV(a74) * z4[q]) + KV(a75) * z5[q]) + KV(a76) * z6[q];
+    }
+    kv_newton(zl, tmp, p, tnl, dtnl, 1.0f);
+    DEG_FOR(q) unew[q] = tmp[q] + KV(gamma) * zl[q];
+    if (!want_error) return 0.0f;
+    DEG_FOR(q) ev[q] = ((((KV(btilde1) * z1[q] + KV(btilde3) * z3[q]) + KV(btilde4) * z4[q]) + KV(btilde5) * z5[q]) +
+                        KV(btilde6) * z6[q]) + KV(btilde7) * zl[q];
+#endif
+    double W[DEG_N][DEG_N];
+    int piv[DEG_N];
+    float e[DEG_N];
+    kv_w(W, piv, KV(gamma) * dt, unew, p, t + 1.0f * dt);
+    w_solve(W, piv, ev, e);
+#if DEG_FASTPOW
+    return scaled_error2_fast(e, uprev, unew, abstol, reltol);
+#else
+    return scaled_error(e, uprev, unew, abstol, reltol);
+#endif
+}
+// Hermite: (1-th) y0 + th y1 + th (th-1) ((1-2th)(y1-y0) + (th-1) dt k1 + th dt k2)
+__device__ __forceinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                           const float (&y1)[DEG_N],
+                                           const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
+    (void)tprev;
+    (void)p;
+    const float a0 = 1.0f - th;
+    const float s = th * (th - 1.0f);
+    const float w = 1.0f - 2.0f * th;
+    const float d1 = (th - 1.0f) * dt;
+    const float d2 = th * dt;
+    DEG_FOR(c) out[c] = (a0 * y0[c] + th * y1[c]) + s * ((w * (y1[c] - y0[c]) + d1 * k[1][c]) + d2 * k[2][c]);
 }
 #endif
 
@@ -1419,6 +1554,15 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                 EEst = deg_error(dt, k, u, unew, a.abstol, a.reltol);
 #elif DEG_METHOD == 2
                 EEst = deg_rb23<true>(u, p, t, dt, k, unew, a.abstol, a.reltol);
+#elif DEG_METHOD == 6 || DEG_METHOD == 7
+                {
+                    // gpu_kvaerno*_perform_step.jl adaptive: k1 = f(uprev, t) in every attempt; the
+                    // nonlinear solver runs at (t, dt); dense output keeps k1 and z_last / dt
+                    float zl[DEG_N];
+                    deg_rhs(u, p, t, k[1]);
+                    EEst = deg_kvaerno<true>(u, p, t, dt, t, dt, k[1], zl, unew, a.abstol, a.reltol);
+                    DEG_FOR(c) k[2][c] = zl[c] / dt;
+                }
 #else
                 EEst = deg_rodas<true>(u, p, t, dt, k, unew, a.abstol, a.reltol);
 #endif
@@ -1491,6 +1635,20 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                 deg_attempt(uprev, p, told, dts, k, unew);
 #elif DEG_METHOD == 2
                 deg_rb23<false>(uprev, p, told, dts, k, unew, 0.0f, 0.0f);
+#elif DEG_METHOD == 6 || DEG_METHOD == 7
+                {
+                    // gpu_kvaerno*_perform_step.jl fixed: k1 = f(uprev, t) after u_modified, else the
+                    // last step's k1; the nonlinear solver is built with the advanced t and the
+                    // integrator's dt; the stored k1 is f(u, t) at the step's start time t
+                    float zl[DEG_N];
+                    if (!have_k) {
+                        deg_rhs(uprev, p, told, k[1]);
+                        have_k = true;
+                    }
+                    deg_kvaerno<false>(uprev, p, told, dts, tn, dt, k[1], zl, unew, 0.0f, 0.0f);
+                    DEG_FOR(c) k[2][c] = zl[c] / dts;
+                    deg_rhs(unew, p, told, k[1]);
+                }
 #else
                 deg_rodas<false>(uprev, p, told, dts, k, unew, 0.0f, 0.0f);
 #endif

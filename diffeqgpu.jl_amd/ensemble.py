@@ -89,7 +89,8 @@ def solve(ensembleprob: EnsembleProblem, alg: GPUODEAlgorithm, ensemblealg: Ense
     if not isinstance(ensemblealg, EnsembleGPUKernel):
         raise TypeError("only EnsembleGPUKernel is provided")
     if not isinstance(alg, GPUODEAlgorithm):
-        raise TypeError("alg must be GPUTsit5(), GPUVern7(), GPUVern9(), GPURosenbrock23(), GPURodas4() or GPURodas5P()")
+        raise TypeError("alg must be a GPU ODE algorithm (GPUTsit5, GPUVern7, GPUVern9, GPURosenbrock23, GPURodas4, "
+                        "GPURodas5P, GPUKvaerno3, GPUKvaerno5)")
     if trajectories < 1:
         raise ValueError("trajectories must be >= 1")
     batch_size = trajectories if batch_size is None else int(batch_size)

@@ -33,14 +33,16 @@ extern "C" {
 #define DEG_E_UNSUPPORTED (-4)
 #define DEG_E_NOMEM (-5)
 
-/* methods: GPUTsit5, GPUVern7, GPURosenbrock23, GPUVern9, GPURodas4, GPURodas5P
- * (src/ensemblegpukernel/gpukernel_algorithms.jl:30, 57, 112, 83, 137, 162) */
+/* methods: GPUTsit5, GPUVern7, GPURosenbrock23, GPUVern9, GPURodas4, GPURodas5P, GPUKvaerno3, GPUKvaerno5
+ * (src/ensemblegpukernel/gpukernel_algorithms.jl:30, 57, 112, 83, 137, 162, 187, 212) */
 #define DEG_TSIT5 0
 #define DEG_VERN7 1
 #define DEG_RB23 2
 #define DEG_VERN9 3
 #define DEG_RODAS4 4
 #define DEG_RODAS5P 5
+#define DEG_KVAERNO3 6
+#define DEG_KVAERNO5 7
 
 /* compile flags (one code object per (rhs, n, np, method, flags)) */
 #define DEG_F_ADAPTIVE (1u << 0)   /* PI-controlled adaptive stepping (vectorized_asolve) */
@@ -137,8 +139,8 @@ int deg_set_cache_dir(const char *dir);
 /* Compile (or fetch from the cache) the solver for one traced right-hand side.
  * rhs_src defines, in HIP C:
  *   __device__ void deg_rhs(const float* u, const float* p, float t, float* du);
- * and for DEG_RB23, DEG_RODAS4 and DEG_RODAS5P also deg_jac (row-major J[r*n + c] = d du_r / d u_c)
- * and deg_tgrad.
+ * and for DEG_RB23, DEG_RODAS4, DEG_RODAS5P, DEG_KVAERNO3 and DEG_KVAERNO5 also deg_jac (row-major
+ * J[r*n + c] = d du_r / d u_c) and deg_tgrad.
  * With DEG_F_CALLBACKS it also defines the callbacks (GPUDiscreteCallback / GPUContinuousCallback,
  * src/ensemblegpukernel/callbacks.jl:1-124), j counting from 0 in CallbackSet order:
  *   #define DEG_NDCB <discrete callbacks>   #define DEG_NCCB <continuous callbacks>

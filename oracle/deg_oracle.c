@@ -12,6 +12,8 @@
  *   Rosenbrock23 fixed / adaptive step      perform_step/gpu_rosenbrock23_perform_step.jl:1-70 / :74-201
  *   Vern9 fixed / adaptive step             perform_step/gpu_vern9_perform_step.jl:1-124 / :128-319
  *   Rodas4 / Rodas5P fixed / adaptive step  perform_step/gpu_rodas4_perform_step.jl, gpu_rodas5P_perform_step.jl
+ *   Kvaerno3 / Kvaerno5 fixed / adaptive    perform_step/gpu_kvaerno3_perform_step.jl, gpu_kvaerno5_perform_step.jl,
+ *   (Newton stage solver)                   nlsolve/utils.jl:1-27, nlsolve/type.jl:129-185
  *   PI controller constants                 integrators/integrator_utils.jl:1-11, alg_utils.jl:5-8
  *   saveat / save_everystep writes          integrators/integrator_utils.jl:13-50
  *   dense output                            integrators/nonstiff/interpolants.jl:28-190 (Vern7), :192-370
@@ -34,8 +36,9 @@
  *     with DtLessThanMin (the reference repeats such a step forever); when the next dt is
  *     larger the lane goes on with t unchanged, as the reference does. A fixed step that
  *     cannot advance t ends the lane with DtLessThanMin;
- *   - Rosenbrock23 forms W = I - gamma J, Rodas4/Rodas5P W = J - I / (dt gamma), and they solve
- *     it in FP64 (LU with partial pivoting) instead of Float32 Cramer's rule (linalg/linsolve.jl:31-57);
+ *   - Rosenbrock23 forms W = I - gamma J, Rodas4/Rodas5P W = J - I / (dt gamma), Kvaerno3/5 (Newton)
+ *     W = -I + gamma dt J, and they solve it in FP64 (LU with partial pivoting) instead of Float32
+ *     Cramer's rule (linalg/linsolve.jl:31-57);
  *   - optional compensated (Kahan) accumulation of t (flag DEG_O_KAHAN).
  * Dense-output polynomials use fused multiply-add Horner steps (the reference's
  * @muladd/@evalpoly), vector combinations are left folds without fusion.
@@ -58,7 +61,8 @@
 #define OMAXN 16
 #define OMAXS 17
 
-enum { OM_TSIT5 = 0, OM_VERN7 = 1, OM_RB23 = 2, OM_VERN9 = 3, OM_RODAS4 = 4, OM_RODAS5P = 5 };
+enum { OM_TSIT5 = 0, OM_VERN7 = 1, OM_RB23 = 2, OM_VERN9 = 3, OM_RODAS4 = 4, OM_RODAS5P = 5, OM_KVAERNO3 = 6,
+       OM_KVAERNO5 = 7 };
 enum { DEG_O_KAHAN = 1, DEG_O_DETPOW = 2 };
 /* per-trajectory return codes (same values as the C ABI of the HIP library) */
 enum { ORET_SUCCESS = 0, ORET_DTMIN = 1, ORET_MAXITERS = 2, ORET_NONFINITE = 3, ORET_TERMINATED = 4,
@@ -606,6 +610,120 @@ static void rodas_interp(int method, float th, int n, const float *y0, const flo
 }
 
 /* ------------------------------------------------------------------------------------ */
+/* Kvaerno3 / Kvaerno5 (gpu_kvaerno3_perform_step.jl, gpu_kvaerno5_perform_step.jl)             */
+/* ------------------------------------------------------------------------------------ */
+/* W = -I + (gamma dt) J(x, tc) (nlsolve/type.jl:129-140), factored in FP64 */
+static void kv_w(olu *F, const osys *S, const float *p, float gdt, const float *x, float tc) {
+    const int n = S->n;
+    float Jm[OMAXN * OMAXN];
+    S->jac(x, p, tc, Jm);
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < n; ++c) F->lu[r][c] = (r == c ? -1.0 : 0.0) + (double)gdt * (double)Jm[r * n + c];
+    w_lu(F, n);
+}
+
+/* nlsolve (nlsolve/utils.jl:1-27): Newton on z for x = tmp + gamma z at tc = tnl + c dtnl */
+static void kv_newton(const osys *S, const float *p, float gamma, float *z, const float *tmp, float tnl, float dtnl,
+                      float c) {
+    const int n = S->n;
+    const float tc = tnl + c * dtnl;
+    const float gdt = gamma * dtnl;
+    for (int it = 0; it < 30; ++it) {
+        float x[OMAXN], fe[OMAXN], r[OMAXN] = {0}, dz[OMAXN];
+        for (int q = 0; q < n; ++q) x[q] = tmp[q] + gamma * z[q];
+        olu F;
+        kv_w(&F, S, p, gdt, x, tc);
+        S->f(x, p, tc, fe);
+        for (int q = 0; q < n; ++q) r[q] = dtnl * fe[q] - z[q];
+        w_solve(&F, n, r, dz);
+        for (int q = 0; q < n; ++q) z[q] = z[q] - dz[q];
+        for (int q = 0; q < n; ++q) x[q] = tmp[q] + gamma * z[q];
+        S->f(x, p, tc, fe);
+        for (int q = 0; q < n; ++q) r[q] = dtnl * fe[q] - z[q];
+        if (rms_norm(r, n) < 100.0f * 1.1920928955078125e-7f) break;
+    }
+}
+
+/* One Kvaerno step: z1 = dt k1, implicit stages at the solver's (tnl, dtnl); zl the last stage;
+ * when err is non-NULL it receives W(u, t + dt) \ (btilde . z). */
+#define K3(x) OKV3_##x
+#define K5(x) OKV5_##x
+static void kvaerno_step(int method, const osys *S, const float *p, const float *uprev, float t, float dt, float tnl,
+                         float dtnl, const float *k1, float *zl, float *unew, float *err) {
+    const int n = S->n;
+    float z1[OMAXN] = {0}, z2[OMAXN] = {0}, z3[OMAXN] = {0}, tmp[OMAXN] = {0}, ev[OMAXN] = {0};
+    const float g = (method == OM_KVAERNO3) ? K3(gamma) : K5(gamma);
+    for (int q = 0; q < n; ++q) z1[q] = dt * k1[q];
+    for (int q = 0; q < n; ++q) {
+        z2[q] = z1[q];
+        tmp[q] = uprev[q] + g * z1[q];
+    }
+    kv_newton(S, p, g, z2, tmp, tnl, dtnl, g);
+    if (method == OM_KVAERNO3) {
+        for (int q = 0; q < n; ++q) {
+            z3[q] = K3(alpha31) * z1[q] + K3(alpha32) * z2[q];
+            tmp[q] = (uprev[q] + K3(a31) * z1[q]) + K3(a32) * z2[q];
+        }
+        kv_newton(S, p, g, z3, tmp, tnl, dtnl, K3(c3));
+        for (int q = 0; q < n; ++q) {
+            zl[q] = (K3(a31) * z1[q] + K3(a32) * z2[q]) + g * z3[q];
+            tmp[q] = ((uprev[q] + K3(a41) * z1[q]) + K3(a42) * z2[q]) + K3(a43) * z3[q];
+        }
+        kv_newton(S, p, g, zl, tmp, tnl, dtnl, 1.0f);
+        for (int q = 0; q < n; ++q) unew[q] = tmp[q] + g * zl[q];
+        for (int q = 0; q < n; ++q)
+            ev[q] = ((K3(btilde1) * z1[q] + K3(btilde2) * z2[q]) + K3(btilde3) * z3[q]) + K3(btilde4) * zl[q];
+    } else {
+        float z4[OMAXN], z5[OMAXN], z6[OMAXN];
+        for (int q = 0; q < n; ++q) {
+            z3[q] = K5(alpha31) * z1[q] + K5(alpha32) * z2[q];
+            tmp[q] = (uprev[q] + K5(a31) * z1[q]) + K5(a32) * z2[q];
+        }
+        kv_newton(S, p, g, z3, tmp, tnl, dtnl, K5(c3));
+        for (int q = 0; q < n; ++q) {
+            z4[q] = (K5(alpha41) * z1[q] + K5(alpha42) * z2[q]) + K5(alpha43) * z3[q];
+            tmp[q] = ((uprev[q] + K5(a41) * z1[q]) + K5(a42) * z2[q]) + K5(a43) * z3[q];
+        }
+        kv_newton(S, p, g, z4, tmp, tnl, dtnl, K5(c4));
+        for (int q = 0; q < n; ++q) {
+            z5[q] = (K5(alpha51) * z1[q] + K5(alpha52) * z2[q]) + K5(alpha53) * z3[q];
+            tmp[q] = (((uprev[q] + K5(a51) * z1[q]) + K5(a52) * z2[q]) + K5(a53) * z3[q]) + K5(a54) * z4[q];
+        }
+        kv_newton(S, p, g, z5, tmp, tnl, dtnl, K5(c5));
+        for (int q = 0; q < n; ++q) {
+            z6[q] = (K5(alpha61) * z1[q] + K5(alpha62) * z2[q]) + K5(alpha63) * z3[q];
+            tmp[q] = (((uprev[q] + K5(a61) * z1[q]) + K5(a63) * z3[q]) + K5(a64) * z4[q]) + K5(a65) * z5[q];
+        }
+        kv_newton(S, p, g, z6, tmp, tnl, dtnl, K5(c6));
+        for (int q = 0; q < n; ++q) {
+            zl[q] = (((K5(a61) * z1[q] + K5(a63) * z3[q]) + K5(a64) * z4[q]) + K5(a65) * z5[q]) + g * z6[q];
+            tmp[q] = ((((uprev[q] + K5(a71) * z1[q]) + K5(a73) * z3[q]) + K5(a74) * z4[q]) + K5(a75) * z5[q]) +
+                     K5(a76) * z6[q];
+        }
+        kv_newton(S, p, g, zl, tmp, tnl, dtnl, 1.0f);
+        for (int q = 0; q < n; ++q) unew[q] = tmp[q] + g * zl[q];
+        for (int q = 0; q < n; ++q)
+            ev[q] = ((((K5(btilde1) * z1[q] + K5(btilde3) * z3[q]) + K5(btilde4) * z4[q]) + K5(btilde5) * z5[q]) +
+                     K5(btilde6) * z6[q]) + K5(btilde7) * zl[q];
+    }
+    if (err) {
+        olu F;
+        kv_w(&F, S, p, g * dt, unew, t + 1.0f * dt);
+        w_solve(&F, n, ev, err);
+    }
+}
+
+/* Hermite dense output (nonstiff/interpolants.jl:1-20) with y1 the integrator's current u */
+static void hermite_interp(float th, float dt, int n, const float *y0, const float *y1, float k[][OMAXN], float *out) {
+    const float a0 = 1.0f - th;
+    const float s = th * (th - 1.0f);
+    const float w = 1.0f - 2.0f * th;
+    const float d1 = (th - 1.0f) * dt;
+    const float d2 = th * dt;
+    for (int c = 0; c < n; ++c) out[c] = (a0 * y0[c] + th * y1[c]) + s * ((w * (y1[c] - y0[c]) + d1 * k[1][c]) + d2 * k[2][c]);
+}
+
+/* ------------------------------------------------------------------------------------ */
 /* Per-trajectory solver                                                                 */
 /* ------------------------------------------------------------------------------------ */
 /* Callback descriptors: the conditions and affects the tests use, restated in C. They play the
@@ -709,6 +827,7 @@ static void interp_at(const ojob *J, const ost *s, float x, float *out) {
     const float th = (x - s->tprev) / s->dtstep;
     const float d = 1.0f / (2.0f + sqrtf(2.0f));
     if (J->method == OM_RODAS4 || J->method == OM_RODAS5P) rodas_interp(J->method, th, s->n, s->uprev, s->u, (float(*)[OMAXN])s->k, out);
+    else if (J->method == OM_KVAERNO3 || J->method == OM_KVAERNO5) hermite_interp(th, s->dtstep, s->n, s->uprev, s->u, (float(*)[OMAXN])s->k, out);
     else if (J->method == OM_TSIT5) tsit5_interp(th, s->dtstep, s->n, s->uprev, (float(*)[OMAXN])s->k, out);
     else if (J->method == OM_RB23) rb23_interp(th, s->dtstep, s->n, d, s->uprev, s->rk1, s->rk2, out);
     else if (J->method == OM_VERN9) vern9_interp(J->S, s->p, th, s->dtstep, s->tprev, s->n, s->uprev, (float(*)[OMAXN])s->k, out);
@@ -987,7 +1106,15 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                     break;
                 }
             }
-            if (J->method == OM_RODAS4 || J->method == OM_RODAS5P) {
+            if (J->method == OM_KVAERNO3 || J->method == OM_KVAERNO5) {
+                /* gpu_kvaerno*_perform_step.jl fixed: the nonlinear solver gets the advanced t and the
+                 * integrator's dt; k1 is the last step's f(u, t) unless u was modified */
+                float zl[OMAXN];
+                if (s->umod) S->f(s->uprev, s->p, told, k[1]);
+                kvaerno_step(J->method, S, s->p, s->uprev, told, dts, s->t, dt, k[1], zl, s->u, NULL);
+                for (int c = 0; c < n; ++c) k[2][c] = zl[c] / dts;
+                S->f(s->u, s->p, told, k[1]);
+            } else if (J->method == OM_RODAS4 || J->method == OM_RODAS5P) {
                 float kerr[OMAXN];
                 if (J->method == OM_RODAS4) rodas4_step(S, s->p, s->uprev, told, dts, k, s->u, kerr);
                 else rodas5p_step(S, s->p, s->uprev, told, dts, k, s->u, kerr);
@@ -1036,7 +1163,7 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
         }
     } else {
         /* ---------------- adaptive: kernels.jl:128-131 -------------------------------- */
-        static const int orders[6] = {5, 7, 2, 9, 4, 5};
+        static const int orders[8] = {5, 7, 2, 9, 4, 5, 3, 5};
         const int order = orders[J->method];
         /* integrator_utils.jl:1-11 */
         const float beta1 = (float)(7.0 / (10.0 * order));
@@ -1062,7 +1189,18 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                 if (attempts >= J->max_attempts) { s->retcode = ORET_MAXITERS; stop = 1; break; }
                 ++attempts;
                 float q11 = 0.0f, q;
-                if (J->method == OM_RODAS4 || J->method == OM_RODAS5P) {
+                if (J->method == OM_KVAERNO3 || J->method == OM_KVAERNO5) {
+                    /* gpu_kvaerno*_perform_step.jl adaptive: k1 = f(uprev, t) in every attempt */
+                    float zl[OMAXN], e[OMAXN];
+                    S->f(s->uprev, s->p, t, k[1]);
+                    kvaerno_step(J->method, S, s->p, s->uprev, t, dt, t, dt, k[1], zl, unew, e);
+                    for (int c = 0; c < n; ++c) k[2][c] = zl[c] / dt;
+                    for (int c = 0; c < n; ++c) {
+                        float sc = J->abstol + jl_max(fabsf(s->uprev[c]), fabsf(unew[c])) * J->reltol;
+                        e[c] = e[c] / sc;
+                    }
+                    EEst = rms_norm(e, n);
+                } else if (J->method == OM_RODAS4 || J->method == OM_RODAS5P) {
                     /* gpu_rodas4_perform_step.jl:158-238, gpu_rodas5P_perform_step.jl:186-296 */
                     float e[OMAXN];
                     if (J->method == OM_RODAS4) rodas4_step(S, s->p, s->uprev, t, dt, k, unew, e);
@@ -1191,7 +1329,7 @@ int deg_oracle_solve_cb(int system, int method, int adaptive, int64_t N,
                         float *ts, float *us, int32_t *retcode, int32_t *naccept, int32_t *nreject,
                         int nthreads, const float *tstops, int ntstops, const ocb *cbs, int ncb) {
     if (system < 0 || system >= OSYS_COUNT) return -1;
-    if (method < OM_TSIT5 || method > OM_RODAS5P) return -1;
+    if (method < OM_TSIT5 || method > OM_KVAERNO5) return -1;
     if (ntstops < 0 || ncb < 0 || (ntstops > 0 && !tstops) || (ncb > 0 && !cbs)) return -1;
     init_tabs();
     const osys *S = &OSYSTEMS[system];

@@ -18,7 +18,8 @@ SYSTEM_IDS = {"lorenz": 0, "decay": 1, "growth": 2, "quad": 3, "robertson": 4, "
               "ball": 8}
 SYSTEM_DIMS = {"lorenz": (3, 3), "decay": (1, 1), "growth": (1, 1), "quad": (1, 1), "robertson": (3, 3),
                "vdp": (2, 1), "tdep": (1, 1), "rot": (2, 1), "ball": (2, 1)}
-METHOD_IDS = {"tsit5": 0, "vern7": 1, "rb23": 2, "vern9": 3, "rodas4": 4, "rodas5p": 5}
+METHOD_IDS = {"tsit5": 0, "vern7": 1, "rb23": 2, "vern9": 3, "rodas4": 4, "rodas5p": 5, "kvaerno3": 6,
+              "kvaerno5": 7}
 KAHAN, DETPOW = 1, 2
 
 _fp = ctypes.POINTER(ctypes.c_float)

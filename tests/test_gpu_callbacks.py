@@ -105,7 +105,8 @@ def _run(deg, torch, system, method, adaptive, names, *, dt, N=192, seed=0, save
     probs = deg.HIPProbVector.from_arrays(f, u0, p, tspan, dev)
     prob = deg.ODEProblem(f, u0[:, 0], tspan, p[:, 0])
     alg = {"tsit5": deg.GPUTsit5, "vern7": deg.GPUVern7, "rb23": deg.GPURosenbrock23, "vern9": deg.GPUVern9,
-           "rodas4": deg.GPURodas4, "rodas5p": deg.GPURodas5P}[method]()
+           "rodas4": deg.GPURodas4, "rodas5p": deg.GPURodas5P, "kvaerno3": deg.GPUKvaerno3,
+           "kvaerno5": deg.GPUKvaerno5}[method]()
     kw = dict(dt=dt, callback=cbs, tstops=tstops)
     if saveat is not None:
         kw["saveat"] = saveat
@@ -136,7 +137,7 @@ def _assert_same(got, o, rows_written=None):
     np.testing.assert_array_equal(got["us"][mask], o["us"][mask], err_msg="us")
 
 
-@pytest.mark.parametrize("method", ["tsit5", "vern7", "rb23", "vern9", "rodas4", "rodas5p"])
+@pytest.mark.parametrize("method", ["tsit5", "vern7", "rb23", "vern9", "rodas4", "rodas5p", "kvaerno3", "kvaerno5"])
 @pytest.mark.parametrize("persistent", [False, True])
 def test_discrete_fixed_step(deg, torch_cuda, method, persistent):
     for names, tst, dt, kw in ((["jump24"], [2.4], 1.0, {}),
@@ -149,7 +150,7 @@ def test_discrete_fixed_step(deg, torch_cuda, method, persistent):
         _assert_same(got, o)
 
 
-@pytest.mark.parametrize("method", ["tsit5", "vern7", "rb23", "vern9", "rodas4", "rodas5p"])
+@pytest.mark.parametrize("method", ["tsit5", "vern7", "rb23", "vern9", "rodas4", "rodas5p", "kvaerno3", "kvaerno5"])
 @pytest.mark.parametrize("persistent", [False, True])
 def test_discrete_adaptive(deg, torch_cuda, method, persistent):
     for names, tst, kw in ((["jump24"], [4.0], {}),
@@ -169,7 +170,7 @@ def test_terminate(deg, torch_cuda, method):
         _assert_same(got, o)
 
 
-@pytest.mark.parametrize("method", ["tsit5", "vern7", "rb23", "rodas4", "rodas5p"])
+@pytest.mark.parametrize("method", ["tsit5", "vern7", "rb23", "rodas4", "rodas5p", "kvaerno3", "kvaerno5"])
 @pytest.mark.parametrize("persistent", [False, True])
 def test_continuous_ball(deg, torch_cuda, method, persistent):
     for adaptive, dt, names, kw in ((False, 0.1, ["bounce"], {}),
@@ -196,7 +197,7 @@ def test_repeated_crossings_itp(deg, torch_cuda, method):
 
 
 def test_tstops_only(deg, torch_cuda):
-    for method in ("tsit5", "vern7", "rb23", "vern9", "rodas4", "rodas5p"):
+    for method in ("tsit5", "vern7", "rb23", "vern9", "rodas4", "rodas5p", "kvaerno3", "kvaerno5"):
         for adaptive in (False, True):
             got, o = _run(deg, torch_cuda, "decay", method, adaptive, [], dt=0.3, tstops=[0.5, 1.25, 2.0, 7.77],
                           saveat=np.linspace(0, 10, 9) if adaptive else None)

@@ -19,7 +19,7 @@ import pyoracle
 pytestmark = pytest.mark.gpu
 F32 = np.float32
 METHODS = {"tsit5": "GPUTsit5", "vern7": "GPUVern7", "rb23": "GPURosenbrock23", "vern9": "GPUVern9",
-           "rodas4": "GPURodas4", "rodas5p": "GPURodas5P"}
+           "rodas4": "GPURodas4", "rodas5p": "GPURodas5P", "kvaerno3": "GPUKvaerno3", "kvaerno5": "GPUKvaerno5"}
 
 
 @pytest.fixture(scope="module")
@@ -89,6 +89,8 @@ FIXED_CASES = [
     ("lorenz", "vern9", (0.0, 10.0), 0.1), ("vdp", "vern9", (0.0, 5.0), 0.05), ("tdep", "vern9", (0.0, 2.0), 0.0625),
     ("lorenz", "rodas4", (0.0, 2.0), 0.01), ("vdp", "rodas4", (0.0, 5.0), 0.01), ("tdep", "rodas4", (0.0, 2.0), 0.0625),
     ("robertson", "rodas5p", (0.0, 1.0), 0.001), ("vdp", "rodas5p", (0.0, 5.0), 0.01), ("tdep", "rodas5p", (0.0, 2.0), 0.0625),
+    ("vdp", "kvaerno3", (0.0, 5.0), 0.01), ("tdep", "kvaerno3", (0.0, 2.0), 0.0625), ("lorenz", "kvaerno3", (0.0, 1.0), 0.01),
+    ("robertson", "kvaerno5", (0.0, 1.0), 0.001), ("vdp", "kvaerno5", (0.0, 5.0), 0.01), ("tdep", "kvaerno5", (0.0, 2.0), 0.0625),
 ]
 
 
@@ -128,7 +130,9 @@ ADAPTIVE_CASES = [
     ("tdep", "tsit5", (0.0, 3.0), 0.1, 1e-7, 1e-7), ("lorenz", "vern9", (0.0, 10.0), 0.1, 1e-8, 1e-8),
     ("vdp", "vern9", (0.0, 10.0), 0.01, 1e-7, 1e-7), ("robertson", "rodas4", (0.0, 1e5), 1e-6, 1e-8, 1e-4),
     ("vdp", "rodas4", (0.0, 10.0), 0.01, 1e-6, 1e-4), ("robertson", "rodas5p", (0.0, 1e5), 1e-6, 1e-8, 1e-4),
-    ("lorenz", "rodas5p", (0.0, 3.0), 0.01, 1e-5, 1e-5),
+    ("lorenz", "rodas5p", (0.0, 3.0), 0.01, 1e-5, 1e-5), ("robertson", "kvaerno3", (0.0, 1e5), 1e-6, 1e-8, 1e-4),
+    ("vdp", "kvaerno3", (0.0, 10.0), 0.01, 1e-6, 1e-4), ("robertson", "kvaerno5", (0.0, 1e5), 1e-6, 1e-8, 1e-4),
+    ("vdp", "kvaerno5", (0.0, 10.0), 0.01, 1e-6, 1e-5),
 ]
 
 

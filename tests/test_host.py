@@ -46,13 +46,13 @@ def test_solve_args_offsets_match_header(deg):
     assert ctypes.sizeof(_lib.SolveArgs) == 248
 
 
-@pytest.mark.parametrize("method", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize("flags", [0, 2, 1, 1 | 2 | 4, 1 | 16, 1 | 64, 1 | 8 | 16])
 def test_jit_compiles_every_method(deg, method, flags):
     from diffeqgpu_amd import _lib, systems
 
     f, n, np_, _ = systems.SYSTEMS["lorenz"]
-    src, _ = f.hip_source(n, np_, method in (2, 4, 5))
+    src, _ = f.hip_source(n, np_, method in (2, 4, 5, 6, 7))
     key = _lib.precompile(src, n, np_, method, flags)
     assert re.fullmatch(r"[0-9a-f]{16}", key)
     assert os.path.exists(os.path.join(_lib.CACHE_DIR, key + ".co"))
