@@ -16,6 +16,7 @@ CACHE_DIR = os.path.join(PKG_DIR, "_jitcache")
 
 DEG_OK = 0
 DEG_TSIT5, DEG_VERN7, DEG_RB23, DEG_VERN9, DEG_RODAS4, DEG_RODAS5P, DEG_KVAERNO3, DEG_KVAERNO5 = range(8)
+DEG_EM, DEG_SIEA = 8, 9
 F_ADAPTIVE = 1 << 0
 F_PARITY = 1 << 1
 F_DETPOW = 1 << 2
@@ -84,6 +85,9 @@ class SolveArgs(ctypes.Structure):
         ("tstops", ctypes.c_void_p),
         ("ntstops", ctypes.c_int32),
         ("reserved0", ctypes.c_int32),
+        ("seeds", ctypes.c_void_p),
+        ("seed", ctypes.c_uint64),
+        ("traj_base", ctypes.c_int64),
     ]
 
 
@@ -93,6 +97,7 @@ EXPORTS = [
     "deg_last_error", "deg_version", "deg_solve_args_size", "deg_draw_uniform", "deg_graph_begin", "deg_graph_end",
     "deg_graph_launch", "deg_graph_num_nodes", "deg_graph_destroy", "deg_mlp_packed_size", "deg_mlp_partial_cols",
     "deg_mlp_grad", "deg_mlp_adam", "deg_mlp_forward", "deg_mlp_pack", "deg_mlp_infer", "deg_order_by_span",
+    "deg_draw_normal",
 ]
 STEPLOG_COLS = 12
 MLP_NPARAM = 1313
@@ -128,6 +133,8 @@ def _declare(lib):
     lib.deg_draw_uniform.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int32,
                                      ctypes.c_int32, ctypes.c_float, ctypes.c_float, vp]
     lib.deg_order_by_span.argtypes = [vp, vp, ctypes.c_float, ctypes.c_float, ctypes.c_int64, vp, vp, vp]
+    lib.deg_draw_normal.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32,
+                                    ctypes.c_int32, vp]
     lib.deg_graph_begin.argtypes = [vp]
     lib.deg_graph_end.argtypes = [vp, ctypes.POINTER(vp)]
     lib.deg_graph_launch.argtypes = [vp, vp]
@@ -145,7 +152,7 @@ def _declare(lib):
                  "deg_kernel_destroy", "deg_kernel_attr", "deg_jit_precompile", "deg_solve", "deg_sync",
                  "deg_draw_uniform", "deg_graph_begin", "deg_graph_end", "deg_graph_launch", "deg_graph_num_nodes",
                  "deg_graph_destroy", "deg_mlp_grad", "deg_mlp_adam", "deg_mlp_forward", "deg_mlp_pack",
-                 "deg_mlp_infer", "deg_order_by_span"):
+                 "deg_mlp_infer", "deg_order_by_span", "deg_draw_normal"):
         getattr(lib, name).restype = ctypes.c_int
 
 
@@ -251,6 +258,13 @@ def draw_uniform(out_ptr: int, n: int, seed: int, rows: int, row: int, lo: float
     """Device draws ``out[i] = lo + scale * U(seed, (id_offset + i) * rows + row)`` (rng.uniform_table)."""
     check(lib().deg_draw_uniform(ctypes.c_void_p(out_ptr), n, id_offset, seed & 0xFFFFFFFFFFFFFFFF, rows, row, lo,
                                  scale, ctypes.c_void_p(stream_ptr)), "deg_draw_uniform")
+
+
+def draw_normal(out_ptr: int, n: int, seed: int, step: int, m: int, stream_ptr, id_offset: int = 0):
+    """Device standard normals of the SDE kernels' generator: ``out[c * n + i]`` = normal c of step
+    ``step`` of noise stream ``id_offset + i`` under ``seed`` (deg_draw_normal)."""
+    check(lib().deg_draw_normal(ctypes.c_void_p(out_ptr), n, id_offset, seed & 0xFFFFFFFFFFFFFFFF, step, m,
+                                ctypes.c_void_p(stream_ptr)), "deg_draw_normal")
 
 
 class Graph:

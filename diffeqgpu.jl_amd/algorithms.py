@@ -1,7 +1,7 @@
 """Algorithm selectors and the ensemble algorithm.
 
 Mirrors src/ensemblegpukernel/gpukernel_algorithms.jl (GPUTsit5 :30, GPUVern7 :57,
-GPURosenbrock23 :112 with the ``autodiff`` keyword :214-220), alg_utils.jl:1-13
+GPURosenbrock23 :112 with the ``autodiff`` keyword :214-220, GPUEM :243, GPUSIEA :266), alg_utils.jl:1-16
 (``alg_order``) and src/algorithms.jl:192-211 (``EnsembleGPUKernel(dev, cpu_offload)``).
 """
 from __future__ import annotations
@@ -93,9 +93,39 @@ class GPUKvaerno5(GPUODEImplicitAlgorithm):
     order = 5
 
 
-def alg_order(alg: GPUODEAlgorithm) -> int:
-    """alg_utils.jl:1-13."""
-    if not isinstance(alg, GPUODEAlgorithm) or alg.order == 0:
+class GPUSDEAlgorithm:
+    """The fixed-step SDE methods of EnsembleGPUKernel (src/DiffEqGPU.jl:138-150)."""
+
+    method: int = -1
+    order: int = 0
+
+    def __repr__(self):
+        return f"{type(self).__name__}()"
+
+    def __eq__(self, other):
+        return type(self) is type(other)
+
+    def __hash__(self):
+        return hash(type(self).__name__)
+
+
+class GPUEM(GPUSDEAlgorithm):
+    """Euler-Maruyama, weak order 1, diagonal and general noise (gpukernel_algorithms.jl:243)."""
+
+    method = _lib.DEG_EM
+    order = 1
+
+
+class GPUSIEA(GPUSDEAlgorithm):
+    """SIEA, weak order 2 for Ito SDEs with diagonal noise (gpukernel_algorithms.jl:266)."""
+
+    method = _lib.DEG_SIEA
+    order = 2
+
+
+def alg_order(alg) -> int:
+    """alg_utils.jl:1-16."""
+    if not isinstance(alg, (GPUODEAlgorithm, GPUSDEAlgorithm)) or alg.order == 0:
         raise ValueError("Order is not defined for this algorithm")
     return alg.order
 
@@ -147,6 +177,19 @@ class HIPDevice:
     blocks_per_cu: int = 0
     compact: bool = False
     batch_saves: bool = False
+
+    def sde_flags(self) -> int:
+        """Compile flags of the fixed-step SDE kernels: contraction (default "off") and the
+        register cap; the ODE-only knobs (persistent grids, kahan, batched saves, compaction,
+        controller powers) do not apply."""
+        contract = self.contract or "off"
+        if contract not in ("off", "fast"):
+            raise ValueError("contract must be 'off' or 'fast'")
+        if self.kahan:
+            raise ValueError("kahan summation of t is not provided by the SDE kernels")
+        if not 0 <= self.waves <= 7:
+            raise ValueError("waves must be in 0..7")
+        return (_lib.F_PARITY if contract == "off" else 0) | (self.waves << _lib.F_WAVES_SHIFT)
 
     def flags(self, adaptive: bool) -> int:
         f = 0

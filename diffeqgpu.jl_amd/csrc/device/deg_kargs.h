@@ -34,4 +34,7 @@ struct deg_kargs {
     const float* tstops;      // [ntstops] stopping times shared by all trajectories, or null
     int ntstops;
     int pad0;
+    const unsigned long long* seeds;  // SDE: [stride] per-trajectory seeds, or null -> seed
+    unsigned long long seed;          // SDE: shared seed
+    long long traj_base;              // SDE: ensemble index of column 0 (the noise stream id is traj_base + column)
 };

@@ -16,7 +16,7 @@ from typing import Any, List, Optional
 import numpy as np
 
 from . import _lib
-from .algorithms import EnsembleGPUKernel, GPUODEAlgorithm
+from .algorithms import EnsembleGPUKernel, GPUODEAlgorithm, GPUSDEAlgorithm
 from .callbacks import merged_callback
 from .lowerlevel import vectorized_asolve, vectorized_solve
 from .problems import EnsembleProblem, adapt, make_prob_compatible
@@ -85,12 +85,14 @@ def _ctx(i: int, seed: Optional[int]) -> EnsembleContext:
 def solve(ensembleprob: EnsembleProblem, alg: GPUODEAlgorithm, ensemblealg: EnsembleGPUKernel, *,
           trajectories: int, batch_size: Optional[int] = None, adaptive: bool = True, seed=None,
           unstable_check=None, **kwargs) -> EnsembleSolution:
-    """solve.jl:1-95. ``adaptive`` selects vectorized_asolve (default) or vectorized_solve."""
+    """solve.jl:1-95. ``adaptive`` selects vectorized_asolve (default) or vectorized_solve. The SDE
+    methods (GPUEM, GPUSIEA) are fixed-step only: they need ``adaptive=False`` and ``dt``, as in the
+    reference, whose adaptive SDE entry point raises."""
     if not isinstance(ensemblealg, EnsembleGPUKernel):
         raise TypeError("only EnsembleGPUKernel is provided")
-    if not isinstance(alg, GPUODEAlgorithm):
+    if not isinstance(alg, (GPUODEAlgorithm, GPUSDEAlgorithm)):
         raise TypeError("alg must be a GPU ODE algorithm (GPUTsit5, GPUVern7, GPUVern9, GPURosenbrock23, GPURodas4, "
-                        "GPURodas5P, GPUKvaerno3, GPUKvaerno5)")
+                        "GPURodas5P, GPUKvaerno3, GPUKvaerno5) or a GPU SDE algorithm (GPUEM, GPUSIEA)")
     if trajectories < 1:
         raise ValueError("trajectories must be >= 1")
     batch_size = trajectories if batch_size is None else int(batch_size)
@@ -123,6 +125,8 @@ def batch_solve(ensembleprob, alg, ensemblealg, I, adaptive, seed=None, **kwargs
                              "length. Use saveats of same size only.")
     saveat = inner_saveat if inner_saveat is not None else kwargs.pop("saveat", None)
     kwargs.pop("saveat", None)
+    if isinstance(alg, GPUSDEAlgorithm):
+        kwargs["traj_base"] = I[0] - 1  # noise streams numbered by ensemble index, whatever the batching
     solts, solus, rc = batch_solve_up_kernel(ensembleprob, probs, alg, ensemblealg, I, adaptive,
                                              saveat=saveat, **kwargs)
     out = []
@@ -148,6 +152,11 @@ def batch_solve_up_kernel(ensembleprob, probs, alg, ensemblealg, I, adaptive, **
     ``merge_callbacks`` is set (ensemblegpuarray/callbacks.jl:61-74), converted to GPU callbacks."""
     kwargs["callback"] = merged_callback(probs[0].kwargs.get("callback"), kwargs.get("callback"),
                                          bool(kwargs.pop("merge_callbacks", False)))
+    if kwargs["callback"] is None:
+        kwargs.pop("callback")
+    if adaptive and isinstance(alg, GPUSDEAlgorithm):
+        # the adaptive SDE entry point raises (lowerlevel_solve.jl:349-356); say so before any device work
+        raise NotImplementedError("Adaptive time-stepping is not supported yet with GPUEM.")
     dev = ensemblealg.dev
     gprobs = adapt(dev, probs)
     if adaptive:

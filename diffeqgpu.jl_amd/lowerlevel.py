@@ -1,8 +1,8 @@
 """Lower-level API: ``vectorized_solve`` / ``vectorized_asolve``.
 
-Host half of src/ensemblegpukernel/lowerlevel_solve.jl:53-131 (fixed step) and :253-346
-(adaptive): saveat normalisation, output sizing and the t0 prefill of ``ts``. The device
-half is one call of ``deg_solve`` in libdeghip.so.
+Host half of src/ensemblegpukernel/lowerlevel_solve.jl:53-131 (fixed step), :134-199 (fixed-step
+SDE) and :253-356 (adaptive; none for SDEs): saveat normalisation, output sizing and the t0
+prefill of ``ts``. The device half is one call of ``deg_solve`` in libdeghip.so.
 
 Layout. The device writes structure-of-arrays: ``ts`` is a (nsave, N) tensor and the state
 output is stored as (nsave, n, N). ``us`` is returned as the (nsave, N, n) view of that
@@ -17,7 +17,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from .algorithms import GPUODEAlgorithm, GPUODEImplicitAlgorithm, HIPDevice
+from .algorithms import GPUODEAlgorithm, GPUODEImplicitAlgorithm, GPUSDEAlgorithm, GPUSIEA, HIPDevice
 from .callbacks import emit_callbacks, gpu_callback_set
 from .problems import HIPProbVector, ODEProblem, get_backend
 from .ranges import JlRange, colon, linrange
@@ -98,7 +98,7 @@ def normalize_saveat_asolve(saveat, tspan) -> Optional[np.ndarray]:
 def _rep_tspan(prob) -> tuple:
     if isinstance(prob, ODEProblem) or hasattr(prob, "tspan"):
         return (np.float32(prob.tspan[0]), np.float32(prob.tspan[1]))
-    raise TypeError("prob must be an ODEProblem")
+    raise TypeError("prob must be an ODEProblem or SDEProblem")
 
 
 def _callback_set(callback):
@@ -119,38 +119,55 @@ def _tstops(tstops) -> Optional[np.ndarray]:
     return a if a.size else None
 
 
-def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, abstol, reltol, saveat_np,
+def _launch(probs: HIPProbVector, alg, adaptive: bool, *, dt, abstol, reltol, saveat_np,
             nsave, save_everystep, t0_fill, timing=False, stream=None, mlp=None, steplog=None, block_size=None,
-            grid_blocks=0, outputs=None, order=None, mlp_every=1, callback=None, tstops_np=None):
+            grid_blocks=0, outputs=None, order=None, mlp_every=1, callback=None, tstops_np=None, traj_base=0):
     """Launch one solve. ``outputs`` (optional) supplies preallocated device tensors
     {"ts", "us_soa", "retcode", "naccept", "nreject", "saveat"} so that the call allocates and
     copies nothing (required inside a graph capture). ``order`` (optional, int32 device tensor)
     is the start order of the persistent queue, e.g. from ``order_by_span``. ``callback`` is a
-    CallbackSet of GPU callbacks or None; ``tstops_np`` a Float32 vector or None."""
+    CallbackSet of GPU callbacks or None; ``tstops_np`` a Float32 vector or None. For an SDE
+    batch, ``traj_base`` is the ensemble index of its first trajectory (its noise stream)."""
     import torch
 
-    if not isinstance(alg, GPUODEAlgorithm):
+    sde = isinstance(alg, GPUSDEAlgorithm)
+    if not (sde or isinstance(alg, GPUODEAlgorithm)):
         raise TypeError(f"unsupported algorithm {alg!r}")
+    if sde != probs.is_sde:
+        raise TypeError("SDE algorithms solve SDE problems and ODE algorithms ODE problems")
     dev: HIPDevice = probs.dev
-    flags = dev.flags(adaptive)
-    if mlp is not None:
-        flags |= _lib.F_LEARNED
-    if steplog is not None:
-        flags |= _lib.F_STEPLOG
-    if saveat_np is not None and len(saveat_np) > _lib.SAVEAT_LDS_CAP:
-        flags |= _lib.F_SAVEAT_GLOBAL
-    if tstops_np is not None:
-        flags |= _lib.F_TSTOPS
-    if callback is not None:
-        flags |= _lib.F_CALLBACKS
-    if flags & (_lib.F_TSTOPS | _lib.F_CALLBACKS):
-        if flags & (_lib.F_LEARNED | _lib.F_STEPLOG | _lib.F_COMPACT | _lib.F_BATCH_SAVES):
-            raise ValueError("tstops and callbacks do not combine with the learned proposer, step logs, "
-                             "tail compaction or batched saves")
-    with_jac = isinstance(alg, GPUODEImplicitAlgorithm)
-    src, _flops = probs.f.hip_source(probs.n, probs.np, with_jac)
-    if callback is not None:
-        src = src + emit_callbacks(callback, probs.n, probs.np)
+    if sde:
+        if adaptive:
+            raise NotImplementedError("Adaptive time-stepping is not supported yet with GPUEM.")
+        if mlp is not None or steplog is not None or order is not None:
+            raise ValueError("the learned proposer, step logs and start orders are ODE-kernel options")
+        if isinstance(alg, GPUSIEA) and probs.noise_m is not None:
+            raise ValueError("The algorithm is not compatible with the chosen noise type. Please see the "
+                             "documentation on the solver methods")
+        if int(traj_base) < 0:
+            raise ValueError("traj_base must be >= 0")
+        flags = dev.sde_flags()
+        src, _flops = probs.f.hip_source(probs.n, probs.np, probs.noise_m)
+    else:
+        flags = dev.flags(adaptive)
+        if mlp is not None:
+            flags |= _lib.F_LEARNED
+        if steplog is not None:
+            flags |= _lib.F_STEPLOG
+        if saveat_np is not None and len(saveat_np) > _lib.SAVEAT_LDS_CAP:
+            flags |= _lib.F_SAVEAT_GLOBAL
+        if tstops_np is not None:
+            flags |= _lib.F_TSTOPS
+        if callback is not None:
+            flags |= _lib.F_CALLBACKS
+        if flags & (_lib.F_TSTOPS | _lib.F_CALLBACKS):
+            if flags & (_lib.F_LEARNED | _lib.F_STEPLOG | _lib.F_COMPACT | _lib.F_BATCH_SAVES):
+                raise ValueError("tstops and callbacks do not combine with the learned proposer, step logs, "
+                                 "tail compaction or batched saves")
+        with_jac = isinstance(alg, GPUODEImplicitAlgorithm)
+        src, _flops = probs.f.hip_source(probs.n, probs.np, with_jac)
+        if callback is not None:
+            src = src + emit_callbacks(callback, probs.n, probs.np)
     ctx = _lib.context(dev.device)
     kern = ctx.kernel(src, probs.n, probs.np, alg.method, flags)
     device = probs.u0.device
@@ -243,6 +260,10 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
             raise ValueError("preallocated tstops has the wrong length")
         a.tstops = tst.data_ptr()
         a.ntstops = int(tst.numel())
+    if sde:
+        a.seeds = probs.seeds.data_ptr() if probs.seeds is not None else None
+        a.seed = int(probs.seed) & 0xFFFFFFFFFFFFFFFF
+        a.traj_base = int(traj_base)
     kern.solve(a)
     if timing:
         ms = float(box.value)
@@ -260,16 +281,21 @@ def _launch(probs: HIPProbVector, alg: GPUODEAlgorithm, adaptive: bool, *, dt, a
 
 def vectorized_solve(probs, prob, alg, *, dt, saveat=None, save_everystep=True, debug=False, callback=None,
                      tstops=None, timing=False, **kwargs):
-    """Fixed-step ensemble solve (lowerlevel_solve.jl:53-131). Returns ``(ts, us)``.
+    """Fixed-step ensemble solve (lowerlevel_solve.jl:53-131; SDEs :134-199). Returns ``(ts, us)``.
 
     ``callback``: GPU callbacks (a callback or CallbackSet; SciML callbacks are converted).
     ``tstops``: stopping times shared by all trajectories; each adds a row to the every-step
-    output unless it lies on the t0:dt:tf grid (lowerlevel_solve.jl:73-77)."""
+    output unless it lies on the t0:dt:tf grid (lowerlevel_solve.jl:73-77).
+    SDE batches (GPUEM, GPUSIEA) take neither; ``traj_base`` (keyword) numbers their noise
+    streams from the ensemble index of the first trajectory."""
     get_backend(probs)
     t0, tf = _rep_tspan(prob)
     dt32 = np.float32(dt)
     if not dt32 > 0:
         raise ValueError("dt must be positive for a fixed-step solve")
+    if isinstance(alg, GPUSDEAlgorithm):
+        return _vectorized_solve_sde(probs, alg, t0, tf, dt32, saveat, save_everystep, callback, tstops, timing,
+                                     **kwargs)
     cbs = _callback_set(callback)
     tst = _tstops(tstops)
     sv = normalize_saveat_solve(saveat, (t0, tf))
@@ -289,10 +315,28 @@ def vectorized_solve(probs, prob, alg, *, dt, saveat=None, save_everystep=True, 
                    tstops_np=tst, **kwargs)
 
 
+def _vectorized_solve_sde(probs, alg, t0, tf, dt32, saveat, save_everystep, callback, tstops, timing, **kwargs):
+    """lowerlevel_solve.jl:134-199: output sizing and saveat normalisation of the SDE kernels."""
+    if callback is not None or tstops is not None:
+        raise NotImplementedError("callbacks and tstops are not provided for SDE solves")
+    sv = normalize_saveat_solve(saveat, (t0, tf))
+    if sv is None:
+        nsave = len(colon(t0, dt32, tf)) if save_everystep else 2
+    else:
+        if len(sv) == 0:
+            raise ValueError("saveat is empty")
+        nsave = len(sv)
+    return _launch(probs, alg, False, dt=dt32, abstol=0.0, reltol=0.0, saveat_np=sv, nsave=nsave,
+                   save_everystep=save_everystep and sv is None, t0_fill=t0, timing=timing, **kwargs)
+
+
 def vectorized_asolve(probs, prob, alg, *, dt=0.1, saveat=None, save_everystep=False, abstol=1.0e-6,
                       reltol=1.0e-3, debug=False, callback=None, tstops=None, timing=False, **kwargs):
     """Adaptive ensemble solve (lowerlevel_solve.jl:253-346). Returns ``(ts, us)``.
-    ``callback`` and ``tstops`` as for vectorized_solve (tstops add no output rows here)."""
+    ``callback`` and ``tstops`` as for vectorized_solve (tstops add no output rows here).
+    SDE methods have no adaptive solve (lowerlevel_solve.jl:349-356)."""
+    if isinstance(alg, GPUSDEAlgorithm) or getattr(probs, "is_sde", False):
+        raise NotImplementedError("Adaptive time-stepping is not supported yet with GPUEM.")
     get_backend(probs)
     cbs = _callback_set(callback)
     tst = _tstops(tstops)

@@ -7,7 +7,7 @@ import multiprocessing as mp
 
 from . import _lib
 from .algorithms import HIPDevice
-from .systems import SYSTEMS
+from .systems import SDE_SYSTEMS, SYSTEMS
 
 _METHODS = (_lib.DEG_TSIT5, _lib.DEG_VERN7, _lib.DEG_RB23)
 
@@ -47,6 +47,11 @@ def variants():
                 base = HIPDevice().flags(True)
                 for extra in (_lib.F_LEARNED, _lib.F_STEPLOG, _lib.F_LEARNED | _lib.F_STEPLOG):
                     out.append((name, src, n, np_, m, base | extra))
+    for name, (f, n, np_, nm, _o) in SDE_SYSTEMS.items():
+        src, _ = f.hip_source(n, np_, nm)
+        for m in ((_lib.DEG_EM,) if nm is not None else (_lib.DEG_EM, _lib.DEG_SIEA)):
+            for dev in (HIPDevice(), HIPDevice(contract="fast")):
+                out.append((name, src, n, np_, m, dev.sde_flags()))
     return out
 
 

@@ -1,9 +1,10 @@
 """Problem types and their device packing.
 
 Mirrors the SciMLBase objects the reference path consumes: ``ODEFunction`` (with optional
-``jac``/``tgrad``), out-of-place ``ODEProblem``, ``remake``, ``EnsembleProblem``, and
+``jac``/``tgrad``), out-of-place ``ODEProblem``, ``SDEFunction`` / ``SDEProblem`` (drift, diffusion,
+``seed``, ``noise_rate_prototype``), ``remake``, ``EnsembleProblem``, and
 DiffEqGPU's ``make_prob_compatible`` (src/utils.jl:36-40), which turns a problem into an
-isbits ``ImmutableODEProblem``. ``adapt(HIPDevice(), probs)`` plays the role of
+isbits ``ImmutableODEProblem`` (an ``ImmutableSDEProblem`` for an SDE). ``adapt(HIPDevice(), probs)`` plays the role of
 ``adapt(backend, probs)`` (solve.jl:399-400): it packs the batch into structure-of-arrays
 device tensors (``HIPProbVector``), the layout the HIP kernels read coalesced.
 """
@@ -55,6 +56,33 @@ class ODEFunction:
         return self._src_cache[key]
 
 
+class SDEFunction:
+    """Out-of-place drift ``f(u, p, t)`` and diffusion ``g(u, p, t)`` of an Ito SDE
+    du = f dt + g dW. For diagonal noise g returns n entries; for general noise the n x m
+    matrix (n rows of m entries, or flat row-major)."""
+
+    def __init__(self, f: Callable, g: Optional[Callable] = None):
+        if isinstance(f, SDEFunction):
+            g = g or f.g
+            f = f.f
+        if g is None:
+            raise TypeError("SDEFunction needs a diffusion g")
+        self.f = f
+        self.g = g
+        self._src_cache = {}
+
+    def __call__(self, u, p, t):
+        return self.f(u, p, t)
+
+    def hip_source(self, n: int, np_: int, m: Optional[int]) -> tuple[str, int]:
+        """Traced HIP C for deg_rhs and deg_noise (with DEG_NM / DEG_DIAG_NOISE); m None is
+        diagonal noise."""
+        key = (n, np_, m)
+        if key not in self._src_cache:
+            self._src_cache[key] = tracer.emit_noise(self.f, self.g, n, np_, m)
+        return self._src_cache[key]
+
+
 def _as_f32_vec(x) -> np.ndarray:
     if x is None:
         return np.zeros(0, np.float32)
@@ -81,10 +109,60 @@ class ODEProblem:
         return f"ODEProblem(u0={self.u0}, tspan={self.tspan}, p={self.p})"
 
 
-def remake(prob: ODEProblem, *, u0=None, p=None, tspan=None, f=None, **kwargs) -> ODEProblem:
+class SDEProblem:
+    """``SDEProblem(f, g, u0, tspan, p; seed = 0, noise_rate_prototype = nothing, kwargs...)``
+    (out-of-place). ``noise_rate_prototype`` None is diagonal noise; an n x m array selects general
+    noise with m Wiener processes (SciMLBase.is_diagonal_noise). ``seed`` keys the trajectory's
+    noise streams (the reference calls Random.seed!(prob.seed) in its kernels,
+    gpu_em_perform_step.jl:8)."""
+
+    def __init__(self, f, g=None, u0=None, tspan=None, p=None, *, seed: int = 0, noise_rate_prototype=None,
+                 **kwargs):
+        self.f = f if isinstance(f, SDEFunction) else SDEFunction(f, g)
+        if g is not None and isinstance(f, SDEFunction):
+            self.f = SDEFunction(f.f, g)
+        self.u0 = _as_f32_vec(u0)
+        self.tspan = _as_tspan(tspan)
+        self.p = _as_f32_vec(p)
+        self.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        if noise_rate_prototype is not None:
+            proto = np.asarray(noise_rate_prototype)
+            if proto.ndim != 2 or proto.shape[0] != self.u0.shape[0]:
+                raise ValueError("noise_rate_prototype must be an n x m matrix")
+            self.noise_rate_prototype = proto
+        else:
+            self.noise_rate_prototype = None
+        self.kwargs = dict(kwargs)
+
+    @property
+    def g(self):
+        return self.f.g
+
+    @property
+    def noise_m(self) -> Optional[int]:
+        """Columns of the noise matrix, or None for diagonal noise."""
+        return None if self.noise_rate_prototype is None else int(self.noise_rate_prototype.shape[1])
+
+    def __repr__(self):
+        return f"SDEProblem(u0={self.u0}, tspan={self.tspan}, p={self.p}, seed={self.seed})"
+
+
+def is_diagonal_noise(prob) -> bool:
+    """SciMLBase.is_diagonal_noise: no noise_rate_prototype."""
+    return getattr(prob, "noise_rate_prototype", None) is None
+
+
+def remake(prob, *, u0=None, p=None, tspan=None, f=None, seed=None, **kwargs):
     """SciMLBase.remake for the fields the ensemble path uses."""
     kw = dict(prob.kwargs)
     kw.update(kwargs)
+    if isinstance(prob, (SDEProblem, ImmutableSDEProblem)):
+        return SDEProblem(f if f is not None else prob.f, None, prob.u0 if u0 is None else u0,
+                          prob.tspan if tspan is None else tspan, prob.p if p is None else p,
+                          seed=prob.seed if seed is None else seed, noise_rate_prototype=prob.noise_rate_prototype,
+                          **kw)
+    if seed is not None:
+        raise TypeError("an ODEProblem has no seed")
     return ODEProblem(f if f is not None else prob.f, prob.u0 if u0 is None else u0,
                       prob.tspan if tspan is None else tspan, prob.p if p is None else p, **kw)
 
@@ -100,12 +178,32 @@ class ImmutableODEProblem:
     kwargs: dict = field(default_factory=dict)
 
 
-def make_prob_compatible(prob) -> ImmutableODEProblem:
-    """src/utils.jl:36-40."""
-    if isinstance(prob, ImmutableODEProblem):
+@dataclass(frozen=True)
+class ImmutableSDEProblem:
+    """The isbits form of an SDEProblem a kernel receives (one per trajectory)."""
+
+    f: SDEFunction
+    u0: np.ndarray
+    tspan: tuple
+    p: np.ndarray
+    seed: int = 0
+    noise_rate_prototype: Any = None
+    kwargs: dict = field(default_factory=dict)
+
+    @property
+    def noise_m(self) -> Optional[int]:
+        return None if self.noise_rate_prototype is None else int(np.shape(self.noise_rate_prototype)[1])
+
+
+def make_prob_compatible(prob):
+    """src/utils.jl:36-40 (an SDEProblem is passed through in its isbits form)."""
+    if isinstance(prob, (ImmutableODEProblem, ImmutableSDEProblem)):
         return prob
+    if isinstance(prob, SDEProblem):
+        return ImmutableSDEProblem(prob.f, prob.u0.copy(), prob.tspan, prob.p.copy(), prob.seed,
+                                   prob.noise_rate_prototype, dict(prob.kwargs))
     if not isinstance(prob, ODEProblem):
-        raise TypeError("make_prob_compatible expects an ODEProblem")
+        raise TypeError("make_prob_compatible expects an ODEProblem or SDEProblem")
     return ImmutableODEProblem(prob.f, prob.u0.copy(), prob.tspan, prob.p.copy(), dict(prob.kwargs))
 
 
@@ -120,7 +218,7 @@ def _default_output_func(sol, ctx):
 class EnsembleProblem:
     """SciMLBase.EnsembleProblem (prob_func(prob, ctx), output_func(sol, ctx), safetycopy)."""
 
-    def __init__(self, prob: ODEProblem, prob_func: Callable = _default_prob_func,
+    def __init__(self, prob, prob_func: Callable = _default_prob_func,
                  output_func: Callable = _default_output_func, reduction=None, u_init=None,
                  safetycopy: bool = True):
         self.prob = prob
@@ -139,12 +237,16 @@ class HIPProbVector:
     u0[c, i] and p[j, i] are float32 tensors of shape (n, N) and (np, N) on the HIP device;
     t0s/tfs are (N,) tensors when the time spans differ, otherwise None and the shared span
     is ``tspan``. ``dev`` is the HIPDevice the batch lives on (``get_backend(probs)``).
+    An SDE batch (``f`` an SDEFunction) also carries ``seed`` (shared) or ``seeds`` (an (N,) int64
+    tensor holding the 64-bit seeds' bits) and ``noise_m`` (None: diagonal noise).
     """
 
-    def __init__(self, f: ODEFunction, u0, p, tspan, dev: HIPDevice, t0s=None, tfs=None):
+    def __init__(self, f, u0, p, tspan, dev: HIPDevice, t0s=None, tfs=None, *, seed: int = 0, seeds=None,
+                 noise_m: Optional[int] = None):
         import torch
 
-        self.f = f if isinstance(f, ODEFunction) else ODEFunction(f)
+        self.f = f if isinstance(f, (ODEFunction, SDEFunction)) else ODEFunction(f)
+        self.is_sde = isinstance(self.f, SDEFunction)
         self.dev = dev
         device = torch.device("cuda", dev.device)
 
@@ -173,13 +275,25 @@ class HIPProbVector:
         for name, a in (("t0s", self.t0s), ("tfs", self.tfs)):
             if a is not None and (a.dim() != 1 or a.shape[0] != self.N):
                 raise ValueError(f"{name} must have shape (N,)")
+        self.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        self.noise_m = None if noise_m is None else int(noise_m)
+        self.seeds = None
+        if seeds is not None:
+            if not self.is_sde:
+                raise ValueError("seeds belong to an SDE batch")
+            a = np.asarray(seeds)
+            if a.shape != (self.N,):
+                raise ValueError("seeds must have shape (N,)")
+            bits = np.ascontiguousarray(a.astype(np.uint64)).view(np.int64)
+            self.seeds = torch.as_tensor(bits).to(device)
 
     def __len__(self):
         return self.N
 
     @classmethod
-    def from_arrays(cls, f, u0, p, tspan, dev: HIPDevice | None = None, t0s=None, tfs=None):
-        return cls(f, u0, p, tspan, dev or HIPDevice(), t0s, tfs)
+    def from_arrays(cls, f, u0, p, tspan, dev: HIPDevice | None = None, t0s=None, tfs=None, **sde):
+        """``sde``: seed / seeds / noise_m for an SDE batch (f an SDEFunction)."""
+        return cls(f, u0, p, tspan, dev or HIPDevice(), t0s, tfs, **sde)
 
 
 def pack_host(probs: Sequence[ImmutableODEProblem]):
@@ -209,6 +323,16 @@ def adapt(dev: HIPDevice, probs) -> HIPProbVector:
         return probs
     probs = [make_prob_compatible(p) for p in probs]
     u0, p, t0s, tfs, tspan = pack_host(probs)
+    if isinstance(probs[0], ImmutableSDEProblem):
+        if not all(isinstance(q, ImmutableSDEProblem) for q in probs):
+            raise TypeError("a batch mixes ODE and SDE problems")
+        ms = {q.noise_m for q in probs}
+        if len(ms) > 1:
+            raise ValueError("all problems of a batch must have the same noise shape")
+        seeds = np.array([q.seed for q in probs], dtype=np.uint64)
+        same = bool(np.all(seeds == seeds[0]))
+        return HIPProbVector(probs[0].f, u0, p, tspan, dev, t0s, tfs, seed=int(seeds[0]),
+                             seeds=None if same else seeds, noise_m=probs[0].noise_m)
     return HIPProbVector(probs[0].f, u0, p, tspan, dev, t0s, tfs)
 
 

@@ -11,8 +11,16 @@ same association order (u and p are 0-based here, 1-based in the Julia sources):
   tdep       u' = p t u
   rot        u1' = -p u2, u2' = p u1
   ball       u1' = u2, u2' = -p1 (test/gpu_kernel_de/gpu_ode_continuous_callbacks.jl:6-10)
+and the SDE problems (drift f, diffusion g; SDE_SYSTEMS):
+  gbm        f = p1 u, g = p2 u (test/gpu_kernel_de/gpu_sde_convergence.jl:8-10)
+  gbm1       f = u, g = u (test/gpu_kernel_de/gpu_sde_regression.jl:13-15)
+  lorenz     Lorenz drift, g = (3, 3, 3) (gpu_sde_regression.jl:48-58, parameters as p)
+  nondiag    f = 1.01 u, 2 x 4 noise matrix (gpu_sde_regression.jl:102-117; SMatrix{2,4} fills
+             its eight arguments column-major, so row 1 is (0.3 u1, 0.9 u1, 1.2 u2, 0.3 u2))
+  ou         Ornstein-Uhlenbeck f = p1 (p2 - u), g = p3
+  bm         f = 0, g = p1 (scaled Brownian motion)
 """
-from .problems import ODEFunction
+from .problems import ODEFunction, SDEFunction
 from .tracer import SVector
 
 
@@ -96,4 +104,68 @@ SYSTEMS = {
     "tdep": (ODEFunction(tdep), 1, 1, 6),
     "rot": (ODEFunction(rot), 2, 1, 7),
     "ball": (ODEFunction(ball, jac=ball_jac), 2, 1, 8),
+}
+
+
+def gbm_f(u, p, t):
+    return SVector(p[0] * u[0])
+
+
+def gbm_g(u, p, t):
+    return SVector(p[1] * u[0])
+
+
+def gbm1_f(u, p, t):
+    return SVector(u[0])
+
+
+def gbm1_g(u, p, t):
+    return SVector(u[0])
+
+
+def sde_lorenz_g(u, p, t):
+    return SVector(3.0, 3.0, 3.0)
+
+
+def nondiag_f(u, p, t):
+    return SVector(1.01 * u[0], 1.01 * u[1])
+
+
+def nondiag_g(u, p, t):
+    du1_1 = 0.3 * u[0]
+    du1_2 = 0.6 * u[0]
+    du1_3 = 0.9 * u[0]
+    du1_4 = 0.12 * u[0]
+    du2_1 = 1.2 * u[1]
+    du2_2 = 0.2 * u[1]
+    du2_3 = 0.3 * u[1]
+    du2_4 = 1.8 * u[1]
+    # SMatrix{2,4}(du1_1, du1_2, ..., du2_4) in row-major rows
+    return [[du1_1, du1_3, du2_1, du2_3], [du1_2, du1_4, du2_2, du2_4]]
+
+
+def ou_f(u, p, t):
+    return SVector(p[0] * (p[1] - u[0]))
+
+
+def ou_g(u, p, t):
+    return SVector(p[2])
+
+
+def bm_f(u, p, t):
+    return SVector(0.0)
+
+
+def bm_g(u, p, t):
+    return SVector(p[0])
+
+
+# (SDEFunction, n, np, noise columns or None for diagonal noise, oracle system name)
+SDE_SYSTEMS = {
+    "gbm": (SDEFunction(gbm_f, gbm_g), 1, 2, None, "gbm"),
+    "gbm1": (SDEFunction(gbm1_f, gbm1_g), 1, 0, None, "gbm1"),
+    "lorenz": (SDEFunction(lorenz, sde_lorenz_g), 3, 3, None, "lorenz"),
+    "nondiag": (SDEFunction(nondiag_f, nondiag_g), 2, 0, 4, "nondiag"),
+    "ou": (SDEFunction(ou_f, ou_g), 1, 3, None, "ou"),
+    "bm": (SDEFunction(bm_f, bm_g), 1, 1, None, "bm"),
 }

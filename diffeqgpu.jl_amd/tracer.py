@@ -20,7 +20,7 @@ from __future__ import annotations
 
 import math as _pymath
 import struct
-from typing import Callable, Dict, List, Sequence, Tuple
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 
 class TraceError(TypeError):
@@ -625,6 +625,47 @@ def emit_rhs(f: Callable, n: int, np_: int, jac: Callable | None = None, tgrad: 
                 "                                          float* __restrict__ dT) {",
                 "    (void)u; (void)p; (void)t;", *tbody, "}"]
     return "\n".join(src) + "\n", flops
+
+
+def emit_noise(f: Callable, g: Callable, n: int, np_: int, m: Optional[int] = None) -> Tuple[str, int]:
+    """HIP C source for an SDE: the drift deg_rhs, the diffusion deg_noise and the noise-shape
+    macros DEG_NM / DEG_DIAG_NOISE, plus the flop count of one f and one g evaluation.
+
+    ``m`` None is diagonal noise: g returns n entries (multiplied elementwise with the n Wiener
+    increments). Otherwise g returns the n x m noise matrix, as n rows of m entries or a flat
+    row-major sequence (the product with the m increments sums over columns in order)."""
+    src, flops = emit_rhs(f, n, np_)
+    diag = m is None
+    mm = n if diag else int(m)
+    if mm < 1:
+        raise TraceError("the noise matrix needs at least one column")
+    g_, u, p, t = _new_inputs(n, np_)
+    out = g(SymVec(u, "u"), SymVec(p, "p"), t)
+    if isinstance(out, (int, float, Node)):
+        out = (out,)
+    if diag:
+        vals = list(out)
+        if len(vals) != n:
+            raise TraceError(f"diagonal noise: g must return {n} entries, got {len(vals)}")
+    else:
+        if len(out) == n and all(isinstance(r, (list, tuple)) for r in out):
+            if any(len(r) != mm for r in out):
+                raise TraceError(f"g must return {n} rows of {mm} entries")
+            vals = [x for r in out for x in r]
+        else:
+            vals = list(out)
+        if len(vals) != n * mm:
+            raise TraceError(f"g must return {n}x{mm} entries, got {len(vals)}")
+    try:
+        gouts = [g_.lift(x) for x in vals]
+    except TypeError as e:
+        raise TraceError(f"g must return scalars: {e}") from e
+    gbody, gflops = _emit_body(g_, gouts, "g")
+    head = [f"#define DEG_NM {mm}", f"#define DEG_DIAG_NOISE {1 if diag else 0}"]
+    noise = ["__device__ __forceinline__ void deg_noise(const float* __restrict__ u, const float* __restrict__ p, float t,",
+             "                                          float* __restrict__ g) {",
+             "    (void)u; (void)p; (void)t;", *gbody, "}"]
+    return "\n".join(head) + "\n" + src + "\n".join(noise) + "\n", flops + gflops
 
 
 def _flatten_matrix(out, n):

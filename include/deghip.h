@@ -43,6 +43,12 @@ extern "C" {
 #define DEG_RODAS5P 5
 #define DEG_KVAERNO3 6
 #define DEG_KVAERNO5 7
+/* fixed-step SDE methods GPUEM, GPUSIEA (gpukernel_algorithms.jl:243, 266; kernels
+ * perform_step/gpu_em_perform_step.jl:1-74, gpu_siea_perform_step.jl:49-139). Their kernels take
+ * DEG_F_PARITY and the register cap only; the traced source also defines the diffusion (see
+ * deg_kernel_compile) and the Wiener increments come from deg_draw_normal's generator. */
+#define DEG_EM 8
+#define DEG_SIEA 9
 
 /* compile flags (one code object per (rhs, n, np, method, flags)) */
 #define DEG_F_ADAPTIVE (1u << 0)   /* PI-controlled adaptive stepping (vectorized_asolve) */
@@ -125,6 +131,10 @@ typedef struct deg_solve_args {
                                 (device); the `tstops` keyword of vectorized_solve / vectorized_asolve */
     int32_t ntstops;
     int32_t reserved0;
+    const uint64_t *seeds;   /* DEG_EM / DEG_SIEA: [stride] per-trajectory seeds (SDEProblem `seed`), or NULL */
+    uint64_t seed;           /* DEG_EM / DEG_SIEA: the seed of every trajectory when seeds is NULL */
+    int64_t traj_base;       /* DEG_EM / DEG_SIEA: ensemble index of column 0; the noise of column i is the
+                                stream traj_base + i, so a chunked ensemble draws what one launch would */
 } deg_solve_args;
 
 /* Context: one device (launches go to the stream given per call). */
@@ -152,6 +162,12 @@ int deg_set_cache_dir(const char *dir);
  *                                  bits 2-3 rootfind (0 NoRootFind, 1 LeftRootFind, 2 RightRootFind)
  *   float deg_ccb_abstol(int j), deg_ccb_nudge(int j) (repeat_nudge), deg_ccb_dtrelax(int j);
  * an affect updates u (and p) in place and returns DEG_AFF_* bits.
+ * For DEG_EM and DEG_SIEA, deg_rhs is the drift f and the source also defines the diffusion g
+ * (SDEProblem(f, g, ...), the noise shape from noise_rate_prototype):
+ *   #define DEG_NM <noise columns m>   #define DEG_DIAG_NOISE <1 diagonal, 0 general>
+ *   void deg_noise(const float* u, const float* p, float t, float* g);
+ * writing n entries (diagonal noise, multiplied elementwise with the n increments) or the n x m
+ * matrix row-major g[r*m + c]. DEG_SIEA requires diagonal noise (lowerlevel_solve.jl:185-186).
  * Replaces the compile of ode_solve_kernel / ode_asolve_kernel for one problem type. */
 #define DEG_AFF_TERMINATE 1u /* terminate!(integrator) */
 #define DEG_AFF_KEEP_FSAL 2u /* u_modified!(integrator, false) */
@@ -196,6 +212,16 @@ uint32_t deg_solve_args_size(void);
  * re-draws of an EnsembleProblem (src/solve.jl:187-202) for sweeps that are graph-captured. */
 int deg_draw_uniform(float *out, int64_t n, int64_t id_offset, uint64_t seed, int32_t rows, int32_t row,
                      float lo, float scale, void *stream);
+
+/* Standard normal draws of the SDE kernels' generator: out[c * n + i] = normal c (0 <= c < m) of
+ * step `step` of noise stream id_offset + i under `seed`, for i in [0, n) -- the increment
+ * sqrt(dt) * out of that step in a DEG_EM / DEG_SIEA solve (traj_base + column = id_offset + i).
+ * Philox4x32-10 with key = seed and counter = (step, c / 4, id low, id high); Box-Muller on word
+ * pairs with a deterministic ln and sincos. Asynchronous on `stream`. Takes the place of the
+ * device `randn` of em_kernel / siea_kernel (gpu_em_perform_step.jl:48-53), exposed for tests
+ * and for callers that want the same Wiener paths outside a solve. */
+int deg_draw_normal(float *out, int64_t n, int64_t id_offset, uint64_t seed, uint32_t step, int32_t m,
+                    void *stream);
 
 /* Longest-first order of trajectories for the persistent queue: writes order[0..n) with a
  * permutation of 0..n-1 sorted by descending span tfs[i] - t0s[i] (NULL arrays: the shared t0/tf),

@@ -105,6 +105,21 @@ def lib():
         L.deg_oracle_tsit5_r.restype = ctypes.c_double
         L.deg_oracle_vern9_dense.argtypes = [_dp, _dp, _dp]
         L.deg_oracle_vern9_btheta.argtypes = [ctypes.c_float, _fp]
+        _u32p = ctypes.POINTER(ctypes.c_uint32)
+        _u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.deg_oracle_philox.argtypes = [_u32p, _u32p, _u32p]
+        L.deg_oracle_bm_m2log.argtypes = [ctypes.c_uint32]
+        L.deg_oracle_bm_m2log.restype = ctypes.c_float
+        L.deg_oracle_bm_sincos.argtypes = [ctypes.c_uint32, _fp, _fp]
+        L.deg_oracle_normals.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint32,
+                                         ctypes.c_int, _fp]
+        L.deg_oracle_sde_dims.argtypes = [ctypes.c_int, _ip, _ip, _ip]
+        L.deg_oracle_sde_dims.restype = ctypes.c_int
+        L.deg_oracle_sde_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _fp, _fp, _fp, _fp,
+                                           ctypes.c_float, ctypes.c_float, ctypes.c_float, _u64p, ctypes.c_uint64,
+                                           ctypes.c_int64, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp, _fp,
+                                           _ip, _ip, _dp]
+        L.deg_oracle_sde_solve.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -219,4 +234,86 @@ def vern9_btheta(theta: float) -> np.ndarray:
     """The Float32 weights b_i(theta) (index 1..26, 0 unused) the oracle's Vern9 dense output uses."""
     out = np.zeros(27, np.float32)
     lib().deg_oracle_vern9_btheta(float(np.float32(theta)), _p(out, _fp))
+    return out
+
+
+# ---- SDE kernels (deg_oracle_sde.c) ----
+SDE_SYSTEM_IDS = {"gbm": 0, "gbm1": 1, "lorenz": 2, "nondiag": 3, "ou": 4, "bm": 5}
+SDE_NP = {"gbm": 2, "gbm1": 0, "lorenz": 3, "nondiag": 0, "ou": 3, "bm": 1}
+SDE_METHOD_IDS = {"em": 8, "siea": 9}
+
+
+def philox(ctr, key) -> np.ndarray:
+    """Philox4x32-10 of a 4-word counter and a 2-word key (uint32 words)."""
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, np.uint32)
+    t = ctypes.POINTER(ctypes.c_uint32)
+    lib().deg_oracle_philox(c.ctypes.data_as(t), k.ctypes.data_as(t), out.ctypes.data_as(t))
+    return out
+
+
+def bm_m2log(w: int) -> float:
+    """-2 ln U of the Box-Muller transform for the 32-bit word w (U = ((w >> 8) + 1) 2^-24)."""
+    return float(lib().deg_oracle_bm_m2log(int(w)))
+
+
+def bm_sincos(w: int):
+    """(cos, sin) of 2 pi (w >> 8) 2^-24 as the Box-Muller transform evaluates them."""
+    c = ctypes.c_float()
+    s = ctypes.c_float()
+    lib().deg_oracle_bm_sincos(int(w), ctypes.byref(c), ctypes.byref(s))
+    return float(c.value), float(s.value)
+
+
+def normals(seed: int, id0: int, n: int, step: int, m: int) -> np.ndarray:
+    """(m, n) standard normals of step `step` for streams id0 .. id0 + n - 1."""
+    out = np.zeros((m, n), np.float32)
+    lib().deg_oracle_normals(int(seed) & 0xFFFFFFFFFFFFFFFF, int(id0), int(n), int(step), int(m), _p(out, _fp))
+    return out
+
+
+def sde_dims(system: str):
+    n, m, d = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    lib().deg_oracle_sde_dims(SDE_SYSTEM_IDS[system], ctypes.byref(n), ctypes.byref(m), ctypes.byref(d))
+    return n.value, m.value, bool(d.value)
+
+
+def sde_solve(system: str, method: str, u0, p, tspan, dt, *, seed=0, seeds=None, traj_base=0, saveat=None,
+              nsave=None, save_everystep=True, t0s=None, tfs=None, want_wsum=False):
+    """Float32 restatement of em_kernel / siea_kernel over an ensemble. u0 (n, N), p (np, N) float32
+    SoA. Returns dict with ts (nsave, N), us (nsave, n, N), retcode, naccept and (want_wsum) wsum
+    (m, N): the sum of the Wiener increments of each noise component (float64)."""
+    n, m, _diag = sde_dims(system)
+    u0 = _f32(u0)
+    N = u0.shape[1]
+    np_ = SDE_NP[system]
+    p = _f32(p) if np_ > 0 else np.zeros((1, N), np.float32)
+    sv = _f32(saveat)
+    if nsave is None:
+        if sv is not None:
+            nsave = len(sv)
+        elif save_everystep:
+            raise ValueError("nsave is required for every-step output")
+        else:
+            nsave = 2
+    ts = np.full((nsave, N), np.float32(tspan[0]), np.float32)
+    us = np.full((nsave, n, N), np.nan, np.float32)
+    rc = np.zeros(N, np.int32)
+    na = np.zeros(N, np.int32)
+    ws = np.zeros((m, N), np.float64) if want_wsum else None
+    sd = None if seeds is None else np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
+    t0s, tfs = _f32(t0s), _f32(tfs)
+    r = lib().deg_oracle_sde_solve(SDE_SYSTEM_IDS[system], SDE_METHOD_IDS[method], N, _p(u0, _fp), _p(p, _fp),
+                                   _p(t0s, _fp), _p(tfs, _fp), float(np.float32(tspan[0])),
+                                   float(np.float32(tspan[1])), float(np.float32(dt)),
+                                   None if sd is None else sd.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                   int(seed) & 0xFFFFFFFFFFFFFFFF, int(traj_base), _p(sv, _fp),
+                                   0 if sv is None else len(sv), int(nsave), int(bool(save_everystep)),
+                                   _p(ts, _fp), _p(us, _fp), _p(rc, _ip), _p(na, _ip), _p(ws, _dp))
+    if r != 0:
+        raise RuntimeError(f"oracle SDE solve failed ({r})")
+    out = dict(ts=ts, us=us, retcode=rc, naccept=na)
+    if want_wsum:
+        out["wsum"] = ws
     return out
