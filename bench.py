@@ -14,6 +14,10 @@ Other workloads (``--workload``), measured the same way:
                  reltol = 1e-4, abstol = 1e-8, FP64 W factorisation, 20 log-spaced save times.
   vdp            2,000,000 Van der Pol, mu ~ logU(0.1, 30), u0 ~ U(-2, 2)^2, tspan (0, 50),
                  GPUTsit5 abstol = reltol = 1e-6, endpoint only.
+  sde_lorenz_em / sde_lorenz_siea
+                 1,000,000 trajectories of the reference's Lorenz SDE (drift p = (10, 28, 8/3),
+                 diagonal additive noise 3, u0 = (1, 0, 0)), GPUEM / GPUSIEA dt = 1/256 on (0, 10):
+                 2560 steps each, endpoint only, seed 123 (Philox4x32-10 increments).
 
 Prints one JSON line. ``roofline`` prices the solver kernel against the FP32 vector peak. The
 flop count is algorithmic:
@@ -22,7 +26,10 @@ flop count is algorithmic:
     with three solves, n^2 + n + (2/3)n^3 + 6n^2 in FP64;
   - dense output per save point: Tsit5 15n + 40, Rosenbrock23 4n + 8.
 F, FJ and FT are the traced RHS, Jacobian and time-gradient flop counts. Attempts come from the
-kernel's own per-trajectory accept/reject counters. The kernel time comes from HIP events
+kernel's own per-trajectory accept/reject counters. SDE steps (n states, m increments, G the
+diffusion's flops): EM 4n + F + G, SIEA 45n + 2F + 3G, plus 45 flops per Box-Muller pair; the
+Philox4x32-10 integer work (100 integer operations per four normals) is reported beside the
+flops and not counted in them. The kernel time comes from HIP events
 recorded on the launch stream around every timed step.
 
 Besides ``value`` (inputs and outputs resident in HBM) the line reports the end-to-end rate
@@ -63,6 +70,17 @@ WORKLOADS = {
                 desc="vdp_2M: Van der Pol mu~logU(0.1,30) u0~U(-2,2)^2 tspan (0,50), GPUTsit5 adaptive "
                      "abstol=reltol=1e-6, endpoint only, PI controller"),
 }
+SDE_WORKLOADS = {
+    "sde_lorenz_em": dict(alg="GPUEM", steps=10, warmup=2, cpu_sample=4_000,
+                          desc="sde_lorenz_em_1M: Lorenz drift p=(10,28,8/3), diagonal additive noise 3, "
+                               "u0=(1,0,0) (gpu_sde_regression.jl:43-58), GPUEM dt=1/256 tspan (0,10) = 2560 "
+                               "steps, endpoint only, seed 123"),
+    "sde_lorenz_siea": dict(alg="GPUSIEA", steps=10, warmup=2, cpu_sample=2_000,
+                            desc="sde_lorenz_siea_1M: Lorenz drift p=(10,28,8/3), diagonal additive noise 3, "
+                                 "u0=(1,0,0) (gpu_sde_regression.jl:43-58), GPUSIEA dt=1/256 tspan (0,10) = "
+                                 "2560 steps, endpoint only, seed 123"),
+}
+WORKLOADS.update(SDE_WORKLOADS)
 UNEVEN_DRAWS = [("p", 0, 5.0, 10.0), ("p", 1, 0.0, 60.0), ("p", 2, 1.0, 3.0), ("tfs", 0, 1.0, 49.0)]
 
 
@@ -226,6 +244,104 @@ def fp64_error(name, us):
                           + ("scipy Radau rtol 1e-11" if name == "robertson" else "Vern9 abstol=reltol=1e-12") + ")")
 
 
+def sde_flop_model(alg, n, m, F, G):
+    """(FP32 flops per step, Philox integer ops per step) of one SDE trajectory step."""
+    pairs = sum(1 + (4 * b + 2 < m) for b in range((m + 3) // 4))
+    bm = 45 * pairs
+    if alg == "GPUEM":
+        return 4 * n + F + G + bm, 100 * ((m + 3) // 4)
+    return 45 * n + 2 * F + 3 * G + bm, 100 * ((m + 3) // 4)
+
+
+def main_sde(args, name):
+    """The SDE workloads: one step is one ensemble solve of all trajectories (inputs in HBM)."""
+    import torch
+
+    import __graft_entry__ as entry
+
+    deg = entry.load_package()
+    from diffeqgpu_amd import systems, tracer
+
+    spec = SDE_WORKLOADS[name]
+    steps = args.steps or spec["steps"]
+    warmup = spec["warmup"] if args.warmup < 0 else args.warmup
+    N = args.n_traj or 1_000_000
+    f, n, np_, m_noise, oname = systems.SDE_SYSTEMS["lorenz"]
+    m = n if m_noise is None else m_noise
+    p0 = np.array([10.0, 28.0, np.float32(8.0) / np.float32(3.0)], np.float32)
+    u0 = np.zeros((3, N), np.float32)
+    u0[0] = 1.0
+    p = np.repeat(p0[:, None], N, axis=1)
+    dt, tspan, seed = np.float32(1.0 / 256), (0.0, 10.0), 123
+    dev = deg.HIPDevice(0, contract=args.contract if args.contract != "fast" else "off",
+                        block_size=args.block_size or None)
+    probs = deg.HIPProbVector.from_arrays(f, u0, p, tspan, dev, seed=seed)
+    prob = deg.SDEProblem(f, None, u0[:, 0], tspan, p0, seed=seed)
+    alg = getattr(deg, spec["alg"])()
+    stream = torch.cuda.current_stream(torch.device("cuda", 0))
+    res = deg.vectorized_solve(probs, prob, alg, dt=dt, save_everystep=False)
+    run = res.relaunch
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        evs[k][0].record(stream)
+        run()
+        evs[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [a.elapsed_time(b) for a, b in evs]
+    nsteps = int(res.naccept.to(torch.int64).sum().item())
+    F = tracer.flop_counts(f.f, n, np_)["F"]
+    G = f.hip_source(n, np_, m_noise)[1] - F
+    fstep, istep = sde_flop_model(spec["alg"], n, m, F, G)
+    k_avg_s = float(np.mean(kernel_ms)) / 1e3
+    achieved = nsteps * fstep / k_avg_s / 1e12
+    retcodes = np.bincount(res.retcode.cpu().numpy(), minlength=4).tolist()
+    out = {
+        "metric": "trajectories/sec (fixed-step SDE ensemble, endpoint output)",
+        "value": N / (wall / steps),
+        "unit": "trajectories/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": wall * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (the reference's SDE test problem; Philox4x32-10 Wiener increments, seed 123)",
+        "config": {"workload": spec["desc"], "trajectories_per_step": N, "kernel_flags": res.flags,
+                   "block_size": dev.block_size or 256},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel_ms": float(np.mean(kernel_ms)),
+                     "sde_steps_per_launch": nsteps, "flops_per_step": fstep, "philox_int_ops_per_step": istep,
+                     "int_tops": nsteps * istep / k_avg_s / 1e12},
+        "retcodes": retcodes,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        ns = min(N, spec["cpu_sample"])
+        meth = "em" if spec["alg"] == "GPUEM" else "siea"
+        reps, c0 = 0, time.perf_counter()
+        while True:
+            pyoracle.sde_solve(oname, meth, u0[:, :ns], p[:, :ns], tspan, dt, seed=seed, save_everystep=False,
+                               nthreads=threads)
+            reps += 1
+            el = time.perf_counter() - c0
+            if el > 10.0 or reps >= 50:
+                break
+        out["cpu_baseline"] = dict(value=reps * ns / el, unit="trajectories/s", cores=threads, kind="port",
+                                   sample=f"{reps} x {ns} trajectories of the same workload, oracle/deg_oracle_sde.c "
+                                          f"Float32 restatement, OpenMP {threads} threads, {el:.1f} s")
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -243,6 +359,8 @@ def main():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
     name = args.workload
+    if name in SDE_WORKLOADS:
+        return main_sde(args, name)
     spec = WORKLOADS[name]
     steps = args.steps or spec["steps"]
     warmup = spec["warmup"] if args.warmup < 0 else args.warmup

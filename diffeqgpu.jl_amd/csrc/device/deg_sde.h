@@ -113,7 +113,7 @@ extern "C" __global__ void __launch_bounds__(256) DEG_ATTR_WAVES DEG_KERNEL(cons
         }
 #endif
 #else
-        // SIEA (gpu_siea_perform_step.jl:97-119) with alpha1 = alpha2 = gamma1 = 1/2,
+        // SIEA (gpu_siea_perform_step.jl:98-119) with alpha1 = alpha2 = gamma1 = 1/2,
         // lambda = (1/4, -1/4, 1/4), mu = (1/4, 1/4, -1/4), mu0 = mubar0 = lambda0 = lambdabar0 = 1,
         // nu1 = 1, nu2 = 0, beta2 = 1, beta3 = 0, delta2 = -1, delta3 = 0
         const float al1 = 0.5f, al2 = 0.5f, ga1 = 0.5f;

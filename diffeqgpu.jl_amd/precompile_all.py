@@ -10,6 +10,8 @@ from .algorithms import HIPDevice
 from .systems import SDE_SYSTEMS, SYSTEMS
 
 _METHODS = (_lib.DEG_TSIT5, _lib.DEG_VERN7, _lib.DEG_RB23)
+_METHODS_MORE = (_lib.DEG_VERN9, _lib.DEG_RODAS4, _lib.DEG_RODAS5P, _lib.DEG_KVAERNO3, _lib.DEG_KVAERNO5)
+_IMPLICIT = (_lib.DEG_RB23, _lib.DEG_RODAS4, _lib.DEG_RODAS5P, _lib.DEG_KVAERNO3, _lib.DEG_KVAERNO5)
 
 
 def variants():
@@ -47,6 +49,15 @@ def variants():
                 base = HIPDevice().flags(True)
                 for extra in (_lib.F_LEARNED, _lib.F_STEPLOG, _lib.F_LEARNED | _lib.F_STEPLOG):
                     out.append((name, src, n, np_, m, base | extra))
+    # the other methods in the variants the parity tests launch (fixed step, bitwise adaptive)
+    more = [(HIPDevice(contract="off"), False),
+            (HIPDevice(contract="off", powmode="det", persistent=False), True),
+            (HIPDevice(contract="off", powmode="det", persistent=True), True)]
+    for name, (f, n, np_, _sid) in SYSTEMS.items():
+        for m in _METHODS_MORE:
+            src, _ = f.hip_source(n, np_, m in _IMPLICIT)
+            for dev, adaptive in more:
+                out.append((name, src, n, np_, m, dev.flags(adaptive)))
     for name, (f, n, np_, nm, _o) in SDE_SYSTEMS.items():
         src, _ = f.hip_source(n, np_, nm)
         for m in ((_lib.DEG_EM,) if nm is not None else (_lib.DEG_EM, _lib.DEG_SIEA)):

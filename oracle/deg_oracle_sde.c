@@ -326,13 +326,14 @@ static int sde_traj(int sys, int method, int64_t N, int64_t i, const float *u0, 
 int deg_oracle_sde_solve(int system, int method, int64_t N, const float *u0, const float *p, const float *t0s,
                          const float *tfs, float t0, float tf, float dt, const uint64_t *seeds, uint64_t seed,
                          int64_t traj_base, const float *saveat, int nsaveat, int nsave, int save_everystep, float *ts,
-                         float *us, int32_t *retcode, int32_t *naccept, double *wsum) {
+                         float *us, int32_t *retcode, int32_t *naccept, double *wsum, int nthreads) {
     if (system < 0 || system >= OSDE_COUNT || (method != 8 && method != 9) || N < 0) return -1;
     int n, m, diag;
     sde_dims(system, &n, &m, &diag);
     if (method == 9 && !diag) return -2;
     if (!(dt > 0.0f)) return -3;
-#pragma omp parallel for schedule(dynamic, 64)
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
     for (int64_t i = 0; i < N; ++i) {
         const float a = t0s ? t0s[i] : t0, b = tfs ? tfs[i] : tf;
         const uint64_t sd = seeds ? seeds[i] : seed;

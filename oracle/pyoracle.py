@@ -118,7 +118,7 @@ def lib():
         L.deg_oracle_sde_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _fp, _fp, _fp, _fp,
                                            ctypes.c_float, ctypes.c_float, ctypes.c_float, _u64p, ctypes.c_uint64,
                                            ctypes.c_int64, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp, _fp,
-                                           _ip, _ip, _dp]
+                                           _ip, _ip, _dp, ctypes.c_int]
         L.deg_oracle_sde_solve.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -280,7 +280,7 @@ def sde_dims(system: str):
 
 
 def sde_solve(system: str, method: str, u0, p, tspan, dt, *, seed=0, seeds=None, traj_base=0, saveat=None,
-              nsave=None, save_everystep=True, t0s=None, tfs=None, want_wsum=False):
+              nsave=None, save_everystep=True, t0s=None, tfs=None, want_wsum=False, nthreads=None):
     """Float32 restatement of em_kernel / siea_kernel over an ensemble. u0 (n, N), p (np, N) float32
     SoA. Returns dict with ts (nsave, N), us (nsave, n, N), retcode, naccept and (want_wsum) wsum
     (m, N): the sum of the Wiener increments of each noise component (float64)."""
@@ -310,7 +310,8 @@ def sde_solve(system: str, method: str, u0, p, tspan, dt, *, seed=0, seeds=None,
                                    None if sd is None else sd.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
                                    int(seed) & 0xFFFFFFFFFFFFFFFF, int(traj_base), _p(sv, _fp),
                                    0 if sv is None else len(sv), int(nsave), int(bool(save_everystep)),
-                                   _p(ts, _fp), _p(us, _fp), _p(rc, _ip), _p(na, _ip), _p(ws, _dp))
+                                   _p(ts, _fp), _p(us, _fp), _p(rc, _ip), _p(na, _ip), _p(ws, _dp),
+                                   int(nthreads or os.cpu_count() or 1))
     if r != 0:
         raise RuntimeError(f"oracle SDE solve failed ({r})")
     out = dict(ts=ts, us=us, retcode=rc, naccept=na)
