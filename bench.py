@@ -104,6 +104,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the end-to-end and JIT-compile timings")
     ap.add_argument("--no-order", action="store_true", help="lorenz_uneven: ascending ids instead of longest first")
+    ap.add_argument("--start-order", default="cost", choices=["cost", "none", "span"],
+                    help="persistent grids: start trajectories by descending span / initial step estimate (cost), "
+                         "in index order (none), or (lorenz_uneven) by descending span (span)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposer leg")
     ap.add_argument("--mlp-every", type=int, default=2, help="learned leg: evaluate the proposer every k iterations")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -375,9 +378,10 @@ def main():
     cfg = workloads.config(name, N=args.n_traj or None)
     N = cfg["N"]
     f, n, np_, _ = systems.SYSTEMS[cfg["system"]]
+    start = "none" if args.start_order == "none" or args.no_order else "cost"
     dev = deg.HIPDevice(0, contract=args.contract, powmode=args.powmode, persistent=bool(args.persistent),
                         block_size=args.block_size or None, refill=args.refill, chunk=args.chunk,
-                        kahan=bool(cfg.get("kahan")))
+                        kahan=bool(cfg.get("kahan")), start_order=start)
     alg = getattr(deg, spec["alg"])()
     prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
     kw = solve_kw(cfg)
@@ -393,7 +397,7 @@ def main():
         sweeps = spec["sweeps"]
         probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev, tfs=cfg["tfs"])
         sw = deg.ParameterSweep(probs, prob, alg, sweeps=sweeps, seed=303, draws=UNEVEN_DRAWS,
-                                longest_first=not args.no_order, **kw)
+                                longest_first=args.start_order == "span" and not args.no_order, **kw)
         stream = sw.stream
         run = sw.replay
         per_step = sweeps * N
@@ -456,7 +460,9 @@ def main():
         "config": {"workload": spec["desc"], "trajectories_per_step": per_step, "controller": "PI",
                    "kernel_flags": results[0].flags, "contract": args.contract, "powmode": args.powmode,
                    "persistent": bool(args.persistent), "block_size": dev.block_size or 256,
-                   "waves": dev.waves, "refill": dev.refill or 4, "chunk": dev.chunk or 16, "autotuned": tuned},
+                   "waves": dev.waves, "refill": dev.refill or 4, "chunk": dev.chunk or 16, "autotuned": tuned,
+                   "start_order": ("span" if name == "lorenz_uneven" and args.start_order == "span" and not args.no_order
+                                   else dev.start_order)},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,

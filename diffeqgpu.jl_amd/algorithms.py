@@ -162,6 +162,10 @@ class HIPDevice:
       compact     persistent adaptive grids: once the queue is empty, waves with few
                   trajectories left move them to other waves of the workgroup (through LDS)
                   and leave, so fewer, fuller waves finish the tail.
+      start_order persistent adaptive grids: "cost" starts the trajectories in descending order of
+                  span / initial step estimate (deg_start_order, on the device before every solve),
+                  "none" in index order; None: "cost". An explicit ``order=`` overrides it. Results
+                  do not depend on the order.
     """
 
     device: int = 0
@@ -177,6 +181,15 @@ class HIPDevice:
     blocks_per_cu: int = 0
     compact: bool = False
     batch_saves: bool = False
+    start_order: str | None = None
+
+    def cost_order(self, adaptive: bool) -> bool:
+        """Whether solves on this device start their trajectories in cost order."""
+        mode = self.start_order or "cost"
+        if mode not in ("cost", "none"):
+            raise ValueError("start_order must be 'cost' or 'none'")
+        persistent = self.persistent if self.persistent is not None else adaptive
+        return mode == "cost" and adaptive and persistent
 
     def sde_flags(self) -> int:
         """Compile flags of the fixed-step SDE kernels: contraction (default "off") and the

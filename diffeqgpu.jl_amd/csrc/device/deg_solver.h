@@ -1134,7 +1134,10 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                 unsigned long long pending = wmask;
                 while (pending != 0ull && qtries < DEG_NQUEUE) {
                     if (c_next >= c_end) {
-                        const long long lim = min((long long)(qi + 1) * slab, a.n_traj);
+                        // with a start order, queue q serves its entries q, q + 8, q + 16, ... (see the
+                        // lookup at init), so every XCD starts on its share of the first entries
+                        const long long lim = a.order ? (long long)qi * slab + (a.n_traj - qi + DEG_NQUEUE - 1) / DEG_NQUEUE
+                                                      : min((long long)(qi + 1) * slab, a.n_traj);
                         const long long base = (long long)qi * slab +
                             (long long)deg_claim(a.queue + qi * DEG_QSTRIDE, leader, lane, (unsigned int)a.chunk);
                         if (base >= lim) {
@@ -1277,7 +1280,12 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
         long long oi = -1;
         if (need_init) {
             need_init = false;
+#if DEG_PERSISTENT
+            // position j of queue q holds start-order entry j * DEG_NQUEUE + q (< n_traj by the queue's limit)
+            oi = a.order ? (long long)a.order[(gid % slab) * DEG_NQUEUE + gid / slab] : gid;
+#else
             oi = a.order ? (long long)a.order[gid] : gid;
+#endif
             if (oi < 0 || oi >= a.n_traj) {
                 // an entry of a caller's start order outside [0, n_traj): nothing to solve
                 oi = -1;
@@ -1937,6 +1945,37 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
 
 // Clears the persistent work counter before a launch (a kernel, not a memset, so the
 // sequence can be captured in a graph).
+// Start-order cost key of each trajectory: its span over the initial step estimate of Hairer,
+// Norsett and Wanner (Solving Ordinary Differential Equations I, II.4): with sc = abstol + |u0| reltol,
+// d0 = ||u0 / sc||, d1 = ||f(u0, p, t0) / sc|| (RMS norms), h0 = 0.01 d0 / d1, or 1e-6 when d0 or d1
+// is below 1e-5. Sorting by it (descending) starts the trajectories that are likely to take the most
+// steps first; a non-finite key becomes 0 (started last).
+extern "C" __global__ void __launch_bounds__(256) DEG_COST_KEY(const deg_kargs a, float* __restrict__ key) {
+    const long long li = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= a.n_traj) return;
+    const long long i = a.id_offset + li;
+    const long long S = a.stride;
+    float u[DEG_N], f0[DEG_N];
+    float p[DEG_NP > 0 ? DEG_NP : 1];
+    DEG_FOR(c) u[c] = a.u0[(long long)c * S + i];
+    _Pragma("unroll") for (int j = 0; j < DEG_NP; ++j) p[j] = a.p[(long long)j * S + i];
+    const float t0 = a.t0s ? a.t0s[i] : a.t0;
+    const float tf = a.tfs ? a.tfs[i] : a.tf;
+    deg_rhs(u, p, t0, f0);
+    float d0 = 0.0f, d1 = 0.0f;
+    DEG_FOR(c) {
+        const float sc = a.abstol + __builtin_fabsf(u[c]) * a.reltol;
+        const float x = u[c] / sc, y = f0[c] / sc;
+        d0 += x * x;
+        d1 += y * y;
+    }
+    d0 = __builtin_sqrtf(d0 / (float)DEG_N);
+    d1 = __builtin_sqrtf(d1 / (float)DEG_N);
+    const float h0 = (d0 < 1e-5f || d1 < 1e-5f) ? 1e-6f : 0.01f * d0 / d1;
+    const float k = __builtin_fabsf(tf - t0) / h0;
+    key[li] = (k == k && k < 3.0e38f) ? k : (k == k ? 3.0e38f : 0.0f);
+}
+
 extern "C" __global__ void DEG_QUEUE_RESET(unsigned int* q) {
     if (blockIdx.x == 0 && threadIdx.x < DEG_NQUEUE) q[threadIdx.x * DEG_QSTRIDE] = 0u;
 }

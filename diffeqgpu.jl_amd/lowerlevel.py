@@ -242,6 +242,18 @@ def _launch(probs: HIPProbVector, alg, adaptive: bool, *, dt, abstol, reltol, sa
     if mlp is not None:
         a.mlp = mlp.data_ptr()
         a.mlp_every = int(mlp_every)
+    okey = owork = None
+    if order is None and not sde and dev.cost_order(adaptive) and N > 0:
+        # cost-ordered start, recomputed on the device before every launch of this solve
+        okey, order, owork = (outputs.get(k) for k in ("order_key", "order_auto", "order_work"))
+        if okey is None:
+            okey = torch.empty(N, dtype=torch.float32, device=device)
+        if order is None:
+            order = torch.empty(N, dtype=torch.int32, device=device)
+        if owork is None:
+            owork = torch.empty(_lib.ORDER_WORK_INTS, dtype=torch.int32, device=device)
+        if okey.numel() != N or order.numel() != N or owork.numel() != _lib.ORDER_WORK_INTS:
+            raise ValueError("preallocated start-order buffers have the wrong size")
     if order is not None:
         if order.dtype != torch.int32 or order.numel() != N or order.device != device:
             raise ValueError("order must be an int32 tensor of N trajectory indices on the problems' device")
@@ -264,19 +276,24 @@ def _launch(probs: HIPProbVector, alg, adaptive: bool, *, dt, abstol, reltol, sa
         a.seeds = probs.seeds.data_ptr() if probs.seeds is not None else None
         a.seed = int(probs.seed) & 0xFFFFFFFFFFFFFFFF
         a.traj_base = int(traj_base)
+    if okey is not None:
+        kern.start_order(a, okey.data_ptr(), order.data_ptr(), owork.data_ptr())
     kern.solve(a)
     if timing:
         ms = float(box.value)
         a.elapsed_ms = None  # the box does not outlive this call
 
     def relaunch():
-        # same arguments, same output buffers: recomputes and overwrites every output
+        # same arguments, same output buffers: recomputes the start order and every output
+        if okey is not None:
+            kern.start_order(a, okey.data_ptr(), order.data_ptr(), owork.data_ptr())
         kern.solve(a)
 
     # the result keeps the argument block and every tensor it points to alive
     return SolveResult(ts, us_soa.permute(0, 2, 1), retcode=retcode, naccept=nacc, nreject=nrej, us_soa=us_soa,
                        kernel_ms=ms, saveat=sv, kernel=kern, flags=flags, relaunch=relaunch, args=a,
-                       probs=probs, tstops=tst)
+                       probs=probs, tstops=tst, order_key=okey, order_auto=order if okey is not None else None,
+                       order_work=owork)
 
 
 def vectorized_solve(probs, prob, alg, *, dt, saveat=None, save_everystep=True, debug=False, callback=None,

@@ -80,7 +80,8 @@ class ParameterSweep:
                 self.results.append(self._solve(None))
         self.stream.synchronize()
         self.outputs = [dict(ts=r.ts, us_soa=r.us_soa, retcode=r.retcode, naccept=r.naccept, nreject=r.nreject,
-                             saveat=r.saveat) for r in self.results]
+                             saveat=r.saveat, order_key=r.order_key, order_auto=r.order_auto,
+                             order_work=r.order_work) for r in self.results]
         # capture
         sp = self.stream.cuda_stream
         _lib.Graph.begin(sp)

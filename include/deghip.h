@@ -125,8 +125,10 @@ typedef struct deg_solve_args {
     int32_t chunk;           /* persistent kernels: trajectory ids a wave claims per queue atomic; 0: 16 */
     int32_t mlp_every;       /* DEG_F_LEARNED: evaluate the proposer every k-th iteration (reusing its last
                                 correction in between); 0 or 1: every attempt */
-    const int32_t *order;    /* optional [n_traj] permutation: the k-th trajectory started is id_offset + order[k]
-                                (e.g. deg_order_by_span, longest first); NULL: ascending ids */
+    const int32_t *order;    /* optional [n_traj] start order of trajectories id_offset + order[k] (e.g. from
+                                deg_start_order or deg_order_by_span); a plain grid starts them in k order, a
+                                persistent grid's 8 queues (one per XCD) take k = q, q + 8, q + 16, ... each,
+                                so every XCD starts on its share of the first entries; NULL: ascending ids */
     const float *tstops;     /* DEG_F_TSTOPS: [ntstops] ascending stopping times shared by all trajectories
                                 (device); the `tstops` keyword of vectorized_solve / vectorized_asolve */
     int32_t ntstops;
@@ -231,6 +233,16 @@ int deg_draw_normal(float *out, int64_t n, int64_t id_offset, uint64_t seed, uin
  * idle (no counterpart in the reference, which launches one thread per trajectory). */
 int deg_order_by_span(const float *t0s, const float *tfs, float t0, float tf, int64_t n, int32_t *order,
                       int32_t *work, void *stream);
+
+/* Cost-ordered start (no counterpart in the reference, which starts trajectories in index order):
+ * key[i] = |tf_i - t0_i| / h0_i for the n_traj trajectories of `a`, with h0 the standard initial step
+ * estimate (Hairer, Norsett, Wanner; sc = abstol + |u0| reltol, h0 = 0.01 ||u0/sc|| / ||f(u0)/sc||)
+ * computed by the kernel's own traced RHS, then order = the trajectories by descending key
+ * (deg_order_by_span's bucketed counting sort; work: 4096 int32). Pass order as a->order to deg_solve:
+ * trajectories likely to take the most steps start first, which shortens the tail of a persistent
+ * grid. ODE kernels only; asynchronous on a->stream, graph-capturable; results do not depend on
+ * the order. key: [n_traj] float, order: [n_traj] int32, device memory. */
+int deg_start_order(deg_kernel *k, const deg_solve_args *a, float *key, int32_t *order, int32_t *work);
 
 typedef struct deg_graph deg_graph;
 /* Capture every launch made on `stream` (non-NULL) by this thread between begin and end --

@@ -56,7 +56,10 @@ def test_device_draws_match_host(deg, torch_cuda):
 
 
 def _snapshot(sw):
-    return [{k: v.cpu().numpy().copy() for k, v in o.items() if v is not None} for o in sw.outputs]
+    # the start order's key, permutation and work counters are scratch (the order inside a key bucket
+    # follows the atomics of the sort, and the solve's results do not depend on it)
+    return [{k: v.cpu().numpy().copy() for k, v in o.items()
+             if v is not None and k not in ("order_key", "order_auto", "order_work")} for o in sw.outputs]
 
 
 def test_sweep_graph_replays_equal_eager_and_oracle(deg, torch_cuda):
@@ -183,3 +186,61 @@ def test_order_entries_outside_range_are_skipped(deg, torch_cuda):
         keep = np.setdiff1d(np.arange(2000), skipped)
         for a, b in ((ref.us_soa, got.us_soa), (ref.ts, got.ts), (ref.naccept, got.naccept)):
             np.testing.assert_array_equal(a.cpu().numpy()[..., keep], b.cpu().numpy()[..., keep])
+
+
+def test_cost_start_order(deg, torch_cuda):
+    """deg_start_order: the key is the span over the initial step estimate h0 = 0.01 d0 / d1, the
+    order a permutation in descending key buckets; persistent solves in cost order, index order and
+    a reversed order agree bitwise, for ensemble sizes that leave the 8 queues unequal."""
+    torch = torch_cuda
+    from diffeqgpu_amd import systems
+
+    f, n, np_, _ = systems.SYSTEMS["lorenz"]
+    abstol, reltol = 1e-6, 1e-6
+    for N in (1, 7, 13, 1001, 5003):
+        p0, tfs = host_uneven(900 + N, N)
+        tfs = (tfs / 5.0).astype(F32)
+        u0 = np.zeros((3, N), F32)
+        u0[0] = 1.0
+        u0[1] = np.linspace(-1, 1, N, dtype=F32)
+        prob = deg.ODEProblem(f, u0[:, 0], (0.0, 10.0), p0[:, 0])
+        kw = dict(dt=0.01, abstol=abstol, reltol=reltol, saveat=np.array([0.0, 0.5, 1.0, 2.0], F32))
+        outs = []
+        print(f"[gpu] cost start order N={N}", flush=True)
+        for dev, ordr in ((deg.HIPDevice(start_order="none"), None), (deg.HIPDevice(), None),
+                          (deg.HIPDevice(), torch.as_tensor(np.arange(N - 1, -1, -1, dtype=np.int32), device="cuda"))):
+            probs = deg.HIPProbVector.from_arrays(f, u0, p0, (0.0, 10.0), dev, tfs=tfs)
+            r = deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), order=ordr, **kw)
+            torch.cuda.synchronize()
+            outs.append([x.cpu().numpy() for x in (r.ts, r.us_soa, r.retcode, r.naccept, r.nreject)])
+            if dev.start_order is None and ordr is None:
+                assert r.order_auto is not None
+                key = r.order_key.cpu().numpy()
+                o = r.order_auto.cpu().numpy()
+                assert np.array_equal(np.sort(o), np.arange(N))
+                assert np.all(np.diff((key.view(np.uint32) >> 20)[o].astype(np.int64)) <= 0)
+                # the key restated in float64 from the inputs
+                x = u0.astype(np.float64)
+                s_, r_, b_ = p0.astype(np.float64)
+                f0 = np.stack([s_ * (x[1] - x[0]), x[0] * (r_ - x[2]) - x[1], x[0] * x[1] - b_ * x[2]])
+                sc = abstol + np.abs(x) * reltol
+                d0 = np.sqrt(((x / sc) ** 2).mean(axis=0))
+                d1 = np.sqrt(((f0 / sc) ** 2).mean(axis=0))
+                h0 = np.where((d0 < 1e-5) | (d1 < 1e-5), 1e-6, 0.01 * d0 / d1)
+                np.testing.assert_allclose(key, tfs / h0, rtol=1e-4)
+            else:
+                assert r.order_auto is None
+        # save rows after a trajectory's own end time are not written (the reference leaves them too)
+        written = np.broadcast_to((kw["saveat"][:, None] <= tfs[None, :])[:, None, :], outs[0][1].shape)
+        for other in outs[1:]:
+            for j, (a, b) in enumerate(zip(outs[0], other)):
+                if j == 1:
+                    np.testing.assert_array_equal(a[written], b[written])
+                else:
+                    np.testing.assert_array_equal(a, b)
+        assert (outs[0][3] > 0).all()  # every trajectory was solved
+    # fixed-step and plain grids take no start order
+    probs = deg.HIPProbVector.from_arrays(f, u0, p0, (0.0, 10.0), deg.HIPDevice(), tfs=tfs)
+    assert deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), dt=0.01, order=None).order_auto is not None
+    probs = deg.HIPProbVector.from_arrays(f, u0, p0, (0.0, 10.0), deg.HIPDevice(persistent=False), tfs=tfs)
+    assert deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), dt=0.01).order_auto is None
