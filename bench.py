@@ -41,6 +41,7 @@ Options: --gpus N (1 only: one GPU), --steps K, --warmup W, --workload, --n-traj
 --no-cpu (skip the CPU baseline), --no-extras (skip the end-to-end and compile timings).
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -104,9 +105,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the end-to-end and JIT-compile timings")
     ap.add_argument("--no-order", action="store_true", help="lorenz_uneven: ascending ids instead of longest first")
-    ap.add_argument("--start-order", default="cost", choices=["cost", "none", "span"],
-                    help="persistent grids: start trajectories by descending span / initial step estimate (cost), "
-                         "in index order (none), or (lorenz_uneven) by descending span (span)")
+    ap.add_argument("--start-order", default="auto", choices=["auto", "cost", "none", "span"],
+                    help="persistent grids: the tuned choice (auto), descending span / initial step estimate (cost), "
+                         "index order (none), or (lorenz_uneven) descending span (span)")
+    ap.add_argument("--retune", action="store_true", help="measure the launch configuration again")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposer leg")
     ap.add_argument("--mlp-every", type=int, default=2, help="learned leg: evaluate the proposer every k iterations")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -378,7 +380,9 @@ def main():
     cfg = workloads.config(name, N=args.n_traj or None)
     N = cfg["N"]
     f, n, np_, _ = systems.SYSTEMS[cfg["system"]]
-    start = "none" if args.start_order == "none" or args.no_order else "cost"
+    start = {"auto": None, "cost": "cost"}.get(args.start_order, "none")
+    if args.no_order:
+        start = "none"
     dev = deg.HIPDevice(0, contract=args.contract, powmode=args.powmode, persistent=bool(args.persistent),
                         block_size=args.block_size or None, refill=args.refill, chunk=args.chunk,
                         kahan=bool(cfg.get("kahan")), start_order=start)
@@ -388,9 +392,11 @@ def main():
     tuned = None
     if not args.no_autotune and not (args.block_size or args.refill or args.chunk):
         tprobs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev, tfs=cfg.get("tfs"))
-        dev = deg.autotune.tune(tprobs, prob, alg, **kw)
+        dev = deg.autotune.tune(tprobs, prob, alg, refresh=args.retune, **kw)
+        if start is not None or args.start_order == "span":
+            dev = dataclasses.replace(dev, start_order=start or "none")  # an explicit choice beats the tuned one
         tuned = dict(block_size=dev.block_size, waves=dev.waves, refill=dev.refill, chunk=dev.chunk,
-                     blocks_per_cu=dev.blocks_per_cu)
+                     blocks_per_cu=dev.blocks_per_cu, start_order=dev.start_order)
         del tprobs
 
     if name == "lorenz_uneven":

@@ -164,8 +164,8 @@ class HIPDevice:
                   and leave, so fewer, fuller waves finish the tail.
       start_order persistent adaptive grids: "cost" starts the trajectories in descending order of
                   span / initial step estimate (deg_start_order, on the device before every solve),
-                  "none" in index order; None: "cost". An explicit ``order=`` overrides it. Results
-                  do not depend on the order.
+                  "none" in index order; None: "none" (the autotuner measures both). An explicit
+                  ``order=`` overrides it. Results do not depend on the order.
     """
 
     device: int = 0
@@ -185,7 +185,7 @@ class HIPDevice:
 
     def cost_order(self, adaptive: bool) -> bool:
         """Whether solves on this device start their trajectories in cost order."""
-        mode = self.start_order or "cost"
+        mode = self.start_order or "none"
         if mode not in ("cost", "none"):
             raise ValueError("start_order must be 'cost' or 'none'")
         persistent = self.persistent if self.persistent is not None else adaptive

@@ -7,12 +7,15 @@ lowest measured kernel time:
                    compiler's own allocation, 4..7 = capped, possibly spilling);
   refill batch     persistent grids: idle lanes per wave that trigger a refill;
   chunk            persistent grids: trajectory ids claimed per queue atomic;
-  blocks per CU    persistent grids: resident blocks per compute unit (0 = as many as fit).
+  blocks per CU    persistent grids: resident blocks per compute unit (0 = as many as fit);
+  start order      persistent adaptive grids: index order or the cost order of deg_start_order
+                   (which helps where the initial step estimate predicts the work, and can hurt
+                   where it points the wrong way).
 The LDS depth of the saveat cache needs no tuning: it is sized to the save grid at every launch.
 
 The search is coordinate descent in that order, starting from the defaults. Each candidate is
-timed with HIP events around the kernel (the best of ``repeats`` launches) on the caller's
-problems, so the result reflects the real trajectory mix. Results are cached in
+timed with HIP events around full relaunches of the solve (start-order kernels included; the best
+of ``repeats``) on the caller's problems, so the result reflects the real trajectory mix. Results are cached in
 ``_jitcache/autotune.json``, keyed by the traced source, n, np, method, flags and the ensemble
 size rounded to a power of two. ``autotune_mi355x.json`` beside this file ships the results measured
 on MI355X for the benchmark workloads; the local cache takes precedence over it. The reference has
@@ -46,7 +49,8 @@ def blocks_per_cu_candidates(block: int):
 def _key(probs, alg, adaptive, nsave_hint) -> str:
     src, _ = probs.f.hip_source(probs.n, probs.np, isinstance(alg, GPUODEImplicitAlgorithm))
     h = hashlib.sha1(src.encode()).hexdigest()[:16]
-    base = dataclasses.replace(probs.dev, block_size=None, refill=0, chunk=0, waves=0, blocks_per_cu=0)
+    base = dataclasses.replace(probs.dev, block_size=None, refill=0, chunk=0, waves=0, blocks_per_cu=0,
+                               start_order=None)
     nb = int(round(math.log2(max(1, probs.N))))
     return f"{h}:{probs.n}:{probs.np}:{alg.method}:{base.flags(adaptive)}:{nb}:{nsave_hint}"
 
@@ -86,15 +90,18 @@ def _with_dev(probs, dev):
     return q
 
 
-def tune(probs, prob, alg, *, adaptive: bool = True, repeats: int = 2, use_cache: bool = True, log=None,
-         **solve_kw) -> HIPDevice:
-    """Returns the fastest HIPDevice for this ensemble (also stored in the cache)."""
+def tune(probs, prob, alg, *, adaptive: bool = True, repeats: int = 2, use_cache: bool = True, refresh: bool = False,
+         log=None, **solve_kw) -> HIPDevice:
+    """Returns the fastest HIPDevice for this ensemble (also stored in the cache). ``refresh``
+    measures again even when a tuned entry exists."""
+    import torch
+
     from .lowerlevel import vectorized_asolve, vectorized_solve
 
     fn = vectorized_asolve if adaptive else vectorized_solve
     nsave_hint = "saveat" if solve_kw.get("saveat") is not None else ("every" if solve_kw.get("save_everystep") else "end")
     key = _key(probs, alg, adaptive, nsave_hint)
-    if use_cache:
+    if use_cache and not refresh:
         hit = lookup(key)
         if hit is not None:
             return dataclasses.replace(probs.dev, **hit)
@@ -107,10 +114,16 @@ def tune(probs, prob, alg, *, adaptive: bool = True, repeats: int = 2, use_cache
         if k not in trials:
             dev = dataclasses.replace(probs.dev, **cfg)
             q = _with_dev(probs, dev)
+            r = fn(q, prob, alg, **solve_kw)  # the first launch also loads (or JIT-compiles) the variant
+            st = torch.cuda.current_stream(q.u0.device)
             best = math.inf
-            for _ in range(repeats + 1):  # the first launch also loads (or JIT-compiles) the variant
-                r = fn(q, prob, alg, timing=True, **solve_kw)
-                best = min(best, r.kernel_ms)
+            for _ in range(repeats):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(st)
+                r.relaunch()
+                e1.record(st)
+                e1.synchronize()
+                best = min(best, e0.elapsed_time(e1))
             trials[k] = best
             if log:
                 log(f"[autotune] {cfg} {best:.3f} ms")
@@ -118,10 +131,14 @@ def tune(probs, prob, alg, *, adaptive: bool = True, repeats: int = 2, use_cache
 
     default = dict(block_size=256, waves=0, refill=4 if persistent else 0, chunk=16 if persistent else 0,
                    blocks_per_cu=0)
+    if persistent and adaptive:
+        default["start_order"] = "none"
     cfg = dict(default)
     axes = [("block_size", BLOCKS), ("waves", WAVES)]
     if persistent:
         axes += [("refill", REFILL), ("chunk", CHUNK), ("blocks_per_cu", None)]
+    if persistent and adaptive:
+        axes += [("start_order", ("none", "cost"))]
     for name, values in axes:
         if values is None:
             values = blocks_per_cu_candidates(cfg["block_size"])

@@ -207,13 +207,14 @@ def test_cost_start_order(deg, torch_cuda):
         kw = dict(dt=0.01, abstol=abstol, reltol=reltol, saveat=np.array([0.0, 0.5, 1.0, 2.0], F32))
         outs = []
         print(f"[gpu] cost start order N={N}", flush=True)
-        for dev, ordr in ((deg.HIPDevice(start_order="none"), None), (deg.HIPDevice(), None),
-                          (deg.HIPDevice(), torch.as_tensor(np.arange(N - 1, -1, -1, dtype=np.int32), device="cuda"))):
+        for dev, ordr in ((deg.HIPDevice(), None), (deg.HIPDevice(start_order="cost"), None),
+                          (deg.HIPDevice(start_order="cost"),
+                           torch.as_tensor(np.arange(N - 1, -1, -1, dtype=np.int32), device="cuda"))):
             probs = deg.HIPProbVector.from_arrays(f, u0, p0, (0.0, 10.0), dev, tfs=tfs)
             r = deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), order=ordr, **kw)
             torch.cuda.synchronize()
             outs.append([x.cpu().numpy() for x in (r.ts, r.us_soa, r.retcode, r.naccept, r.nreject)])
-            if dev.start_order is None and ordr is None:
+            if dev.start_order == "cost" and ordr is None:
                 assert r.order_auto is not None
                 key = r.order_key.cpu().numpy()
                 o = r.order_auto.cpu().numpy()
@@ -240,7 +241,9 @@ def test_cost_start_order(deg, torch_cuda):
                     np.testing.assert_array_equal(a, b)
         assert (outs[0][3] > 0).all()  # every trajectory was solved
     # fixed-step and plain grids take no start order
-    probs = deg.HIPProbVector.from_arrays(f, u0, p0, (0.0, 10.0), deg.HIPDevice(), tfs=tfs)
+    probs = deg.HIPProbVector.from_arrays(f, u0, p0, (0.0, 10.0), deg.HIPDevice(start_order="cost"), tfs=tfs)
     assert deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), dt=0.01, order=None).order_auto is not None
+    probs = deg.HIPProbVector.from_arrays(f, u0, p0, (0.0, 10.0), deg.HIPDevice(), tfs=tfs)
+    assert deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), dt=0.01).order_auto is None
     probs = deg.HIPProbVector.from_arrays(f, u0, p0, (0.0, 10.0), deg.HIPDevice(persistent=False), tfs=tfs)
     assert deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), dt=0.01).order_auto is None
