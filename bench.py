@@ -326,6 +326,22 @@ def main_sde(args, name):
                      "int_tops": nsteps * istep / k_avg_s / 1e12},
         "retcodes": retcodes,
     }
+    # the same solve with FP contraction and the hardware Box-Muller (not bitwise with the oracle)
+    fdev = dataclasses.replace(dev, contract="fast")
+    fres = deg.vectorized_solve(deg.HIPProbVector.from_arrays(f, u0, p, tspan, fdev, seed=seed), prob, alg, dt=dt,
+                                save_everystep=False)
+    for _ in range(warmup):
+        fres.relaunch()
+    torch.cuda.synchronize()
+    fevs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a_, b_ in fevs:
+        a_.record(stream)
+        fres.relaunch()
+        b_.record(stream)
+    torch.cuda.synchronize()
+    fms = float(np.mean([a_.elapsed_time(b_) for a_, b_ in fevs]))
+    out["fast_normals"] = {"kernel_ms": fms, "trajectories_per_s": N / (fms / 1e3), "kernel_flags": fres.flags,
+                           "what": "contract=fast: FMA contraction and the hardware log2/sqrt/sin/cos Box-Muller"}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle

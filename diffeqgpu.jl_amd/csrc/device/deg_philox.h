@@ -17,7 +17,15 @@
 //       the reduced argument x = r 2 pi 2^-24 (r <= 2^21) lies in [0, pi/4] with one rounding;
 //       sin and cos by their Taylor series to x^9 and x^10.
 // Relative error of the normals: a few Float32 ulps. The tail is cut at sqrt(2 * 24 ln 2) = 5.77.
+//
+// DEG_SDE_FASTNORM (contraction-allowed SDE kernels) evaluates the same transform of the same words
+// with the hardware log2, square root and sine/cosine instead: the same normals to a few ulps, not
+// bit for bit. deg_draw_normal always uses the exact form.
 #pragma once
+
+#ifndef DEG_SDE_FASTNORM
+#define DEG_SDE_FASTNORM 0
+#endif
 
 #define DEG_PHILOX_M0 0xD2511F53u
 #define DEG_PHILOX_M1 0xCD9E8D57u
@@ -95,6 +103,22 @@ __device__ __forceinline__ void deg_bm_pair(unsigned int w0, unsigned int w1, fl
     z1 = rr * si;
 }
 
+// Hardware form of deg_bm_pair: v_log_f32 (log2: -2 ln U = -2 ln 2 log2 U), v_sqrt_f32, and v_cos_f32 /
+// v_sin_f32, which take their argument in revolutions (sin(2 pi x)), so the angle needs no scaling.
+__device__ __forceinline__ void deg_bm_pair_fast(unsigned int w0, unsigned int w1, float& z0, float& z1) {
+    const float u = (float)((w0 >> 8) + 1u) * 5.9604644775390625e-08f;
+    const float rr = __builtin_amdgcn_sqrtf(-1.38629436f * __builtin_amdgcn_logf(u));
+    const float x = (float)(w1 >> 8) * 5.9604644775390625e-08f;
+    z0 = rr * __builtin_amdgcn_cosf(x);
+    z1 = rr * __builtin_amdgcn_sinf(x);
+}
+
+#if DEG_SDE_FASTNORM
+#define DEG_BM_PAIR deg_bm_pair_fast
+#else
+#define DEG_BM_PAIR deg_bm_pair
+#endif
+
 // The M standard normals of one step of trajectory `id` under `seed`.
 template <int M>
 __device__ __forceinline__ void deg_step_normals(unsigned long long seed, unsigned long long id, unsigned int step,
@@ -103,8 +127,8 @@ __device__ __forceinline__ void deg_step_normals(unsigned long long seed, unsign
         const deg_u4 r = deg_philox4x32_10(deg_u4{step, (unsigned int)b, (unsigned int)id, (unsigned int)(id >> 32)},
                                            (unsigned int)seed, (unsigned int)(seed >> 32));
         float z[4];
-        deg_bm_pair(r.x, r.y, z[0], z[1]);
-        if (4 * b + 2 < M) deg_bm_pair(r.z, r.w, z[2], z[3]);
+        DEG_BM_PAIR(r.x, r.y, z[0], z[1]);
+        if (4 * b + 2 < M) DEG_BM_PAIR(r.z, r.w, z[2], z[3]);
         _Pragma("unroll") for (int j = 0; j < 4; ++j) if (4 * b + j < M) w[4 * b + j] = z[j];
     }
 }

@@ -18,8 +18,9 @@
 //   DEG_DIAG_NOISE       1: g returns DEG_N entries multiplied elementwise with the increments;
 //                        0: g returns the DEG_N x DEG_NM matrix, row-major g[r * DEG_NM + c]
 //   deg_rhs(u, p, t, du) drift f, deg_noise(u, p, t, g) diffusion g
-// Without DEG_F_PARITY the host compiles with FP contraction; the increments themselves are always
-// computed without it.
+// Without DEG_F_PARITY the host compiles with FP contraction and DEG_SDE_FASTNORM, which draws the
+// normals through the hardware transcendentals (deg_philox.h); with it every operation, the normals
+// included, is the oracle's Float32 sequence.
 #pragma once
 
 #define DEG_RET_SUCCESS 0

@@ -187,6 +187,38 @@ def test_sde_contract_fast(deg, torch_cuda):
         np.testing.assert_allclose(g["us"][1], o["us"][1], rtol=1e-4)
 
 
+def test_sde_fast_normals(deg, torch_cuda):
+    """contract="fast": the hardware Box-Muller draws the oracle's normals to a few ulps (the Brownian
+    path of f = 0, g = sigma within a relative 1e-5 after 64 steps), and 1M endpoints of a standard
+    Brownian motion have mean 0, variance 1 and kurtosis 3 within 5 standard errors."""
+    torch = torch_cuda
+    N = 4096
+    u0, p = sde_inputs("bm", N, 12)
+    for method in ("em", "siea"):
+        g = gpu_sde(deg, torch, "bm", method, u0, p, (0.0, 1.0), F32(1.0 / 64), dev=deg.HIPDevice(contract="fast"),
+                    seed=5, save_everystep=False)
+        o = oracle_sde("bm", method, u0, p, (0.0, 1.0), F32(1.0 / 64), 2, seed=5, save_everystep=False)
+        scale = p[0] * 1.0  # sigma sqrt(T)
+        np.testing.assert_allclose(g["us"][1, 0] / scale, o["us"][1, 0] / scale, rtol=0, atol=1e-5)
+        assert (g["us"][1] != o["us"][1]).any() or method == "siea"  # not the exact path
+    N = 1_000_000
+    from diffeqgpu_amd import systems
+
+    f = systems.SDE_SYSTEMS["bm"][0]
+    U0 = torch.zeros((1, N), dtype=torch.float32, device="cuda")
+    P = torch.ones((1, N), dtype=torch.float32, device="cuda")
+    probs = deg.HIPProbVector.from_arrays(f, U0, P, (0.0, 1.0), deg.HIPDevice(contract="fast"), seed=77)
+    prob = deg.SDEProblem(f, None, [0.0], (0.0, 1.0), [1.0], seed=77)
+    print("[gpu] fast normals moments", flush=True)
+    res = deg.vectorized_solve(probs, prob, deg.GPUEM(), dt=F32(1.0 / 16), save_everystep=False)
+    x = res.us_soa[1, 0].double()
+    mu = float(x.mean())
+    c = x - mu
+    v = float((c * c).mean())
+    ku = float((c ** 4).mean()) / v ** 2 - 3.0
+    assert abs(mu) < 5 / math.sqrt(N) and abs(v - 1) < 5 * math.sqrt(2 / N) and abs(ku) < 5 * math.sqrt(24 / N)
+
+
 def test_sde_batching_invariance(deg, torch_cuda):
     """solve() numbers the noise streams by ensemble index: batch_size does not change any trajectory."""
     from diffeqgpu_amd import systems
