@@ -190,7 +190,10 @@ int64_t deg_jit_source(const char *rhs_src, int n, int np, int method, uint32_t 
 
 /* Launch one ensemble solve on args->stream (asynchronous unless elapsed_ms is set).
  * Replaces the kernel launch of vectorized_solve / vectorized_asolve
- * (lowerlevel_solve.jl:119-123, :339-343). */
+ * (lowerlevel_solve.jl:119-123, :339-343). A persistent kernel (DEG_F_PERSISTENT) keeps its work
+ * counters in the kernel object, so launches of one kernel object must be ordered: one stream, or
+ * streams synchronised between the launches (a second kernel object of the same configuration
+ * has its own counters). */
 int deg_solve(deg_kernel *k, const deg_solve_args *args);
 
 /* Wait for all work on the context's device. */
