@@ -455,9 +455,28 @@ __device__ __noinline__ void deg_interp(float th, float dt, float tprev, const f
 #endif
 
 // ------------------------------------------------------------------------------------
-// Rosenbrock23 (gpu_rosenbrock23_perform_step.jl:33-60, :109-153) with an FP64 W = I - gamma J
+// Rosenbrock23 (gpu_rosenbrock23_perform_step.jl:33-60, :109-153) with an FP64 W = M - gamma J
 // ------------------------------------------------------------------------------------
+#ifndef DEG_MASS
+#define DEG_MASS 0
+#endif
 #if DEG_METHOD == 2 || DEG_METHOD == 4 || DEG_METHOD == 5 || DEG_METHOD == 6 || DEG_METHOD == 7
+#if DEG_MASS
+// The constant mass matrix M of the traced source (DEG_MASS_ENTRIES, row-major): out = M v, each
+// row summed left to right (utils.jl:42-57 turns it into an SMatrix; the implicit methods use
+// it in W and in their right-hand sides).
+__device__ __forceinline__ float deg_mass_at(int r, int c) {
+    constexpr float M[DEG_N * DEG_N] = {DEG_MASS_ENTRIES};
+    return M[r * DEG_N + c];
+}
+__device__ __forceinline__ void deg_mass_mul(const float (&v)[DEG_N], float (&out)[DEG_N]) {
+    DEG_FOR(r) {
+        float acc = deg_mass_at(r, 0) * v[0];
+        _Pragma("unroll") for (int c = 1; c < DEG_N; ++c) acc = acc + deg_mass_at(r, c) * v[c];
+        out[r] = acc;
+    }
+}
+#endif
 // LU with partial pivoting; row swaps are applied with compile-time indices so W stays
 // in VGPRs.
 __device__ __forceinline__ void w_lu(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N]) {
@@ -517,12 +536,16 @@ __device__ __forceinline__ void w_solve(const double (&W)[DEG_N][DEG_N], const i
 
 #if DEG_METHOD == 2
 #define DEG_NK 2
-// W = I - gamma J
+// W = M - gamma J (M = I without a mass matrix)
 __device__ __forceinline__ void w_factor(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N], float gam,
                                          const float (&J)[DEG_N * DEG_N]) {
     const double g = (double)gam;
     DEG_FOR(r) {
+#if DEG_MASS
+        _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (double)deg_mass_at(r, c) - g * (double)J[r * DEG_N + c];
+#else
         _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (r == c ? 1.0 : 0.0) - g * (double)J[r * DEG_N + c];
+#endif
     }
     w_lu(W, piv);
 }
@@ -548,7 +571,12 @@ __device__ __forceinline__ float deg_rb23(const float (&uprev)[DEG_N], const flo
     w_solve(W, piv, rhs, k[1]);
     DEG_FOR(c) tmp[c] = uprev[c] + dto2 * k[1][c];
     deg_rhs(tmp, p, t + dto2, F1);
+#if DEG_MASS
+    deg_mass_mul(k[1], x);  // F1 - M k1 (gpu_rosenbrock23_perform_step.jl:54-58)
+    DEG_FOR(c) rhs[c] = F1[c] - x[c];
+#else
     DEG_FOR(c) rhs[c] = F1[c] - k[1][c];
+#endif
     w_solve(W, piv, rhs, x);
     DEG_FOR(c) k[2][c] = x[c] + k[1][c];
     DEG_FOR(c) unew[c] = uprev[c] + dt * k[2][c];
@@ -557,7 +585,17 @@ __device__ __forceinline__ float deg_rb23(const float (&uprev)[DEG_N], const flo
     const float e32 = RB23_E32;
     float F2[DEG_N], k3[DEG_N], e[DEG_N];
     deg_rhs(unew, p, t + dt, F2);
+#if DEG_MASS
+    {
+        // F2 - M (e32 k2 + 2 k1) + e32 F1 + 2 F0 + dt dT (:144-148)
+        float v[DEG_N], mv[DEG_N];
+        DEG_FOR(c) v[c] = e32 * k[2][c] + 2.0f * k[1][c];
+        deg_mass_mul(v, mv);
+        DEG_FOR(c) rhs[c] = (((F2[c] - mv[c]) + e32 * F1[c]) + 2.0f * F0[c]) + dt * dT[c];
+    }
+#else
     DEG_FOR(c) rhs[c] = ((F2[c] - e32 * (k[2][c] - F1[c])) - 2.0f * (k[1][c] - F0[c])) + dt * dT[c];
+#endif
     w_solve(W, piv, rhs, k3);
     DEG_FOR(c) e[c] = dto6 * ((k[1][c] - 2.0f * k[2][c]) + k3[c]);
 #if DEG_FASTPOW
@@ -587,6 +625,19 @@ __device__ __forceinline__ void deg_interp(float th, float dt, float tprev, cons
 // from the step's end state y1 (stiff/interpolants.jl:1-23)
 // ------------------------------------------------------------------------------------
 #if DEG_METHOD == 4 || DEG_METHOD == 5
+// lt = -(A + C), with C the stage's combination of earlier stages (times M with a mass matrix:
+// linsolve_tmp = du + dtd dT + M (dtC k), gpu_rodas4_perform_step.jl:77-101)
+#if DEG_MASS
+#define DEG_RODAS_LT(A, COMBO)                  \
+    {                                           \
+        float cm_[DEG_N], mc_[DEG_N];           \
+        DEG_FOR(c) cm_[c] = (COMBO);            \
+        deg_mass_mul(cm_, mc_);                 \
+        DEG_FOR(c) lt[c] = -((A) + mc_[c]);     \
+    }
+#else
+#define DEG_RODAS_LT(A, COMBO) DEG_FOR(c) lt[c] = -((A) + (COMBO));
+#endif
 #if DEG_METHOD == 4
 #define DEG_NK 2
 #define RO(x) RO4_##x
@@ -613,9 +664,15 @@ __device__ __forceinline__ float deg_rodas(const float (&uprev)[DEG_N], const fl
     double W[DEG_N][DEG_N];
     int piv[DEG_N];
     {
-        const double ig = (double)(1.0f / dtgamma);
+        // W = J - M inv(dt gamma) (gpu_rodas4_perform_step.jl:66-67), the product with M in Float32
+        const float igf = 1.0f / dtgamma;
+        const double ig = (double)igf;
         DEG_FOR(r) {
+#if DEG_MASS
+            _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (double)Jm[r * DEG_N + c] - (double)(deg_mass_at(r, c) * igf);
+#else
             _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (double)Jm[r * DEG_N + c] - (r == c ? ig : 0.0);
+#endif
         }
         w_lu(W, piv);
     }
@@ -627,28 +684,28 @@ __device__ __forceinline__ float deg_rodas(const float (&uprev)[DEG_N], const fl
     DEG_FOR(c) u[c] = uprev[c] + RO(a21) * K1[c];
     deg_rhs(u, p, t + RO(c2) * dt, du);
     // stage 2
-    DEG_FOR(c) lt[c] = -((du[c] + dtd2 * dT[c]) + dtC21 * K1[c]);
+    DEG_RODAS_LT((du[c] + dtd2 * dT[c]), dtC21 * K1[c])
     w_solve(W, piv, lt, K2);
     DEG_FOR(c) u[c] = (uprev[c] + RO(a31) * K1[c]) + RO(a32) * K2[c];
     deg_rhs(u, p, t + RO(c3) * dt, du);
     // stage 3
-    DEG_FOR(c) lt[c] = -((du[c] + dtd3 * dT[c]) + (dtC31 * K1[c] + dtC32 * K2[c]));
+    DEG_RODAS_LT((du[c] + dtd3 * dT[c]), (dtC31 * K1[c] + dtC32 * K2[c]))
     w_solve(W, piv, lt, K3);
     DEG_FOR(c) u[c] = ((uprev[c] + RO(a41) * K1[c]) + RO(a42) * K2[c]) + RO(a43) * K3[c];
     deg_rhs(u, p, t + RO(c4) * dt, du);
     // stage 4
-    DEG_FOR(c) lt[c] = -((du[c] + dtd4 * dT[c]) + ((dtC41 * K1[c] + dtC42 * K2[c]) + dtC43 * K3[c]));
+    DEG_RODAS_LT((du[c] + dtd4 * dT[c]), ((dtC41 * K1[c] + dtC42 * K2[c]) + dtC43 * K3[c]))
     w_solve(W, piv, lt, K4);
     DEG_FOR(c) u[c] = (((uprev[c] + RO(a51) * K1[c]) + RO(a52) * K2[c]) + RO(a53) * K3[c]) + RO(a54) * K4[c];
 #if DEG_METHOD == 4
     deg_rhs(u, p, t + dt, du);
     // stage 5 (its C terms in the reference's order 2, 4, 1, 3)
-    DEG_FOR(c) lt[c] = -(du[c] + (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]));
+    DEG_RODAS_LT(du[c], (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]))
     w_solve(W, piv, lt, K5);
     DEG_FOR(c) u[c] = u[c] + K5[c];
     deg_rhs(u, p, t + dt, du);
     // stage 6 (order 1, 2, 5, 4, 3)
-    DEG_FOR(c) lt[c] = -(du[c] + ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC65 * K5[c]) + dtC64 * K4[c]) + dtC63 * K3[c]));
+    DEG_RODAS_LT(du[c], ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC65 * K5[c]) + dtC64 * K4[c]) + dtC63 * K3[c]))
     w_solve(W, piv, lt, K6);
     DEG_FOR(c) unew[c] = u[c] + K6[c];
     DEG_FOR(c) {
@@ -665,22 +722,22 @@ __device__ __forceinline__ float deg_rodas(const float (&uprev)[DEG_N], const fl
     float K7[DEG_N], K8[DEG_N];
     deg_rhs(u, p, t + RO(c5) * dt, du);
     // stage 5 (its C terms in the reference's order 2, 4, 1, 3)
-    DEG_FOR(c) lt[c] = -((du[c] + dtd5 * dT[c]) + (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]));
+    DEG_RODAS_LT((du[c] + dtd5 * dT[c]), (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]))
     w_solve(W, piv, lt, K5);
     DEG_FOR(c) u[c] = ((((uprev[c] + RO(a61) * K1[c]) + RO(a62) * K2[c]) + RO(a63) * K3[c]) + RO(a64) * K4[c]) + RO(a65) * K5[c];
     deg_rhs(u, p, t + dt, du);
     // stage 6
-    DEG_FOR(c) lt[c] = -(du[c] + ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC63 * K3[c]) + dtC64 * K4[c]) + dtC65 * K5[c]));
+    DEG_RODAS_LT(du[c], ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC63 * K3[c]) + dtC64 * K4[c]) + dtC65 * K5[c]))
     w_solve(W, piv, lt, K6);
     DEG_FOR(c) u[c] = u[c] + K6[c];
     deg_rhs(u, p, t + dt, du);
     // stage 7
-    DEG_FOR(c) lt[c] = -(du[c] + (((((dtC71 * K1[c] + dtC72 * K2[c]) + dtC73 * K3[c]) + dtC74 * K4[c]) + dtC75 * K5[c]) + dtC76 * K6[c]));
+    DEG_RODAS_LT(du[c], (((((dtC71 * K1[c] + dtC72 * K2[c]) + dtC73 * K3[c]) + dtC74 * K4[c]) + dtC75 * K5[c]) + dtC76 * K6[c]))
     w_solve(W, piv, lt, K7);
     DEG_FOR(c) u[c] = u[c] + K7[c];
     deg_rhs(u, p, t + dt, du);
     // stage 8
-    DEG_FOR(c) lt[c] = -(du[c] + ((((((dtC81 * K1[c] + dtC82 * K2[c]) + dtC83 * K3[c]) + dtC84 * K4[c]) + dtC85 * K5[c]) + dtC86 * K6[c]) + dtC87 * K7[c]));
+    DEG_RODAS_LT(du[c], ((((((dtC81 * K1[c] + dtC82 * K2[c]) + dtC83 * K3[c]) + dtC84 * K4[c]) + dtC85 * K5[c]) + dtC86 * K6[c]) + dtC87 * K7[c]))
     w_solve(W, piv, lt, K8);
     DEG_FOR(c) unew[c] = u[c] + K8[c];
     DEG_FOR(c) {
@@ -729,14 +786,18 @@ __device__ __forceinline__ void deg_interp(float th, float dt, float tprev, cons
 #endif
 #define DEG_KV_NEWTON_MAX 30
 #define DEG_KV_NEWTON_TOL (100.0f * 1.1920928955078125e-7f)  // 100 eps(Float32)
-// W = -I + (gamma dt) J(x, tc), LU-factored in FP64
+// W = -M + (gamma dt) J(x, tc) (M = I without a mass matrix), LU-factored in FP64
 __device__ __forceinline__ void kv_w(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N], float gdt,
                                      const float (&x)[DEG_N], const float* p, float tc) {
     float Jm[DEG_N * DEG_N];
     deg_jac(x, p, tc, Jm);
     const double g = (double)gdt;
     DEG_FOR(r) {
+#if DEG_MASS
+        _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = -(double)deg_mass_at(r, c) + g * (double)Jm[r * DEG_N + c];
+#else
         _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (r == c ? -1.0 : 0.0) + g * (double)Jm[r * DEG_N + c];
+#endif
     }
     w_lu(W, piv);
 }
@@ -753,12 +814,23 @@ __device__ __forceinline__ void kv_newton(float (&z)[DEG_N], const float (&tmp)[
         int piv[DEG_N];
         kv_w(W, piv, gdt, x, p, tc);
         deg_rhs(x, p, tc, fe);
+#if DEG_MASS
+        float mz[DEG_N];
+        deg_mass_mul(z, mz);  // dt f - M z (nlsolve/utils.jl:17, :21)
+        DEG_FOR(q) r[q] = dtnl * fe[q] - mz[q];
+#else
         DEG_FOR(q) r[q] = dtnl * fe[q] - z[q];
+#endif
         w_solve(W, piv, r, dz);
         DEG_FOR(q) z[q] = z[q] - dz[q];
         DEG_FOR(q) x[q] = tmp[q] + KV(gamma) * z[q];
         deg_rhs(x, p, tc, fe);
+#if DEG_MASS
+        deg_mass_mul(z, mz);
+        DEG_FOR(q) r[q] = dtnl * fe[q] - mz[q];
+#else
         DEG_FOR(q) r[q] = dtnl * fe[q] - z[q];
+#endif
         if (rms_norm(r) < DEG_KV_NEWTON_TOL) break;
     }
 }

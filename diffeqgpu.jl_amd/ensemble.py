@@ -18,7 +18,7 @@ import numpy as np
 from . import _lib
 from .algorithms import EnsembleGPUKernel, GPUODEAlgorithm, GPUSDEAlgorithm
 from .callbacks import merged_callback
-from .lowerlevel import vectorized_asolve, vectorized_solve
+from .lowerlevel import check_mass, vectorized_asolve, vectorized_solve
 from .problems import EnsembleProblem, adapt, make_prob_compatible
 
 
@@ -154,6 +154,8 @@ def batch_solve_up_kernel(ensembleprob, probs, alg, ensemblealg, I, adaptive, **
                                          bool(kwargs.pop("merge_callbacks", False)))
     if kwargs["callback"] is None:
         kwargs.pop("callback")
+    if isinstance(alg, GPUODEAlgorithm):
+        check_mass(probs[0].f, alg)
     if adaptive and isinstance(alg, GPUSDEAlgorithm):
         # the adaptive SDE entry point raises (lowerlevel_solve.jl:349-356); say so before any device work
         raise NotImplementedError("Adaptive time-stepping is not supported yet with GPUEM.")

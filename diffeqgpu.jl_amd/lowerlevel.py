@@ -101,6 +101,13 @@ def _rep_tspan(prob) -> tuple:
     raise TypeError("prob must be an ODEProblem or SDEProblem")
 
 
+def check_mass(f, alg):
+    """A mass matrix is solved by the implicit methods only (the explicit kernels have no W)."""
+    if getattr(f, "mass_matrix", None) is not None and not isinstance(alg, GPUODEImplicitAlgorithm):
+        raise ValueError(f"a mass matrix needs an implicit method (GPURosenbrock23, GPURodas4, GPURodas5P, "
+                         f"GPUKvaerno3, GPUKvaerno5), not {type(alg).__name__}")
+
+
 def _callback_set(callback):
     """The GPU CallbackSet for the kernel (SciML callbacks are converted as solve.jl:386-397
     does), or None when there are no callbacks."""
@@ -165,6 +172,7 @@ def _launch(probs: HIPProbVector, alg, adaptive: bool, *, dt, abstol, reltol, sa
                 raise ValueError("tstops and callbacks do not combine with the learned proposer, step logs, "
                                  "tail compaction or batched saves")
         with_jac = isinstance(alg, GPUODEImplicitAlgorithm)
+        check_mass(probs.f, alg)
         src, _flops = probs.f.hip_source(probs.n, probs.np, with_jac)
         if callback is not None:
             src = src + emit_callbacks(callback, probs.n, probs.np)

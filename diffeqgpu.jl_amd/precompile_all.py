@@ -29,6 +29,8 @@ def variants():
     ]
     out = []
     for name, (f, n, np_, _sid) in SYSTEMS.items():
+        if f.mass_matrix is not None:
+            continue  # the mass-matrix systems are warmed below with the implicit methods
         for m in _METHODS:
             src, _ = f.hip_source(n, np_, m == _lib.DEG_RB23)
             for dev, adaptive in devs:
@@ -54,7 +56,7 @@ def variants():
             (HIPDevice(contract="off", powmode="det", persistent=False), True),
             (HIPDevice(contract="off", powmode="det", persistent=True), True)]
     for name, (f, n, np_, _sid) in SYSTEMS.items():
-        for m in _METHODS_MORE:
+        for m in (_IMPLICIT if f.mass_matrix is not None else _METHODS_MORE):
             src, _ = f.hip_source(n, np_, m in _IMPLICIT)
             for dev, adaptive in more:
                 out.append((name, src, n, np_, m, dev.flags(adaptive)))

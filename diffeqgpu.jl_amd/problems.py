@@ -21,10 +21,14 @@ from .algorithms import HIPDevice
 
 
 class ODEFunction:
-    """Out-of-place RHS ``f(u, p, t) -> SVector`` with optional analytic ``jac`` / ``tgrad``.
+    """Out-of-place RHS ``f(u, p, t) -> SVector`` with optional analytic ``jac`` / ``tgrad`` and a
+    constant ``mass_matrix`` M (M u' = f(u, p, t)).
 
     ``jac(u, p, t)`` returns n rows of n entries (row r, column c = d f_r / d u_c), or a flat
-    row-major sequence. Only the identity mass matrix is supported.
+    row-major sequence. ``mass_matrix`` None (or the string "I") is the identity, SciMLBase's
+    UniformScaling default; an n x n array (even the identity matrix) is applied as a matrix by
+    the implicit methods, as the reference does with an SMatrix (utils.jl:42-57). Singular M
+    gives a DAE (algebraic rows).
     """
 
     def __init__(self, f: Callable, jac: Optional[Callable] = None, tgrad: Optional[Callable] = None,
@@ -32,14 +36,24 @@ class ODEFunction:
         if isinstance(f, ODEFunction):
             jac = jac or f.jac
             tgrad = tgrad or f.tgrad
+            if mass_matrix is None:
+                mass_matrix = f.mass_matrix
             f = f.f
+        if isinstance(mass_matrix, str):
+            if mass_matrix != "I":
+                raise ValueError("mass_matrix must be None, 'I' or an n x n matrix")
+            mass_matrix = None
         if mass_matrix is not None:
             m = np.asarray(mass_matrix, dtype=np.float64)
-            if not (m.ndim == 2 and m.shape[0] == m.shape[1] and np.array_equal(m, np.eye(m.shape[0]))):
-                raise NotImplementedError("only the identity mass matrix is supported")
+            if m.ndim != 2 or m.shape[0] != m.shape[1]:
+                raise ValueError("mass_matrix must be square")
+            if not np.all(np.isfinite(m)):
+                raise ValueError("mass_matrix must be finite")
+            mass_matrix = m.astype(np.float32)
         self.f = f
         self.jac = jac
         self.tgrad = tgrad
+        self.mass_matrix = mass_matrix
         self._src_cache = {}
 
     def __call__(self, u, p, t):
@@ -49,10 +63,13 @@ class ODEFunction:
         return self.jac is not None
 
     def hip_source(self, n: int, np_: int, with_jac: bool) -> tuple[str, int]:
-        """Traced HIP C for deg_rhs (+ deg_jac/deg_tgrad), and the RHS flop count."""
+        """Traced HIP C for deg_rhs (+ deg_jac/deg_tgrad and the mass matrix), and the RHS flop count."""
         key = (n, np_, with_jac)
         if key not in self._src_cache:
-            self._src_cache[key] = tracer.emit_rhs(self.f, n, np_, self.jac, self.tgrad, with_jac)
+            src, flops = tracer.emit_rhs(self.f, n, np_, self.jac, self.tgrad, with_jac)
+            if self.mass_matrix is not None:
+                src = tracer.emit_mass(self.mass_matrix, n) + src
+            self._src_cache[key] = (src, flops)
         return self._src_cache[key]
 
 

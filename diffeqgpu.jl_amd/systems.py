@@ -11,6 +11,12 @@ same association order (u and p are 0-based here, 1-based in the Julia sources):
   tdep       u' = p t u
   rot        u1' = -p u2, u2' = p u1
   ball       u1' = u2, u2' = -p1 (test/gpu_kernel_de/gpu_ode_continuous_callbacks.jl:6-10)
+  rober_dae  Robertson as a DAE with mass matrix diag(1, 1, 0) and the algebraic row
+             y1 + y2 + y3 - 1 (test/gpu_kernel_de/stiff_ode/gpu_ode_mass_matrix.jl:5-27)
+  robertson_eye  the ODE form with an explicit identity mass matrix (the M branches of the
+             implicit methods)
+  dae2       u1' = -0.04 u1 + 1e4 u2, 0 = u1 + u2 - 1, M = diag(1, 0)
+             (test/gpu_kernel_de/stiff_ode/gpu_ode_modelingtoolkit_dae.jl:22-44)
 and the SDE problems (drift f, diffusion g; SDE_SYSTEMS):
   gbm        f = p1 u, g = p2 u (test/gpu_kernel_de/gpu_sde_convergence.jl:8-10)
   gbm1       f = u, g = u (test/gpu_kernel_de/gpu_sde_regression.jl:13-15)
@@ -39,6 +45,10 @@ def lorenz_jac(u, p, t):
 
 def zero_tgrad3(u, p, t):
     return SVector(0.0, 0.0, 0.0)
+
+
+def zero_tgrad2(u, p, t):
+    return SVector(0.0, 0.0)
 
 
 def decay(u, p, t):
@@ -93,6 +103,31 @@ def ball_jac(u, p, t):
     return [[0.0, 1.0], [0.0, 0.0]]
 
 
+def rober_dae(u, p, t):
+    k1, k2, k3 = p[0], p[1], p[2]
+    return SVector(-k1 * u[0] + k3 * u[1] * u[2],
+                   k1 * u[0] - k2 * u[1] ** 2 - k3 * u[1] * u[2],
+                   u[0] + u[1] + u[2] - 1.0)
+
+
+def rober_dae_jac(u, p, t):
+    k1, k2, k3 = p[0], p[1], p[2]
+    return [[-k1, k3 * u[2], k3 * u[1]],
+            [k1, -2.0 * k2 * u[1] - k3 * u[2], -k3 * u[1]],
+            [1.0, 1.0, 1.0]]
+
+
+def dae2(u, p, t):
+    return SVector(-0.04 * u[0] + 1.0e4 * u[1], u[0] + u[1] - 1.0)
+
+
+def dae2_jac(u, p, t):
+    return [[-0.04, 1.0e4], [1.0, 1.0]]
+
+
+MASS_DIAG110 = ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 0.0))
+EYE3 = ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0))
+
 # (ODEFunction, n, np, oracle system id)
 SYSTEMS = {
     "lorenz": (ODEFunction(lorenz, jac=lorenz_jac, tgrad=zero_tgrad3), 3, 3, 0),
@@ -104,7 +139,12 @@ SYSTEMS = {
     "tdep": (ODEFunction(tdep), 1, 1, 6),
     "rot": (ODEFunction(rot), 2, 1, 7),
     "ball": (ODEFunction(ball, jac=ball_jac), 2, 1, 8),
+    "rober_dae": (ODEFunction(rober_dae, jac=rober_dae_jac, tgrad=zero_tgrad3, mass_matrix=MASS_DIAG110), 3, 3, 9),
+    "robertson_eye": (ODEFunction(robertson, jac=robertson_jac, tgrad=zero_tgrad3, mass_matrix=EYE3), 3, 3, 10),
+    "dae2": (ODEFunction(dae2, jac=dae2_jac, tgrad=zero_tgrad2, mass_matrix=((1.0, 0.0), (0.0, 0.0))), 2, 0, 11),
 }
+# the systems with a mass matrix, solved by the implicit methods only
+MASS_SYSTEMS = ("rober_dae", "robertson_eye", "dae2")
 
 
 def gbm_f(u, p, t):

@@ -668,6 +668,18 @@ def emit_noise(f: Callable, g: Callable, n: int, np_: int, m: Optional[int] = No
     return "\n".join(head) + "\n" + src + "\n".join(noise) + "\n", flops + gflops
 
 
+def emit_mass(M, n: int) -> str:
+    """The constant mass matrix as compile-time entries: DEG_MASS and DEG_MASS_ENTRIES (row-major
+    Float32 literals, bit-exact)."""
+    import numpy as np
+
+    m = np.asarray(M, dtype=np.float32)
+    if m.shape != (n, n):
+        raise TraceError(f"mass_matrix must be {n}x{n}, got {m.shape}")
+    ents = ", ".join(f"__builtin_bit_cast(float, 0x{int(v):08x}u)" for v in m.reshape(-1).view(np.uint32))
+    return f"#define DEG_MASS 1\n#define DEG_MASS_ENTRIES {ents}\n"
+
+
 def _flatten_matrix(out, n):
     """Accept a nested row list [[J11, J12, ...], ...] or a flat row-major sequence."""
     if len(out) == n and all(isinstance(r, (list, tuple)) for r in out):

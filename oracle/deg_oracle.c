@@ -441,18 +441,34 @@ static void w_lu(olu *F, int n) {
     }
 }
 
-/* Rosenbrock23: W = I - gamma J */
-static void w_factor(olu *F, int n, float gam, const float *J) {
+/* M v for a mass matrix (row sums left to right, unfused), the identity when M is NULL */
+static void mass_mul(const float *M, int n, const float *v, float *out) {
+    for (int r = 0; r < n; ++r) {
+        if (!M) {
+            out[r] = v[r];
+            continue;
+        }
+        float acc = M[r * n] * v[0];
+        for (int c = 1; c < n; ++c) acc = acc + M[r * n + c] * v[c];
+        out[r] = acc;
+    }
+}
+
+/* Rosenbrock23: W = M - gamma J (gpu_rosenbrock23_perform_step.jl:44-45), M = I without a mass matrix */
+static void w_factor(olu *F, int n, float gam, const float *J, const float *M) {
     double g = (double)gam;
     for (int r = 0; r < n; ++r)
-        for (int c = 0; c < n; ++c) F->lu[r][c] = (r == c ? 1.0 : 0.0) - g * (double)J[r * n + c];
+        for (int c = 0; c < n; ++c)
+            F->lu[r][c] = (M ? (double)M[r * n + c] : (r == c ? 1.0 : 0.0)) - g * (double)J[r * n + c];
     w_lu(F, n);
 }
 
-/* Rodas: W = J - I / (dt gamma), the reciprocal taken in Float32 as inv(dtgamma) */
-static void w_factor_rodas(olu *F, int n, float invg, const float *J) {
+/* Rodas: W = J - M inv(dt gamma) (gpu_rodas4_perform_step.jl:66-67), the reciprocal and its product
+ * with M taken in Float32 */
+static void w_factor_rodas(olu *F, int n, float invg, const float *J, const float *M) {
     for (int r = 0; r < n; ++r)
-        for (int c = 0; c < n; ++c) F->lu[r][c] = (double)J[r * n + c] - (r == c ? (double)invg : 0.0);
+        for (int c = 0; c < n; ++c)
+            F->lu[r][c] = (double)J[r * n + c] - (M ? (double)(M[r * n + c] * invg) : (r == c ? (double)invg : 0.0));
     w_lu(F, n);
 }
 
@@ -482,7 +498,7 @@ static void w_solve(const olu *F, int n, const float *b, float *x) {
 static void rodas4_step(const osys *S, const float *p, const float *uprev, float t, float dt, float k[][OMAXN],
                         float *unew, float *kerr) {
     const int n = S->n;
-    float Jm[OMAXN * OMAXN], dT[OMAXN], du[OMAXN], lt[OMAXN] = {0}, u[OMAXN];
+    float Jm[OMAXN * OMAXN], dT[OMAXN], du[OMAXN], lt[OMAXN] = {0}, u[OMAXN], cm[OMAXN] = {0}, mc[OMAXN] = {0};
     float K1[OMAXN], K2[OMAXN], K3[OMAXN], K4[OMAXN], K5[OMAXN], K6[OMAXN];
     S->jac(uprev, p, t, Jm);
     S->tgrad(uprev, p, t, dT);
@@ -494,31 +510,40 @@ static void rodas4_step(const osys *S, const float *p, const float *uprev, float
     const float dtd1 = dt * R4(d1), dtd2 = dt * R4(d2), dtd3 = dt * R4(d3), dtd4 = dt * R4(d4);
     const float dtgamma = dt * R4(gamma);
     olu F;
-    w_factor_rodas(&F, n, 1.0f / dtgamma, Jm);
+    w_factor_rodas(&F, n, 1.0f / dtgamma, Jm, S->M);
     S->f(uprev, p, t, du);
     for (int c = 0; c < n; ++c) lt[c] = -(du[c] + dtd1 * dT[c]);
     w_solve(&F, n, lt, K1);
     for (int c = 0; c < n; ++c) u[c] = uprev[c] + R4(a21) * K1[c];
     S->f(u, p, t + R4(c2) * dt, du);
-    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd2 * dT[c]) + dtC21 * K1[c]);
+    for (int c = 0; c < n; ++c) cm[c] = dtC21 * K1[c];
+    mass_mul(S->M, n, cm, mc);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd2 * dT[c]) + mc[c]);
     w_solve(&F, n, lt, K2);
     for (int c = 0; c < n; ++c) u[c] = (uprev[c] + R4(a31) * K1[c]) + R4(a32) * K2[c];
     S->f(u, p, t + R4(c3) * dt, du);
-    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd3 * dT[c]) + (dtC31 * K1[c] + dtC32 * K2[c]));
+    for (int c = 0; c < n; ++c) cm[c] = (dtC31 * K1[c] + dtC32 * K2[c]);
+    mass_mul(S->M, n, cm, mc);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd3 * dT[c]) + mc[c]);
     w_solve(&F, n, lt, K3);
     for (int c = 0; c < n; ++c) u[c] = ((uprev[c] + R4(a41) * K1[c]) + R4(a42) * K2[c]) + R4(a43) * K3[c];
     S->f(u, p, t + R4(c4) * dt, du);
-    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd4 * dT[c]) + ((dtC41 * K1[c] + dtC42 * K2[c]) + dtC43 * K3[c]));
+    for (int c = 0; c < n; ++c) cm[c] = ((dtC41 * K1[c] + dtC42 * K2[c]) + dtC43 * K3[c]);
+    mass_mul(S->M, n, cm, mc);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd4 * dT[c]) + mc[c]);
     w_solve(&F, n, lt, K4);
     for (int c = 0; c < n; ++c)
         u[c] = (((uprev[c] + R4(a51) * K1[c]) + R4(a52) * K2[c]) + R4(a53) * K3[c]) + R4(a54) * K4[c];
     S->f(u, p, t + dt, du);
-    for (int c = 0; c < n; ++c) lt[c] = -(du[c] + (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]));
+    for (int c = 0; c < n; ++c) cm[c] = (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]);
+    mass_mul(S->M, n, cm, mc);
+    for (int c = 0; c < n; ++c) lt[c] = -(du[c] + mc[c]);
     w_solve(&F, n, lt, K5);
     for (int c = 0; c < n; ++c) u[c] = u[c] + K5[c];
     S->f(u, p, t + dt, du);
-    for (int c = 0; c < n; ++c)
-        lt[c] = -(du[c] + ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC65 * K5[c]) + dtC64 * K4[c]) + dtC63 * K3[c]));
+    for (int c = 0; c < n; ++c) cm[c] = ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC65 * K5[c]) + dtC64 * K4[c]) + dtC63 * K3[c]);
+    mass_mul(S->M, n, cm, mc);
+    for (int c = 0; c < n; ++c) lt[c] = -(du[c] + mc[c]);
     w_solve(&F, n, lt, K6);
     for (int c = 0; c < n; ++c) {
         unew[c] = u[c] + K6[c];
@@ -531,7 +556,7 @@ static void rodas4_step(const osys *S, const float *p, const float *uprev, float
 static void rodas5p_step(const osys *S, const float *p, const float *uprev, float t, float dt, float k[][OMAXN],
                          float *unew, float *kerr) {
     const int n = S->n;
-    float Jm[OMAXN * OMAXN], dT[OMAXN], du[OMAXN], lt[OMAXN] = {0}, u[OMAXN];
+    float Jm[OMAXN * OMAXN], dT[OMAXN], du[OMAXN], lt[OMAXN] = {0}, u[OMAXN], cm[OMAXN] = {0}, mc[OMAXN] = {0};
     float K1[OMAXN], K2[OMAXN], K3[OMAXN], K4[OMAXN], K5[OMAXN], K6[OMAXN], K7[OMAXN], K8[OMAXN];
     S->jac(uprev, p, t, Jm);
     S->tgrad(uprev, p, t, dT);
@@ -547,45 +572,53 @@ static void rodas5p_step(const osys *S, const float *p, const float *uprev, floa
     const float dtd1 = dt * R5(d1), dtd2 = dt * R5(d2), dtd3 = dt * R5(d3), dtd4 = dt * R5(d4), dtd5 = dt * R5(d5);
     const float dtgamma = dt * R5(gamma);
     olu F;
-    w_factor_rodas(&F, n, 1.0f / dtgamma, Jm);
+    w_factor_rodas(&F, n, 1.0f / dtgamma, Jm, S->M);
     S->f(uprev, p, t, du);
     for (int c = 0; c < n; ++c) lt[c] = -(du[c] + dtd1 * dT[c]);
     w_solve(&F, n, lt, K1);
     for (int c = 0; c < n; ++c) u[c] = uprev[c] + R5(a21) * K1[c];
     S->f(u, p, t + R5(c2) * dt, du);
-    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd2 * dT[c]) + dtC21 * K1[c]);
+    for (int c = 0; c < n; ++c) cm[c] = dtC21 * K1[c];
+    mass_mul(S->M, n, cm, mc);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd2 * dT[c]) + mc[c]);
     w_solve(&F, n, lt, K2);
     for (int c = 0; c < n; ++c) u[c] = (uprev[c] + R5(a31) * K1[c]) + R5(a32) * K2[c];
     S->f(u, p, t + R5(c3) * dt, du);
-    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd3 * dT[c]) + (dtC31 * K1[c] + dtC32 * K2[c]));
+    for (int c = 0; c < n; ++c) cm[c] = (dtC31 * K1[c] + dtC32 * K2[c]);
+    mass_mul(S->M, n, cm, mc);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd3 * dT[c]) + mc[c]);
     w_solve(&F, n, lt, K3);
     for (int c = 0; c < n; ++c) u[c] = ((uprev[c] + R5(a41) * K1[c]) + R5(a42) * K2[c]) + R5(a43) * K3[c];
     S->f(u, p, t + R5(c4) * dt, du);
-    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd4 * dT[c]) + ((dtC41 * K1[c] + dtC42 * K2[c]) + dtC43 * K3[c]));
+    for (int c = 0; c < n; ++c) cm[c] = ((dtC41 * K1[c] + dtC42 * K2[c]) + dtC43 * K3[c]);
+    mass_mul(S->M, n, cm, mc);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd4 * dT[c]) + mc[c]);
     w_solve(&F, n, lt, K4);
     for (int c = 0; c < n; ++c)
         u[c] = (((uprev[c] + R5(a51) * K1[c]) + R5(a52) * K2[c]) + R5(a53) * K3[c]) + R5(a54) * K4[c];
     S->f(u, p, t + R5(c5) * dt, du);
-    for (int c = 0; c < n; ++c)
-        lt[c] = -((du[c] + dtd5 * dT[c]) + (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]));
+    for (int c = 0; c < n; ++c) cm[c] = (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]);
+    mass_mul(S->M, n, cm, mc);
+    for (int c = 0; c < n; ++c) lt[c] = -((du[c] + dtd5 * dT[c]) + mc[c]);
     w_solve(&F, n, lt, K5);
     for (int c = 0; c < n; ++c)
         u[c] = ((((uprev[c] + R5(a61) * K1[c]) + R5(a62) * K2[c]) + R5(a63) * K3[c]) + R5(a64) * K4[c]) + R5(a65) * K5[c];
     S->f(u, p, t + dt, du);
-    for (int c = 0; c < n; ++c)
-        lt[c] = -(du[c] + ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC63 * K3[c]) + dtC64 * K4[c]) + dtC65 * K5[c]));
+    for (int c = 0; c < n; ++c) cm[c] = ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC63 * K3[c]) + dtC64 * K4[c]) + dtC65 * K5[c]);
+    mass_mul(S->M, n, cm, mc);
+    for (int c = 0; c < n; ++c) lt[c] = -(du[c] + mc[c]);
     w_solve(&F, n, lt, K6);
     for (int c = 0; c < n; ++c) u[c] = u[c] + K6[c];
     S->f(u, p, t + dt, du);
-    for (int c = 0; c < n; ++c)
-        lt[c] = -(du[c] + (((((dtC71 * K1[c] + dtC72 * K2[c]) + dtC73 * K3[c]) + dtC74 * K4[c]) + dtC75 * K5[c]) +
-                           dtC76 * K6[c]));
+    for (int c = 0; c < n; ++c) cm[c] = (((((dtC71 * K1[c] + dtC72 * K2[c]) + dtC73 * K3[c]) + dtC74 * K4[c]) + dtC75 * K5[c]) + dtC76 * K6[c]);
+    mass_mul(S->M, n, cm, mc);
+    for (int c = 0; c < n; ++c) lt[c] = -(du[c] + mc[c]);
     w_solve(&F, n, lt, K7);
     for (int c = 0; c < n; ++c) u[c] = u[c] + K7[c];
     S->f(u, p, t + dt, du);
-    for (int c = 0; c < n; ++c)
-        lt[c] = -(du[c] + ((((((dtC81 * K1[c] + dtC82 * K2[c]) + dtC83 * K3[c]) + dtC84 * K4[c]) + dtC85 * K5[c]) +
-                            dtC86 * K6[c]) + dtC87 * K7[c]));
+    for (int c = 0; c < n; ++c) cm[c] = ((((((dtC81 * K1[c] + dtC82 * K2[c]) + dtC83 * K3[c]) + dtC84 * K4[c]) + dtC85 * K5[c]) + dtC86 * K6[c]) + dtC87 * K7[c]);
+    mass_mul(S->M, n, cm, mc);
+    for (int c = 0; c < n; ++c) lt[c] = -(du[c] + mc[c]);
     w_solve(&F, n, lt, K8);
     for (int c = 0; c < n; ++c) {
         unew[c] = u[c] + K8[c];
@@ -612,13 +645,14 @@ static void rodas_interp(int method, float th, int n, const float *y0, const flo
 /* ------------------------------------------------------------------------------------ */
 /* Kvaerno3 / Kvaerno5 (gpu_kvaerno3_perform_step.jl, gpu_kvaerno5_perform_step.jl)             */
 /* ------------------------------------------------------------------------------------ */
-/* W = -I + (gamma dt) J(x, tc) (nlsolve/type.jl:129-140), factored in FP64 */
+/* W = -M + (gamma dt) J(x, tc) (nlsolve/type.jl:129-140), M = I without a mass matrix, factored in FP64 */
 static void kv_w(olu *F, const osys *S, const float *p, float gdt, const float *x, float tc) {
     const int n = S->n;
     float Jm[OMAXN * OMAXN];
     S->jac(x, p, tc, Jm);
     for (int r = 0; r < n; ++r)
-        for (int c = 0; c < n; ++c) F->lu[r][c] = (r == c ? -1.0 : 0.0) + (double)gdt * (double)Jm[r * n + c];
+        for (int c = 0; c < n; ++c)
+            F->lu[r][c] = (S->M ? -(double)S->M[r * n + c] : (r == c ? -1.0 : 0.0)) + (double)gdt * (double)Jm[r * n + c];
     w_lu(F, n);
 }
 
@@ -629,17 +663,19 @@ static void kv_newton(const osys *S, const float *p, float gamma, float *z, cons
     const float tc = tnl + c * dtnl;
     const float gdt = gamma * dtnl;
     for (int it = 0; it < 30; ++it) {
-        float x[OMAXN], fe[OMAXN], r[OMAXN] = {0}, dz[OMAXN];
+        float x[OMAXN], fe[OMAXN], r[OMAXN] = {0}, dz[OMAXN], mz[OMAXN];
         for (int q = 0; q < n; ++q) x[q] = tmp[q] + gamma * z[q];
         olu F;
         kv_w(&F, S, p, gdt, x, tc);
         S->f(x, p, tc, fe);
-        for (int q = 0; q < n; ++q) r[q] = dtnl * fe[q] - z[q];
+        mass_mul(S->M, n, z, mz); /* dt f - M z (nlsolve/utils.jl:17, :21) */
+        for (int q = 0; q < n; ++q) r[q] = dtnl * fe[q] - mz[q];
         w_solve(&F, n, r, dz);
         for (int q = 0; q < n; ++q) z[q] = z[q] - dz[q];
         for (int q = 0; q < n; ++q) x[q] = tmp[q] + gamma * z[q];
         S->f(x, p, tc, fe);
-        for (int q = 0; q < n; ++q) r[q] = dtnl * fe[q] - z[q];
+        mass_mul(S->M, n, z, mz);
+        for (int q = 0; q < n; ++q) r[q] = dtnl * fe[q] - mz[q];
         if (rms_norm(r, n) < 100.0f * 1.1920928955078125e-7f) break;
     }
 }
@@ -1126,13 +1162,15 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                 S->jac(s->uprev, s->p, told, Jm);
                 S->tgrad(s->uprev, s->p, told, dT);
                 olu F;
-                w_factor(&F, n, gam, Jm);
+                w_factor(&F, n, gam, Jm, S->M);
                 S->f(s->uprev, s->p, told, F0);
                 for (int c = 0; c < n; ++c) rhs[c] = F0[c] + gam * dT[c];
                 w_solve(&F, n, rhs, s->rk1);
                 for (int c = 0; c < n; ++c) tmp[c] = s->uprev[c] + dto2 * s->rk1[c];
                 S->f(tmp, s->p, told + dto2, F1);
-                for (int c = 0; c < n; ++c) rhs[c] = F1[c] - s->rk1[c];
+                /* F1 - k1, or F1 - M k1 with a mass matrix (:54-58) */
+                mass_mul(S->M, n, s->rk1, x);
+                for (int c = 0; c < n; ++c) rhs[c] = F1[c] - x[c];
                 w_solve(&F, n, rhs, x);
                 for (int c = 0; c < n; ++c) s->rk2[c] = x[c] + s->rk1[c];
                 for (int c = 0; c < n; ++c) s->u[c] = s->uprev[c] + dts * s->rk2[c];
@@ -1219,19 +1257,29 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                     S->jac(s->uprev, s->p, t, Jm);
                     S->tgrad(s->uprev, s->p, t, dT);
                     olu F;
-                    w_factor(&F, n, gam, Jm);
+                    w_factor(&F, n, gam, Jm, S->M);
                     S->f(s->uprev, s->p, t, F0);
                     for (int c = 0; c < n; ++c) rhs[c] = F0[c] + gam * dT[c];
                     w_solve(&F, n, rhs, s->rk1);
                     for (int c = 0; c < n; ++c) tmp[c] = s->uprev[c] + dto2 * s->rk1[c];
                     S->f(tmp, s->p, t + dto2, F1);
-                    for (int c = 0; c < n; ++c) rhs[c] = F1[c] - s->rk1[c];
+                    /* F1 - k1, or F1 - M k1 with a mass matrix (:129-133) */
+                    mass_mul(S->M, n, s->rk1, x);
+                    for (int c = 0; c < n; ++c) rhs[c] = F1[c] - x[c];
                     w_solve(&F, n, rhs, x);
                     for (int c = 0; c < n; ++c) s->rk2[c] = x[c] + s->rk1[c];
                     for (int c = 0; c < n; ++c) unew[c] = s->uprev[c] + dt * s->rk2[c];
                     S->f(unew, s->p, t + dt, F2);
-                    for (int c = 0; c < n; ++c)
-                        rhs[c] = ((F2[c] - e32 * (s->rk2[c] - F1[c])) - 2.0f * (s->rk1[c] - F0[c])) + dt * dT[c];
+                    if (!S->M) {
+                        for (int c = 0; c < n; ++c)
+                            rhs[c] = ((F2[c] - e32 * (s->rk2[c] - F1[c])) - 2.0f * (s->rk1[c] - F0[c])) + dt * dT[c];
+                    } else {
+                        /* F2 - M (e32 k2 + 2 k1) + e32 F1 + 2 F0 + dt dT (:144-148) */
+                        float v[OMAXN], mv[OMAXN];
+                        for (int c = 0; c < n; ++c) v[c] = e32 * s->rk2[c] + 2.0f * s->rk1[c];
+                        mass_mul(S->M, n, v, mv);
+                        for (int c = 0; c < n; ++c) rhs[c] = (((F2[c] - mv[c]) + e32 * F1[c]) + 2.0f * F0[c]) + dt * dT[c];
+                    }
                     float k3[OMAXN];
                     w_solve(&F, n, rhs, k3);
                     float e[OMAXN] = {0};

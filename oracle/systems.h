@@ -11,13 +11,21 @@
  *   vdp       Van der Pol, u1' = u2, u2' = mu (1 - u1^2) u2 - u1
  *   tdep      u' = p t u (non-autonomous, closed form u0 exp(p t^2 / 2))
  *   rot       u1' = -p u2, u2' = p u1 (closed form rotation)
- * Jacobians are row-major J[r*n + c] = d f_r / d u_c.
+ *   ball      u1' = u2, u2' = -p (test/gpu_kernel_de/gpu_ode_continuous_callbacks.jl:6-10)
+ *   rober_dae Robertson as a DAE, M = diag(1, 1, 0), third equation y1 + y2 + y3 - 1
+ *             (test/gpu_kernel_de/stiff_ode/gpu_ode_mass_matrix.jl:5-27)
+ *   robertson_eye  the ODE form with an explicit identity mass matrix
+ *   dae2      u1' = -0.04 u1 + 1e4 u2, 0 = u1 + u2 - 1, M = diag(1, 0)
+ *             (test/gpu_kernel_de/stiff_ode/gpu_ode_modelingtoolkit_dae.jl:22-44)
+ * Jacobians are row-major J[r*n + c] = d f_r / d u_c; M is the row-major mass matrix or NULL
+ * (no mass matrix: the reference's UniformScaling I).
  */
 #ifndef DEG_ORACLE_SYSTEMS_H
 #define DEG_ORACLE_SYSTEMS_H
 
 enum { OSYS_LORENZ = 0, OSYS_DECAY = 1, OSYS_GROWTH = 2, OSYS_QUAD = 3, OSYS_ROBERTSON = 4,
-       OSYS_VDP = 5, OSYS_TDEP = 6, OSYS_ROT = 7, OSYS_BALL = 8, OSYS_COUNT = 9 };
+       OSYS_VDP = 5, OSYS_TDEP = 6, OSYS_ROT = 7, OSYS_BALL = 8, OSYS_ROBER_DAE = 9, OSYS_ROBERTSON_EYE = 10,
+       OSYS_DAE2 = 11, OSYS_COUNT = 12 };
 
 typedef void (*orhs_fn)(const float *u, const float *p, float t, float *du);
 typedef void (*ojac_fn)(const float *u, const float *p, float t, float *J);
@@ -27,6 +35,7 @@ typedef struct {
     orhs_fn f;
     ojac_fn jac;
     orhs_fn tgrad;
+    const float *M;
 } osys;
 
 static void lorenz_f(const float *u, const float *p, float t, float *du) {
@@ -146,16 +155,48 @@ static void ball_jac(const float *u, const float *p, float t, float *J) {
     J[2] = 0.0f;  J[3] = 0.0f;
 }
 
+/* Robertson DAE: y1' = -k1 y1 + k3 y2 y3, y2' = k1 y1 - k2 y2^2 - k3 y2 y3, 0 = y1 + y2 + y3 - 1 */
+static void rober_dae_f(const float *u, const float *p, float t, float *du) {
+    (void)t;
+    float k1 = p[0], k2 = p[1], k3 = p[2];
+    du[0] = -k1 * u[0] + k3 * u[1] * u[2];
+    du[1] = k1 * u[0] - k2 * (u[1] * u[1]) - k3 * u[1] * u[2];
+    du[2] = u[0] + u[1] + u[2] - 1.0f;
+}
+static void rober_dae_jac(const float *u, const float *p, float t, float *J) {
+    (void)t;
+    float k1 = p[0], k2 = p[1], k3 = p[2];
+    J[0] = -k1;  J[1] = k3 * u[2];                          J[2] = k3 * u[1];
+    J[3] = k1;   J[4] = -2.0f * k2 * u[1] - k3 * u[2];      J[5] = -k3 * u[1];
+    J[6] = 1.0f; J[7] = 1.0f;                               J[8] = 1.0f;
+}
+static void dae2_f(const float *u, const float *p, float t, float *du) {
+    (void)p; (void)t;
+    du[0] = -0.04f * u[0] + 1.0e4f * u[1];
+    du[1] = u[0] + u[1] - 1.0f;
+}
+static void dae2_jac(const float *u, const float *p, float t, float *J) {
+    (void)u; (void)p; (void)t;
+    J[0] = -0.04f; J[1] = 1.0e4f;
+    J[2] = 1.0f;   J[3] = 1.0f;
+}
+static const float M_DIAG10[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+static const float M_DIAG110[9] = {1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+static const float M_EYE3[9] = {1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+
 static const osys OSYSTEMS[OSYS_COUNT] = {
-    {3, 3, lorenz_f, lorenz_jac, zero_tgrad3},
-    {1, 1, decay_f, decay_jac, zero_tgrad1},
-    {1, 1, growth_f, growth_jac, zero_tgrad1},
-    {1, 1, quad_f, quad_jac, zero_tgrad1},
-    {3, 3, robertson_f, robertson_jac, zero_tgrad3},
-    {2, 1, vdp_f, vdp_jac, zero_tgrad2},
-    {1, 1, tdep_f, tdep_jac, tdep_tgrad},
-    {2, 1, rot_f, rot_jac, zero_tgrad2},
-    {2, 1, ball_f, ball_jac, zero_tgrad2},
+    {3, 3, lorenz_f, lorenz_jac, zero_tgrad3, NULL},
+    {1, 1, decay_f, decay_jac, zero_tgrad1, NULL},
+    {1, 1, growth_f, growth_jac, zero_tgrad1, NULL},
+    {1, 1, quad_f, quad_jac, zero_tgrad1, NULL},
+    {3, 3, robertson_f, robertson_jac, zero_tgrad3, NULL},
+    {2, 1, vdp_f, vdp_jac, zero_tgrad2, NULL},
+    {1, 1, tdep_f, tdep_jac, tdep_tgrad, NULL},
+    {2, 1, rot_f, rot_jac, zero_tgrad2, NULL},
+    {2, 1, ball_f, ball_jac, zero_tgrad2, NULL},
+    {3, 3, rober_dae_f, rober_dae_jac, zero_tgrad3, M_DIAG110},
+    {3, 3, robertson_f, robertson_jac, zero_tgrad3, M_EYE3},
+    {2, 0, dae2_f, dae2_jac, zero_tgrad2, M_DIAG10},
 };
 
 /* The same right-hand sides in double precision, for the FP64 oracle. */
@@ -204,6 +245,8 @@ static void ball_fd(const double *u, const double *p, double t, double *du) {
 typedef void (*orhs_fn_d)(const double *u, const double *p, double t, double *du);
 static const orhs_fn_d OSYSTEMS_D[OSYS_COUNT] = {
     lorenz_fd, decay_fd, growth_fd, quad_fd, robertson_fd, vdp_fd, tdep_fd, rot_fd, ball_fd,
+    robertson_fd, robertson_fd, /* the DAE's solution is the ODE form's (not integrated in FP64 here) */
+    rot_fd,                     /* placeholder: the 2 x 2 DAE has no FP64 ODE form here */
 };
 
 #endif
