@@ -48,14 +48,15 @@ class GPUVern9(GPUODEAlgorithm):
 
 class GPUODEImplicitAlgorithm(GPUODEAlgorithm):
     """The methods that use the Jacobian (gpukernel_algorithms.jl: ``GPUODEImplicitAlgorithm{AD}``).
-    ``autodiff=True`` derives the Jacobian by forward-mode differentiation of the traced RHS
-    (unless the problem's ODEFunction supplies ``jac``/``tgrad``); ``autodiff=False`` is not
-    provided (the reference then uses finite differences, alg_utils.jl:17-27)."""
+    The problem's ODEFunction ``jac``/``tgrad`` are used when given; otherwise ``autodiff=True``
+    derives them by forward-mode differentiation of the traced RHS and ``autodiff=False`` by
+    forward differences with step sqrt(eps(Float32)) (alg_utils.jl:17-27, nlsolve/type.jl:129-157)."""
 
     def __init__(self, autodiff: bool = True):
-        if not autodiff:
-            raise NotImplementedError(f"{type(self).__name__}(autodiff=False) (finite-difference Jacobian) is not provided")
-        self.autodiff = True
+        self.autodiff = bool(autodiff)
+
+    def __repr__(self):
+        return f"{type(self).__name__}(autodiff={self.autodiff})"
 
 
 class GPURosenbrock23(GPUODEImplicitAlgorithm):

@@ -47,7 +47,8 @@ def blocks_per_cu_candidates(block: int):
 
 
 def _key(probs, alg, adaptive, nsave_hint) -> str:
-    src, _ = probs.f.hip_source(probs.n, probs.np, isinstance(alg, GPUODEImplicitAlgorithm))
+    implicit = isinstance(alg, GPUODEImplicitAlgorithm)
+    src, _ = probs.f.hip_source(probs.n, probs.np, implicit, fd=implicit and not alg.autodiff)
     h = hashlib.sha1(src.encode()).hexdigest()[:16]
     base = dataclasses.replace(probs.dev, block_size=None, refill=0, chunk=0, waves=0, blocks_per_cu=0,
                                start_order=None)

@@ -173,7 +173,7 @@ def _launch(probs: HIPProbVector, alg, adaptive: bool, *, dt, abstol, reltol, sa
                                  "tail compaction or batched saves")
         with_jac = isinstance(alg, GPUODEImplicitAlgorithm)
         check_mass(probs.f, alg)
-        src, _flops = probs.f.hip_source(probs.n, probs.np, with_jac)
+        src, _flops = probs.f.hip_source(probs.n, probs.np, with_jac, fd=with_jac and not alg.autodiff)
         if callback is not None:
             src = src + emit_callbacks(callback, probs.n, probs.np)
     ctx = _lib.context(dev.device)

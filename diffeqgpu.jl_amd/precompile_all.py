@@ -7,6 +7,7 @@ import multiprocessing as mp
 
 from . import _lib
 from .algorithms import HIPDevice
+from .problems import ODEFunction
 from .systems import SDE_SYSTEMS, SYSTEMS
 
 _METHODS = (_lib.DEG_TSIT5, _lib.DEG_VERN7, _lib.DEG_RB23)
@@ -59,6 +60,14 @@ def variants():
         for m in (_IMPLICIT if f.mass_matrix is not None else _METHODS_MORE):
             src, _ = f.hip_source(n, np_, m in _IMPLICIT)
             for dev, adaptive in more:
+                out.append((name, src, n, np_, m, dev.flags(adaptive)))
+    # autodiff = false: differenced Jacobian and time gradient (no user jac / tgrad)
+    for name in ("vdp", "tdep", "lorenz", "quad", "dae2"):
+        f0, n, np_, _sid = SYSTEMS[name]
+        f = ODEFunction(f0.f, mass_matrix=f0.mass_matrix)
+        src, _ = f.hip_source(n, np_, True, fd=True)
+        for m in _IMPLICIT:
+            for dev, adaptive in more + ([(HIPDevice(), True)] if name == "quad" else []):
                 out.append((name, src, n, np_, m, dev.flags(adaptive)))
     for name, (f, n, np_, nm, _o) in SDE_SYSTEMS.items():
         src, _ = f.hip_source(n, np_, nm)

@@ -62,11 +62,13 @@ class ODEFunction:
     def has_jac(self):
         return self.jac is not None
 
-    def hip_source(self, n: int, np_: int, with_jac: bool) -> tuple[str, int]:
-        """Traced HIP C for deg_rhs (+ deg_jac/deg_tgrad and the mass matrix), and the RHS flop count."""
-        key = (n, np_, with_jac)
+    def hip_source(self, n: int, np_: int, with_jac: bool, fd: bool = False) -> tuple[str, int]:
+        """Traced HIP C for deg_rhs (+ deg_jac/deg_tgrad and the mass matrix), and the RHS flop count.
+        ``fd``: a Jacobian / time gradient not given here comes from forward differences
+        (``autodiff=False``) instead of forward-mode differentiation."""
+        key = (n, np_, with_jac, bool(fd) and with_jac)
         if key not in self._src_cache:
-            src, flops = tracer.emit_rhs(self.f, n, np_, self.jac, self.tgrad, with_jac)
+            src, flops = tracer.emit_rhs(self.f, n, np_, self.jac, self.tgrad, with_jac, fd=bool(fd))
             if self.mass_matrix is not None:
                 src = tracer.emit_mass(self.mass_matrix, n) + src
             self._src_cache[key] = (src, flops)

@@ -594,15 +594,76 @@ def count_flops(tr: Trace) -> int:
     return _emit_body(tr.g, tr.outs, "du")[1]
 
 
+# sqrt(eps(Float32)) = 2^-11.5 rounded to Float32: the finite-difference step of alg_utils.jl:18
+# and nlsolve/type.jl:152
+FD_STEP_BITS = 0x39B504F3
+
+
+def _fd_jac_lines(n: int):
+    """deg_jac by forward differences (finite_diff_jac, alg_utils.jl:17-27): column i is
+    (f(u + dx e_i) - f(u)) / dx, with the perturbed entry rounded to Float32 and a true division."""
+    return [
+        "__device__ __forceinline__ void deg_jac(const float* __restrict__ u, const float* __restrict__ p, float t,",
+        "                                        float* __restrict__ J) {",
+        f"    const float dx = __builtin_bit_cast(float, 0x{FD_STEP_BITS:08x}u);",
+        f"    float f0[{n}], f1[{n}], ux[{n}];",
+        "    deg_rhs(u, p, t, f0);",
+        f"    _Pragma(\"unroll\") for (int i = 0; i < {n}; ++i) {{",
+        f"        _Pragma(\"unroll\") for (int r = 0; r < {n}; ++r) ux[r] = u[r];",
+        "        ux[i] = u[i] + dx;",
+        "        deg_rhs(ux, p, t, f1);",
+        f"        _Pragma(\"unroll\") for (int r = 0; r < {n}; ++r) J[r * {n} + i] = (f1[r] - f0[r]) / dx;",
+        "    }",
+        "}"]
+
+
+def _fd_tgrad_lines(n: int):
+    """deg_tgrad by a forward difference in t (nlsolve/type.jl:150-154): (f(u, p, t + dt) - f(u, p, t)) / dt
+    with dt = sqrt(eps(Float32))."""
+    return [
+        "__device__ __forceinline__ void deg_tgrad(const float* __restrict__ u, const float* __restrict__ p, float t,",
+        "                                          float* __restrict__ dT) {",
+        f"    const float dx = __builtin_bit_cast(float, 0x{FD_STEP_BITS:08x}u);",
+        f"    float f0[{n}], f1[{n}];",
+        "    deg_rhs(u, p, t, f0);",
+        "    deg_rhs(u, p, t + dx, f1);",
+        f"    _Pragma(\"unroll\") for (int r = 0; r < {n}; ++r) dT[r] = (f1[r] - f0[r]) / dx;",
+        "}"]
+
+
 def emit_rhs(f: Callable, n: int, np_: int, jac: Callable | None = None, tgrad: Callable | None = None,
-             with_jac: bool = False) -> Tuple[str, int]:
-    """HIP C source defining deg_rhs (and deg_jac/deg_tgrad when with_jac) plus the RHS flop count."""
+             with_jac: bool = False, fd: bool = False) -> Tuple[str, int]:
+    """HIP C source defining deg_rhs (and deg_jac/deg_tgrad when with_jac) plus the RHS flop count.
+
+    The Jacobian and time gradient come from ``jac``/``tgrad`` when given, else from forward-mode
+    differentiation of the traced DAG, or with ``fd`` (``autodiff=False``) from forward differences
+    of deg_rhs, in the order build_J_W / build_tgrad choose them (nlsolve/type.jl:129-157)."""
     tr = trace(f, n, np_, nout=n)
     body, flops = _emit_body(tr.g, tr.outs, "du")
     src = ["__device__ __forceinline__ void deg_rhs(const float* __restrict__ u, const float* __restrict__ p, float t,",
            "                                        float* __restrict__ du) {",
            "    (void)u; (void)p; (void)t;", *body, "}"]
-    if with_jac:
+    if with_jac and fd:
+        if jac is not None:
+            g, u, p, t = _new_inputs(n, np_)
+            out = jac(SymVec(u, "u"), SymVec(p, "p"), t)
+            rows = _flatten_matrix(out, n)
+            jt = Trace(g, u, p, t, [g.lift(x) for x in rows])
+            jbody, _ = _emit_body(jt.g, jt.outs, "J")
+            src += ["__device__ __forceinline__ void deg_jac(const float* __restrict__ u, const float* __restrict__ p, float t,",
+                    "                                        float* __restrict__ J) {",
+                    "    (void)u; (void)p; (void)t;", *jbody, "}"]
+        else:
+            src += _fd_jac_lines(n)
+        if tgrad is not None:
+            tt = trace(tgrad, n, np_, nout=n)
+            tbody, _ = _emit_body(tt.g, tt.outs, "dT")
+            src += ["__device__ __forceinline__ void deg_tgrad(const float* __restrict__ u, const float* __restrict__ p, float t,",
+                    "                                          float* __restrict__ dT) {",
+                    "    (void)u; (void)p; (void)t;", *tbody, "}"]
+        else:
+            src += _fd_tgrad_lines(n)
+    elif with_jac:
         if jac is not None:
             g, u, p, t = _new_inputs(n, np_)
             out = jac(SymVec(u, "u"), SymVec(p, "p"), t)

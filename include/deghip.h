@@ -152,7 +152,8 @@ int deg_set_cache_dir(const char *dir);
  * rhs_src defines, in HIP C:
  *   __device__ void deg_rhs(const float* u, const float* p, float t, float* du);
  * and for DEG_RB23, DEG_RODAS4, DEG_RODAS5P, DEG_KVAERNO3 and DEG_KVAERNO5 also deg_jac (row-major
- * J[r*n + c] = d du_r / d u_c) and deg_tgrad. For those methods a constant mass matrix M (M u' = f,
+ * J[r*n + c] = d du_r / d u_c) and deg_tgrad (analytic, forward-mode, or forward differences of deg_rhs
+ * for autodiff = false, alg_utils.jl:17-27: the library only calls them). For those methods a constant mass matrix M (M u' = f,
  * ODEFunction(f; mass_matrix = M), nlsolve/type.jl:138 and the mass_matrix branches of the
  * Rosenbrock / Rodas steps) is given as
  *   #define DEG_MASS 1   #define DEG_MASS_ENTRIES <n*n float literals, row-major>

@@ -63,7 +63,48 @@
 
 enum { OM_TSIT5 = 0, OM_VERN7 = 1, OM_RB23 = 2, OM_VERN9 = 3, OM_RODAS4 = 4, OM_RODAS5P = 5, OM_KVAERNO3 = 6,
        OM_KVAERNO5 = 7 };
-enum { DEG_O_KAHAN = 1, DEG_O_DETPOW = 2 };
+enum { DEG_O_KAHAN = 1, DEG_O_DETPOW = 2, DEG_O_FDJAC = 4 };
+
+/* autodiff = false: J by forward differences, column i = (f(u + dx e_i) - f(u)) / dx with the
+ * perturbed entry rounded to Float32 (finite_diff_jac, alg_utils.jl:17-27), and the time gradient
+ * (f(u, p, t + dx) - f(u, p, t)) / dx (nlsolve/type.jl:150-154); dx = sqrt(eps(Float32)) = 2^-11.5
+ * rounded to Float32. */
+static float fd_step(void) {
+    const uint32_t b = 0x39B504F3u;
+    float x;
+    memcpy(&x, &b, sizeof x);
+    return x;
+}
+
+static void sys_jac(const osys *S, const float *u, const float *p, float t, float *J) {
+    if (!S->fd) {
+        S->jac(u, p, t, J);
+        return;
+    }
+    const int n = S->n;
+    const float dx = fd_step();
+    float f0[OMAXN], f1[OMAXN], ux[OMAXN];
+    S->f(u, p, t, f0);
+    for (int i = 0; i < n; ++i) {
+        for (int r = 0; r < n; ++r) ux[r] = u[r];
+        ux[i] = u[i] + dx;
+        S->f(ux, p, t, f1);
+        for (int r = 0; r < n; ++r) J[r * n + i] = (f1[r] - f0[r]) / dx;
+    }
+}
+
+static void sys_tgrad(const osys *S, const float *u, const float *p, float t, float *dT) {
+    if (!S->fd) {
+        S->tgrad(u, p, t, dT);
+        return;
+    }
+    const int n = S->n;
+    const float dx = fd_step();
+    float f0[OMAXN], f1[OMAXN];
+    S->f(u, p, t, f0);
+    S->f(u, p, t + dx, f1);
+    for (int r = 0; r < n; ++r) dT[r] = (f1[r] - f0[r]) / dx;
+}
 /* per-trajectory return codes (same values as the C ABI of the HIP library) */
 enum { ORET_SUCCESS = 0, ORET_DTMIN = 1, ORET_MAXITERS = 2, ORET_NONFINITE = 3, ORET_TERMINATED = 4,
        ORET_FAILURE = 5 };
@@ -500,8 +541,8 @@ static void rodas4_step(const osys *S, const float *p, const float *uprev, float
     const int n = S->n;
     float Jm[OMAXN * OMAXN], dT[OMAXN], du[OMAXN], lt[OMAXN] = {0}, u[OMAXN], cm[OMAXN] = {0}, mc[OMAXN] = {0};
     float K1[OMAXN], K2[OMAXN], K3[OMAXN], K4[OMAXN], K5[OMAXN], K6[OMAXN];
-    S->jac(uprev, p, t, Jm);
-    S->tgrad(uprev, p, t, dT);
+    sys_jac(S, uprev, p, t, Jm);
+    sys_tgrad(S, uprev, p, t, dT);
     const float dtC21 = R4(C21) / dt, dtC31 = R4(C31) / dt, dtC32 = R4(C32) / dt;
     const float dtC41 = R4(C41) / dt, dtC42 = R4(C42) / dt, dtC43 = R4(C43) / dt;
     const float dtC51 = R4(C51) / dt, dtC52 = R4(C52) / dt, dtC53 = R4(C53) / dt, dtC54 = R4(C54) / dt;
@@ -558,8 +599,8 @@ static void rodas5p_step(const osys *S, const float *p, const float *uprev, floa
     const int n = S->n;
     float Jm[OMAXN * OMAXN], dT[OMAXN], du[OMAXN], lt[OMAXN] = {0}, u[OMAXN], cm[OMAXN] = {0}, mc[OMAXN] = {0};
     float K1[OMAXN], K2[OMAXN], K3[OMAXN], K4[OMAXN], K5[OMAXN], K6[OMAXN], K7[OMAXN], K8[OMAXN];
-    S->jac(uprev, p, t, Jm);
-    S->tgrad(uprev, p, t, dT);
+    sys_jac(S, uprev, p, t, Jm);
+    sys_tgrad(S, uprev, p, t, dT);
     const float dtC21 = R5(C21) / dt, dtC31 = R5(C31) / dt, dtC32 = R5(C32) / dt;
     const float dtC41 = R5(C41) / dt, dtC42 = R5(C42) / dt, dtC43 = R5(C43) / dt;
     const float dtC51 = R5(C51) / dt, dtC52 = R5(C52) / dt, dtC53 = R5(C53) / dt, dtC54 = R5(C54) / dt;
@@ -649,7 +690,7 @@ static void rodas_interp(int method, float th, int n, const float *y0, const flo
 static void kv_w(olu *F, const osys *S, const float *p, float gdt, const float *x, float tc) {
     const int n = S->n;
     float Jm[OMAXN * OMAXN];
-    S->jac(x, p, tc, Jm);
+    sys_jac(S, x, p, tc, Jm);
     for (int r = 0; r < n; ++r)
         for (int c = 0; c < n; ++c)
             F->lu[r][c] = (S->M ? -(double)S->M[r * n + c] : (r == c ? -1.0 : 0.0)) + (double)gdt * (double)Jm[r * n + c];
@@ -1159,8 +1200,8 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                 float Jm[OMAXN * OMAXN], dT[OMAXN], F0[OMAXN], rhs[OMAXN], tmp[OMAXN], F1[OMAXN], x[OMAXN];
                 float gam = dts * d;
                 float dto2 = dts / 2.0f;
-                S->jac(s->uprev, s->p, told, Jm);
-                S->tgrad(s->uprev, s->p, told, dT);
+                sys_jac(S, s->uprev, s->p, told, Jm);
+                sys_tgrad(S, s->uprev, s->p, told, dT);
                 olu F;
                 w_factor(&F, n, gam, Jm, S->M);
                 S->f(s->uprev, s->p, told, F0);
@@ -1254,8 +1295,8 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                     float gam = dt * d;
                     float dto2 = dt / 2.0f;
                     float dto6 = dt / 6.0f;
-                    S->jac(s->uprev, s->p, t, Jm);
-                    S->tgrad(s->uprev, s->p, t, dT);
+                    sys_jac(S, s->uprev, s->p, t, Jm);
+                    sys_tgrad(S, s->uprev, s->p, t, dT);
                     olu F;
                     w_factor(&F, n, gam, Jm, S->M);
                     S->f(s->uprev, s->p, t, F0);
@@ -1380,7 +1421,9 @@ int deg_oracle_solve_cb(int system, int method, int adaptive, int64_t N,
     if (method < OM_TSIT5 || method > OM_KVAERNO5) return -1;
     if (ntstops < 0 || ncb < 0 || (ntstops > 0 && !tstops) || (ncb > 0 && !cbs)) return -1;
     init_tabs();
-    const osys *S = &OSYSTEMS[system];
+    osys Sys = OSYSTEMS[system];
+    Sys.fd = (flags & DEG_O_FDJAC) != 0;
+    const osys *S = &Sys;
     ojob J;
     J.S = S;
     J.method = method;
@@ -1419,6 +1462,17 @@ int deg_oracle_solve_cb(int system, int method, int adaptive, int64_t N,
         if (nreject) nreject[i] = nr;
     }
     return bad ? -2 : 0;
+}
+
+/* The Jacobian and time gradient an implicit step would use at (u, p, t): the system's analytic
+ * ones (fd = 0) or the forward differences of autodiff = false (fd = 1). */
+int deg_oracle_jac(int system, int fd, const float *u, const float *p, float t, float *J, float *dT) {
+    if (system < 0 || system >= OSYS_COUNT) return -1;
+    osys Sys = OSYSTEMS[system];
+    Sys.fd = fd != 0;
+    sys_jac(&Sys, u, p, t, J);
+    sys_tgrad(&Sys, u, p, t, dT);
+    return 0;
 }
 
 int deg_oracle_solve(int system, int method, int adaptive, int64_t N,

@@ -21,7 +21,7 @@ SYSTEM_DIMS = {"lorenz": (3, 3), "decay": (1, 1), "growth": (1, 1), "quad": (1, 
                "robertson_eye": (3, 3), "dae2": (2, 0)}
 METHOD_IDS = {"tsit5": 0, "vern7": 1, "rb23": 2, "vern9": 3, "rodas4": 4, "rodas5p": 5, "kvaerno3": 6,
               "kvaerno5": 7}
-KAHAN, DETPOW = 1, 2
+KAHAN, DETPOW, FDJAC = 1, 2, 4
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -97,6 +97,8 @@ def lib():
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
                                        ctypes.c_int, _dp, _ip, _lp, ctypes.c_int]
         L.deg_oracle_vern9.restype = ctypes.c_int
+        L.deg_oracle_jac.argtypes = [ctypes.c_int, ctypes.c_int, _fp, _fp, ctypes.c_float, _fp, _fp]
+        L.deg_oracle_jac.restype = ctypes.c_int
         L.deg_oracle_det_pow.argtypes = [ctypes.c_float, ctypes.c_float]
         L.deg_oracle_det_pow.restype = ctypes.c_float
         L.deg_oracle_tableau.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
@@ -133,13 +135,29 @@ def _f32(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
 
 
+def jac(system: str, u, p, t, fd=False):
+    """(J, dT) at one state: the system's analytic Jacobian and time gradient, or with fd the
+    forward differences an ``autodiff=False`` implicit method uses."""
+    n, np_ = SYSTEM_DIMS[system]
+    u = np.ascontiguousarray(u, np.float32).reshape(n)
+    p = np.ascontiguousarray(p if np_ else [0.0], np.float32).reshape(-1)
+    J = np.zeros(n * n, np.float32)
+    dT = np.zeros(n, np.float32)
+    r = lib().deg_oracle_jac(SYSTEM_IDS[system], int(bool(fd)), _p(u, _fp), _p(p, _fp), ctypes.c_float(t),
+                             _p(J, _fp), _p(dT, _fp))
+    if r != 0:
+        raise ValueError(f"deg_oracle_jac failed ({r})")
+    return J.reshape(n, n), dT
+
+
 def solve(system: str, method: str, adaptive: bool, u0, p, tspan, dt, *, abstol=1e-6, reltol=1e-3, saveat=None,
           nsave=None, save_everystep=False, t0s=None, tfs=None, kahan=False, detpow=False, max_attempts=0,
-          nthreads=None, tstops=None, callbacks=()):
+          nthreads=None, tstops=None, callbacks=(), fdjac=False):
     """Float32 restatement over an ensemble. u0 (n, N), p (np, N) float32 SoA. Returns dict with
     ts (nsave, N), us (nsave, n, N), retcode, naccept, nreject. Rows the reference leaves
     uninitialised are NaN here. tstops: stopping times; callbacks: descriptors from discrete() /
-    continuous()."""
+    continuous(). fdjac: the implicit methods take J and the time gradient by forward differences
+    of f (autodiff = false) instead of the system's analytic ones."""
     n, np_ = SYSTEM_DIMS[system]
     u0 = _f32(u0)
     p = _f32(p) if np_ > 0 else np.zeros((0, u0.shape[1]), np.float32)
@@ -152,7 +170,7 @@ def solve(system: str, method: str, adaptive: bool, u0, p, tspan, dt, *, abstol=
     rc = np.zeros(N, np.int32)
     na = np.zeros(N, np.int32)
     nr = np.zeros(N, np.int32)
-    flags = (KAHAN if kahan else 0) | (DETPOW if detpow else 0)
+    flags = (KAHAN if kahan else 0) | (DETPOW if detpow else 0) | (FDJAC if fdjac else 0)
     t0s, tfs = _f32(t0s), _f32(tfs)
     tst = None if tstops is None else np.ascontiguousarray(np.asarray(tstops, np.float64).astype(np.float32).reshape(-1))
     cbs = list(callbacks or ())

@@ -36,6 +36,7 @@ typedef struct {
     ojac_fn jac;
     orhs_fn tgrad;
     const float *M;
+    int fd; /* 1: jac and tgrad by forward differences of f (autodiff = false) */
 } osys;
 
 static void lorenz_f(const float *u, const float *p, float t, float *du) {
@@ -185,18 +186,18 @@ static const float M_DIAG110[9] = {1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0
 static const float M_EYE3[9] = {1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
 
 static const osys OSYSTEMS[OSYS_COUNT] = {
-    {3, 3, lorenz_f, lorenz_jac, zero_tgrad3, NULL},
-    {1, 1, decay_f, decay_jac, zero_tgrad1, NULL},
-    {1, 1, growth_f, growth_jac, zero_tgrad1, NULL},
-    {1, 1, quad_f, quad_jac, zero_tgrad1, NULL},
-    {3, 3, robertson_f, robertson_jac, zero_tgrad3, NULL},
-    {2, 1, vdp_f, vdp_jac, zero_tgrad2, NULL},
-    {1, 1, tdep_f, tdep_jac, tdep_tgrad, NULL},
-    {2, 1, rot_f, rot_jac, zero_tgrad2, NULL},
-    {2, 1, ball_f, ball_jac, zero_tgrad2, NULL},
-    {3, 3, rober_dae_f, rober_dae_jac, zero_tgrad3, M_DIAG110},
-    {3, 3, robertson_f, robertson_jac, zero_tgrad3, M_EYE3},
-    {2, 0, dae2_f, dae2_jac, zero_tgrad2, M_DIAG10},
+    {3, 3, lorenz_f, lorenz_jac, zero_tgrad3, NULL, 0},
+    {1, 1, decay_f, decay_jac, zero_tgrad1, NULL, 0},
+    {1, 1, growth_f, growth_jac, zero_tgrad1, NULL, 0},
+    {1, 1, quad_f, quad_jac, zero_tgrad1, NULL, 0},
+    {3, 3, robertson_f, robertson_jac, zero_tgrad3, NULL, 0},
+    {2, 1, vdp_f, vdp_jac, zero_tgrad2, NULL, 0},
+    {1, 1, tdep_f, tdep_jac, tdep_tgrad, NULL, 0},
+    {2, 1, rot_f, rot_jac, zero_tgrad2, NULL, 0},
+    {2, 1, ball_f, ball_jac, zero_tgrad2, NULL, 0},
+    {3, 3, rober_dae_f, rober_dae_jac, zero_tgrad3, M_DIAG110, 0},
+    {3, 3, robertson_f, robertson_jac, zero_tgrad3, M_EYE3, 0},
+    {2, 0, dae2_f, dae2_jac, zero_tgrad2, M_DIAG10, 0},
 };
 
 /* The same right-hand sides in double precision, for the FP64 oracle. */

@@ -211,8 +211,8 @@ def test_api_surface(deg):
     assert [deg.alg_order(a) for a in (deg.GPUTsit5(), deg.GPUVern7(), deg.GPURosenbrock23())] == [5, 7, 2]
     with pytest.raises(NotImplementedError):
         deg.EnsembleGPUKernel(deg.HIPDevice(), 0.5)
-    with pytest.raises(NotImplementedError):
-        deg.GPURosenbrock23(autodiff=False)
+    assert deg.GPURosenbrock23(autodiff=False).autodiff is False and deg.GPURodas5P().autodiff is True
+    assert deg.GPUKvaerno3(autodiff=False) != deg.GPUKvaerno3()
     r = deg.remake(prob, p=[2.0])
     assert r.p[0] == 2.0 and r.u0[0] == 1.0
     # device flags
