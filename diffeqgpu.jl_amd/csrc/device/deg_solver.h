@@ -1224,7 +1224,9 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                         const bool mine = (pending >> lane) & 1ull;
                         const long long rank = (long long)__popcll(pending & ((1ull << lane) - 1ull));
                         if (mine && rank < avail) {
-                            gid = c_next + rank;
+                            // with a start order, position j of queue qi holds order entry j * DEG_NQUEUE + qi
+                            // (< n_traj by the queue's limit): the lane keeps that entry's index
+                            gid = a.order ? (c_next + rank - (long long)qi * slab) * DEG_NQUEUE + qi : c_next + rank;
                             want = false;
                             need_init = true;
                         }
@@ -1352,12 +1354,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
         long long oi = -1;
         if (need_init) {
             need_init = false;
-#if DEG_PERSISTENT
-            // position j of queue q holds start-order entry j * DEG_NQUEUE + q (< n_traj by the queue's limit)
-            oi = a.order ? (long long)a.order[(gid % slab) * DEG_NQUEUE + gid / slab] : gid;
-#else
             oi = a.order ? (long long)a.order[gid] : gid;
-#endif
             if (oi < 0 || oi >= a.n_traj) {
                 // an entry of a caller's start order outside [0, n_traj): nothing to solve
                 oi = -1;
