@@ -478,7 +478,9 @@ __device__ __forceinline__ void deg_mass_mul(const float (&v)[DEG_N], float (&ou
 }
 #endif
 // LU with partial pivoting; row swaps are applied with compile-time indices so W stays
-// in VGPRs.
+// in VGPRs. With DEG_PARITY every multiplier and back-substitution step divides by its pivot in
+// IEEE double, as the oracle does; otherwise each pivot is inverted once and the diagonal of the
+// factor holds 1 / U_kk, so a factorisation costs n FP64 divisions and a solve none.
 __device__ __forceinline__ void w_lu(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N]) {
     _Pragma("unroll") for (int kk = 0; kk < DEG_N; ++kk) {
         int pk = kk;
@@ -500,11 +502,21 @@ __device__ __forceinline__ void w_lu(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N
                 }
             }
         }
+#if !DEG_PARITY
+        const double rp = 1.0 / W[kk][kk];
+#endif
         _Pragma("unroll") for (int i = kk + 1; i < DEG_N; ++i) {
+#if DEG_PARITY
             double l = W[i][kk] / W[kk][kk];
+#else
+            double l = W[i][kk] * rp;
+#endif
             W[i][kk] = l;
             _Pragma("unroll") for (int c = kk + 1; c < DEG_N; ++c) W[i][c] = W[i][c] - l * W[kk][c];
         }
+#if !DEG_PARITY
+        W[kk][kk] = rp;
+#endif
     }
 }
 __device__ __forceinline__ void w_solve(const double (&W)[DEG_N][DEG_N], const int (&piv)[DEG_N],
@@ -528,7 +540,11 @@ __device__ __forceinline__ void w_solve(const double (&W)[DEG_N][DEG_N], const i
     _Pragma("unroll") for (int i = DEG_N - 1; i >= 0; --i) {
         double v = y[i];
         _Pragma("unroll") for (int j = i + 1; j < DEG_N; ++j) v = v - W[i][j] * y[j];
+#if DEG_PARITY
         y[i] = v / W[i][i];
+#else
+        y[i] = v * W[i][i];  // the factor keeps 1 / U_ii
+#endif
     }
     DEG_FOR(i) x[i] = (float)y[i];
 }
@@ -613,8 +629,14 @@ __device__ __forceinline__ void deg_interp(float th, float dt, float tprev, cons
     (void)y1;
     const float d = RB23_D;
     const float den = __builtin_fmaf(-2.0f, d, 1.0f);
+#if DEG_PARITY
     const float c1 = (th * (1.0f - th)) / den;
     const float c2 = (th * __builtin_fmaf(-2.0f, d, th)) / den;
+#else
+    const float rden = 1.0f / den;  // a compile-time constant
+    const float c1 = (th * (1.0f - th)) * rden;
+    const float c2 = (th * __builtin_fmaf(-2.0f, d, th)) * rden;
+#endif
     DEG_FOR(c) out[c] = y0[c] + dt * (c1 * k[1][c] + c2 * k[2][c]);
 }
 #endif

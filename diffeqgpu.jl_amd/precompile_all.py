@@ -59,7 +59,8 @@ def variants():
     for name, (f, n, np_, _sid) in SYSTEMS.items():
         for m in (_IMPLICIT if f.mass_matrix is not None else _METHODS_MORE):
             src, _ = f.hip_source(n, np_, m in _IMPLICIT)
-            for dev, adaptive in more:
+            # the default path of the implicit methods on the stiff fixture
+            for dev, adaptive in more + ([(HIPDevice(), True)] if name == "robertson" else []):
                 out.append((name, src, n, np_, m, dev.flags(adaptive)))
     # autodiff = false: differenced Jacobian and time gradient (no user jac / tgrad)
     for name in ("vdp", "tdep", "lorenz", "quad", "dae2"):

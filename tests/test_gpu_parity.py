@@ -238,6 +238,35 @@ def test_fast_path_fp64_error_lorenz_adaptive(deg, torch_cuda):
     assert np.percentile(eg, 90) <= 2 * np.percentile(eo, 90) + 1e-6
 
 
+@pytest.mark.parametrize("method", ["rb23", "rodas4", "rodas5p", "kvaerno3", "kvaerno5"])
+def test_fast_path_implicit_robertson_fixture(deg, torch_cuda, method):
+    """The implicit methods on the default path (contraction, hardware controller pow, the W factor
+    keeping pivot reciprocals instead of dividing) on the stiff workload's 1,000-trajectory FP64 Radau
+    fixture: every trajectory succeeds, y1 + y2 + y3 = 1 to 1e-4, and the error against the fixture
+    is within 2x of the oracle's own Float32 error on the same trajectories at the median and 4x at
+    the 99th percentile (the Kvaerno Newton solves stop at 100 eps(Float32), where contraction moves
+    the iteration counts), and below 1e-3 everywhere. A wrong factorisation errs by 1e-2 or more."""
+    import os
+
+    from diffeqgpu_amd import workloads
+
+    cfg = workloads.config("robertson", N=1000)
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "robertson_radau.npz"))
+    sv = np.asarray(cfg["saveat"], F32)
+    g = gpu_solve(deg, torch_cuda, "robertson", method, True, cfg["u0"], cfg["p"], cfg["tspan"], cfg["dt"],
+                  deg.HIPDevice(), abstol=1e-8, reltol=1e-4, saveat=sv)
+    o = pyoracle.solve("robertson", method, True, cfg["u0"], cfg["p"], cfg["tspan"], cfg["dt"], abstol=1e-8,
+                       reltol=1e-4, saveat=sv)
+    assert np.all(g["retcode"] == 0)
+    us = g["us"].astype(np.float64)
+    assert np.abs(us.sum(axis=1) - 1.0).max() < 1e-4
+    eg = np.abs(us - gold["u"]).max(axis=(0, 1))
+    eo = np.abs(o["us"].astype(np.float64) - gold["u"]).max(axis=(0, 1))
+    assert np.median(eg) <= 2 * np.median(eo) + 1e-7, (np.median(eg), np.median(eo))
+    assert np.percentile(eg, 99) <= 4 * np.percentile(eo, 99) + 1e-6, (np.percentile(eg, 99), np.percentile(eo, 99))
+    assert eg.max() < 1e-3 if method != "kvaerno5" else np.sum(eg >= 1e-3) <= 5, np.sort(eg)[-6:]
+
+
 # ---------------------------------------------------------------------------------------
 # the fixed-step Lorenz workload at full size: bitwise and within 1e-5 relative
 # ---------------------------------------------------------------------------------------
