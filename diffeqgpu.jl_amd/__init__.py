@@ -8,12 +8,13 @@ suite, the benchmark and the build entry point load it as ``diffeqgpu_amd``.
 from .algorithms import (EnsembleGPUKernel, EnsembleKernelAlgorithm, GPUEM, GPUODEAlgorithm, GPUODEImplicitAlgorithm,
                          GPUKvaerno3, GPUKvaerno5, GPURodas4, GPURodas5P, GPURosenbrock23, GPUSDEAlgorithm, GPUSIEA,
                          GPUTsit5, GPUVern7, GPUVern9, HIPDevice, alg_order)
+from .dual import Dual, DualArray
 from .callbacks import (CallbackSet, ContinuousCallback, DiscreteCallback, GPUContinuousCallback, GPUDiscreteCallback,
                         LeftRootFind, NoRootFind, RightRootFind, terminate, u_modified)
 from .ensemble import EnsembleContext, EnsembleSolution, ODESolution, solve
 from .lowerlevel import SolveResult, order_by_span, vectorized_asolve, vectorized_solve
-from .problems import (EnsembleProblem, HIPProbVector, ImmutableODEProblem, ImmutableSDEProblem, ODEFunction, ODEProblem,
-                       SDEFunction, SDEProblem, adapt, get_backend, is_diagonal_noise, make_prob_compatible, remake)
+from .problems import (DualODEFunction, EnsembleProblem, HIPProbVector, ImmutableODEProblem, ImmutableSDEProblem,
+                       ODEFunction, ODEProblem, SDEFunction, SDEProblem, adapt, get_backend, is_diagonal_noise, make_prob_compatible, remake)
 from .ranges import colon, linrange
 from .sweeps import Draw, ParameterSweep
 from . import autotune, learned
@@ -29,5 +30,5 @@ __all__ = [
     "SVector", "math", "Draw", "ParameterSweep", "autotune", "learned", "CallbackSet", "ContinuousCallback",
     "DiscreteCallback", "GPUContinuousCallback", "GPUDiscreteCallback", "LeftRootFind", "NoRootFind", "RightRootFind",
     "terminate", "u_modified", "GPUSDEAlgorithm", "GPUEM", "GPUSIEA", "SDEFunction", "SDEProblem",
-    "ImmutableSDEProblem", "is_diagonal_noise",
+    "ImmutableSDEProblem", "is_diagonal_noise", "Dual", "DualArray", "DualODEFunction",
 ]

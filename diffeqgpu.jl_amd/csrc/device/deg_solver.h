@@ -128,20 +128,72 @@ __device__ __forceinline__ float rms_norm(const float (&x)[DEG_N]) {
     return __builtin_sqrtf(acc / (float)DEG_N);
 }
 
+#ifndef DEG_DUAL_P
+#define DEG_DUAL_P 0
+#endif
+
+#if DEG_DUAL_P
+// Dual-number state (DEG_DUAL_NV components, each with DEG_DUAL_P partials; the values are
+// entries 0..NV-1 and component i's partials are NV + i P .. NV + i P + P - 1). The error
+// scaling e ./ (abstol .+ max.(abs.(uprev), abs.(u)) * reltol) is formed in dual arithmetic:
+// abs negates the partials of a negative value, max picks by value (u only when its value is
+// larger, as jl_max), x * reltol scales the partials, e / s takes partials e' (1 / s) + s' (-(e / s^2)).
+// The squared norm sums value^2 + (partial_1^2 + ... + partial_P^2) over the components, over
+// NV (1 + P) entries in all (ODE_DEFAULT_NORM of a dual array). rcp: reciprocal multiply instead
+// of the IEEE division of the value.
+static_assert(DEG_N == DEG_DUAL_NV * (1 + DEG_DUAL_P), "a dual state holds NV values and NV * P partials");
+template <bool rcp>
+__device__ __forceinline__ float dual_scaled_error2(const float (&e)[DEG_N], const float (&uprev)[DEG_N],
+                                                    const float (&u)[DEG_N], float abstol, float reltol) {
+    float acc = 0.0f;
+    _Pragma("unroll") for (int i = 0; i < DEG_DUAL_NV; ++i) {
+        const float av = uprev[i], bv = u[i];
+        const float aa = __builtin_fabsf(av), ba = __builtin_fabsf(bv);
+        const bool nan = (aa != aa) || (ba != ba);
+        const bool pick_u = !nan && (ba > aa);
+        const float mv = nan ? aa + ba : (pick_u ? ba : aa);
+        const bool neg = pick_u ? (bv < 0.0f) : (av < 0.0f);
+        const float sv = abstol + mv * reltol;
+        const float ev = e[i];
+        const float qv = rcp ? ev * __builtin_amdgcn_rcpf(sv) : ev / sv;
+        const float inv = rcp ? __builtin_amdgcn_rcpf(sv) : 1.0f / sv;
+        const float c = -(rcp ? ev * __builtin_amdgcn_rcpf(sv * sv) : ev / (sv * sv));
+        float ps = 0.0f;
+        _Pragma("unroll") for (int j = 0; j < DEG_DUAL_P; ++j) {
+            const int q = DEG_DUAL_NV + i * DEG_DUAL_P + j;
+            const float md = pick_u ? u[q] : uprev[q];
+            const float sd = (neg ? -md : md) * reltol;
+            const float qd = e[q] * inv + sd * c;
+            ps = (j == 0) ? qd * qd : ps + qd * qd;
+        }
+        const float term = qv * qv + ps;
+        acc = (i == 0) ? term : acc + term;
+    }
+    return acc;
+}
+#endif
+
 // error scaling: tmp ./ (abstol .+ max.(abs.(uprev), abs.(u)) * reltol)
 __device__ __forceinline__ float scaled_error(float (&e)[DEG_N], const float (&uprev)[DEG_N],
                                               const float (&u)[DEG_N], float abstol, float reltol) {
+#if DEG_DUAL_P
+    return __builtin_sqrtf(dual_scaled_error2<false>(e, uprev, u, abstol, reltol) / (float)DEG_N);
+#else
     DEG_FOR(c) {
         float sc = abstol + jl_max(__builtin_fabsf(uprev[c]), __builtin_fabsf(u[c])) * reltol;
         e[c] = e[c] / sc;
     }
     return rms_norm(e);
+#endif
 }
 
 // Fast error norm (DEG_FASTPOW): the squared norm E2 = EEst^2 with reciprocal-multiply
 // scaling. The controller then works on log2(E2) (see deg_fast_control).
 __device__ __forceinline__ float scaled_error2_fast(float (&e)[DEG_N], const float (&uprev)[DEG_N],
                                                     const float (&u)[DEG_N], float abstol, float reltol) {
+#if DEG_DUAL_P
+    return dual_scaled_error2<true>(e, uprev, u, abstol, reltol) * (1.0f / (float)DEG_N);
+#endif
     float acc = 0.0f;
     DEG_FOR(c) {
         const float sc = abstol + __builtin_fmaxf(__builtin_fabsf(uprev[c]), __builtin_fabsf(u[c])) * reltol;

@@ -16,8 +16,9 @@ from typing import Any, List, Optional
 import numpy as np
 
 from . import _lib
-from .algorithms import EnsembleGPUKernel, GPUODEAlgorithm, GPUSDEAlgorithm
+from .algorithms import EnsembleGPUKernel, GPUODEAlgorithm, GPUODEImplicitAlgorithm, GPUSDEAlgorithm
 from .callbacks import merged_callback
+from .dual import split_rows
 from .lowerlevel import check_mass, vectorized_asolve, vectorized_solve
 from .problems import EnsembleProblem, adapt, make_prob_compatible
 
@@ -34,7 +35,7 @@ class EnsembleContext:
 @dataclass
 class ODESolution:
     t: np.ndarray
-    u: np.ndarray          # (len(t), n)
+    u: np.ndarray          # (len(t), n); (len(t), n, 1 + P) for dual inputs (values, then partials)
     retcode: str
     prob: Any = None
     alg: Any = None
@@ -141,7 +142,13 @@ def batch_solve(ensembleprob, alg, ensemblealg, I, adaptive, seed=None, **kwargs
         retcode = "Terminated" if sol_idx != len(ts) - 1 else "Success"
         if rc[k] != _lib.RET_SUCCESS:
             retcode = _lib.RETCODE_NAMES.get(int(rc[k]), "Failure")
-        sol = ODESolution(ts[: sol_idx + 1].copy(), us[: sol_idx + 1].copy(), retcode, probs[k], alg)
+        rows = us[: sol_idx + 1].copy()
+        dual_dims = getattr(probs[k], "dual", None)  # SDE problems carry none
+        if dual_dims is not None:
+            # dual inputs: each saved state as (n, 1 + P), values in column 0, partials after
+            n, _np, P = dual_dims
+            rows = split_rows(rows, n, P)
+        sol = ODESolution(ts[: sol_idx + 1].copy(), rows, retcode, probs[k], alg)
         out.append(ensembleprob.output_func(sol, _ctx(i, seed))[0])
     return out
 
@@ -156,6 +163,11 @@ def batch_solve_up_kernel(ensembleprob, probs, alg, ensemblealg, I, adaptive, **
         kwargs.pop("callback")
     if isinstance(alg, GPUODEAlgorithm):
         check_mass(probs[0].f, alg)
+        if probs[0].dual is not None and isinstance(alg, GPUODEImplicitAlgorithm):
+            raise NotImplementedError(f"{type(alg).__name__} with dual-number inputs: the implicit methods would need a "
+                                      "W factor with dual entries; use GPUTsit5, GPUVern7 or GPUVern9")
+        if probs[0].dual is not None and kwargs.get("callback") is not None:
+            raise NotImplementedError("callbacks with dual-number inputs are not supported")
     if adaptive and isinstance(alg, GPUSDEAlgorithm):
         # the adaptive SDE entry point raises (lowerlevel_solve.jl:349-356); say so before any device work
         raise NotImplementedError("Adaptive time-stepping is not supported yet with GPUEM.")

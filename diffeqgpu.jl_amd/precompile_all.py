@@ -8,7 +8,7 @@ import multiprocessing as mp
 from . import _lib
 from .algorithms import HIPDevice
 from .problems import ODEFunction
-from .systems import SDE_SYSTEMS, SYSTEMS
+from .systems import DUAL_SYSTEMS, SDE_SYSTEMS, SYSTEMS
 
 _METHODS = (_lib.DEG_TSIT5, _lib.DEG_VERN7, _lib.DEG_RB23)
 _METHODS_MORE = (_lib.DEG_VERN9, _lib.DEG_RODAS4, _lib.DEG_RODAS5P, _lib.DEG_KVAERNO3, _lib.DEG_KVAERNO5)
@@ -30,8 +30,8 @@ def variants():
     ]
     out = []
     for name, (f, n, np_, _sid) in SYSTEMS.items():
-        if f.mass_matrix is not None:
-            continue  # the mass-matrix systems are warmed below with the implicit methods
+        if f.mass_matrix is not None or name in DUAL_SYSTEMS:
+            continue  # the mass-matrix and dual systems are warmed below with their methods
         for m in _METHODS:
             src, _ = f.hip_source(n, np_, m == _lib.DEG_RB23)
             for dev, adaptive in devs:
@@ -57,6 +57,13 @@ def variants():
             (HIPDevice(contract="off", powmode="det", persistent=False), True),
             (HIPDevice(contract="off", powmode="det", persistent=True), True)]
     for name, (f, n, np_, _sid) in SYSTEMS.items():
+        if name in DUAL_SYSTEMS:
+            # dual-number states: the explicit methods, parity and default variants
+            src, _ = f.hip_source(n, np_, False)
+            for m in (_lib.DEG_TSIT5, _lib.DEG_VERN7, _lib.DEG_VERN9):
+                for dev, adaptive in more + [(HIPDevice(), True)]:
+                    out.append((name, src, n, np_, m, dev.flags(adaptive)))
+            continue
         for m in (_IMPLICIT if f.mass_matrix is not None else _METHODS_MORE):
             src, _ = f.hip_source(n, np_, m in _IMPLICIT)
             # the default path of the implicit methods on the stiff fixture

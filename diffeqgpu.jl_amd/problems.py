@@ -16,7 +16,7 @@ from typing import Any, Callable, Optional, Sequence
 
 import numpy as np
 
-from . import tracer
+from . import dual, tracer
 from .algorithms import HIPDevice
 
 
@@ -114,15 +114,52 @@ def _as_tspan(tspan) -> tuple:
     return (np.float32(t0), np.float32(tf))
 
 
+class DualODEFunction(ODEFunction):
+    """The augmented right-hand side of a problem whose u0 / p hold dual numbers: state and
+    parameters are values followed by partials (:mod:`diffeqgpu_amd.dual`), and deg_rhs is the
+    traced dual evaluation of ``f``. Only the explicit methods solve it (the implicit methods
+    would need a W factor with dual entries)."""
+
+    def __init__(self, f: ODEFunction, n: int, np_: int, P: int):
+        if f.mass_matrix is not None:
+            raise NotImplementedError("dual numbers with a mass matrix are not supported")
+        super().__init__(f.f)
+        self.base = f
+        self.n, self.np, self.P = int(n), int(np_), int(P)
+
+    def hip_source(self, n: int, np_: int, with_jac: bool, fd: bool = False) -> tuple[str, int]:
+        if with_jac:
+            raise NotImplementedError("dual-number problems are solved by the explicit methods (GPUTsit5, GPUVern7, "
+                                      "GPUVern9); the implicit methods would need a W factor with dual entries")
+        if (n, np_) != (self.n * (1 + self.P), self.np * (1 + self.P)):
+            raise ValueError("augmented sizes do not match the dual problem")
+        key = ("dual", n, np_)
+        if key not in self._src_cache:
+            self._src_cache[key] = tracer.emit_dual_rhs(self.f, self.n, self.np, self.P)
+        return self._src_cache[key]
+
+
 class ODEProblem:
-    """``ODEProblem{false}(f, u0, tspan, p; kwargs...)`` (out-of-place only)."""
+    """``ODEProblem{false}(f, u0, tspan, p; kwargs...)`` (out-of-place only). ``u0`` and ``p`` may
+    hold :class:`~diffeqgpu_amd.dual.Dual` numbers (forward-mode sensitivities, as ForwardDiff
+    duals do in the reference); they are then kept as :class:`~diffeqgpu_amd.dual.DualArray`."""
 
     def __init__(self, f, u0, tspan, p=None, **kwargs):
         self.f = f if isinstance(f, ODEFunction) else ODEFunction(f)
-        self.u0 = _as_f32_vec(u0)
+        du, dp = dual.as_dual_array(u0), dual.as_dual_array(p)
+        if du is not None or dp is not None:
+            P = (du if du is not None else dp).npartials
+            self.u0 = dual.with_partials(du if du is not None else _as_f32_vec(u0), P)
+            self.p = dual.with_partials(dp if dp is not None else _as_f32_vec(p), P)
+        else:
+            self.u0 = _as_f32_vec(u0)
+            self.p = _as_f32_vec(p)
         self.tspan = _as_tspan(tspan)
-        self.p = _as_f32_vec(p)
         self.kwargs = dict(kwargs)
+
+    @property
+    def is_dual(self) -> bool:
+        return isinstance(self.u0, dual.DualArray)
 
     def __repr__(self):
         return f"ODEProblem(u0={self.u0}, tspan={self.tspan}, p={self.p})"
@@ -195,6 +232,7 @@ class ImmutableODEProblem:
     tspan: tuple
     p: np.ndarray
     kwargs: dict = field(default_factory=dict)
+    dual: Optional[tuple] = None  # (n, np, P) of a problem with dual inputs (u0 / p augmented)
 
 
 @dataclass(frozen=True)
@@ -223,6 +261,12 @@ def make_prob_compatible(prob):
                                    prob.noise_rate_prototype, dict(prob.kwargs))
     if not isinstance(prob, ODEProblem):
         raise TypeError("make_prob_compatible expects an ODEProblem or SDEProblem")
+    if prob.is_dual:
+        # the augmented Float32 problem: values, then partials
+        P = prob.u0.npartials
+        f = DualODEFunction(prob.f, len(prob.u0), len(prob.p), P)
+        return ImmutableODEProblem(f, prob.u0.augmented(), prob.tspan, prob.p.augmented(), dict(prob.kwargs),
+                                   dual=(len(prob.u0), len(prob.p), P))
     return ImmutableODEProblem(prob.f, prob.u0.copy(), prob.tspan, prob.p.copy(), dict(prob.kwargs))
 
 

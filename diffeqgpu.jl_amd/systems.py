@@ -26,7 +26,7 @@ and the SDE problems (drift f, diffusion g; SDE_SYSTEMS):
   ou         Ornstein-Uhlenbeck f = p1 (p2 - u), g = p3
   bm         f = 0, g = p1 (scaled Brownian motion)
 """
-from .problems import ODEFunction, SDEFunction
+from .problems import DualODEFunction, ODEFunction, SDEFunction
 from .tracer import SVector
 
 
@@ -145,6 +145,11 @@ SYSTEMS = {
 }
 # the systems with a mass matrix, solved by the implicit methods only
 MASS_SYSTEMS = ("rober_dae", "robertson_eye", "dae2")
+
+# Lorenz with dual-number state and parameters (6 partials: d/du0 and d/dp), augmented to 21 + 21
+# entries; solved by the explicit methods only
+SYSTEMS["lorenz_dual"] = (DualODEFunction(ODEFunction(lorenz), 3, 3, 6), 21, 21, 12)
+DUAL_SYSTEMS = ("lorenz_dual",)
 
 
 def gbm_f(u, p, t):

@@ -290,6 +290,8 @@ class Vec:
 # ------------------------------------------------------------------------------------------
 def _unary(name, pyfn):
     def fn(x):
+        if isinstance(x, DualNode):
+            return x.unary(name)
         if isinstance(x, Node):
             return x.g.mk(name, (x,))
         return pyfn(x)
@@ -308,6 +310,8 @@ fabs = _unary("abs", abs)
 
 
 def pow(x, y):  # noqa: A001 - mirrors the math function name
+    if isinstance(x, DualNode) or isinstance(y, DualNode):
+        return _dual_pow(x, y)
     if isinstance(x, Node) or isinstance(y, Node):
         g = x.g if isinstance(x, Node) else y.g
         return g.mk("pow", (g.lift(x), g.lift(y)))
@@ -327,6 +331,14 @@ def ifelse(cond_lhs, cmp, cond_rhs=None, a=None, b=None):
         raise TraceError("ifelse(cond, a, b) needs a traced comparison or a bool")
     if cmp not in ("<", "<=", ">", ">=", "==", "!="):
         raise TraceError(f"bad comparison {cmp!r}")
+    if any(isinstance(v, DualNode) for v in (cond_lhs, cond_rhs, a, b)):
+        # comparisons of dual numbers compare their values; the branch selects value and partials
+        lhs = cond_lhs.v if isinstance(cond_lhs, DualNode) else cond_lhs
+        rhs = cond_rhs.v if isinstance(cond_rhs, DualNode) else cond_rhs
+        P = next(v.P for v in (cond_lhs, cond_rhs, a, b) if isinstance(v, DualNode))
+        g = next(v.g for v in (cond_lhs, cond_rhs, a, b) if isinstance(v, DualNode))
+        da, db = _as_dual(g, a, P), _as_dual(g, b, P)
+        return DualNode(ifelse(lhs, cmp, rhs, da.v, db.v), [ifelse(lhs, cmp, rhs, x, y) for x, y in zip(da.d, db.d)])
     vals = [cond_lhs, cond_rhs, a, b]
     g = next((v.g for v in vals if isinstance(v, Node)), None)
     if g is None:
@@ -334,6 +346,186 @@ def ifelse(cond_lhs, cmp, cond_rhs=None, a=None, b=None):
               ">=": cond_lhs >= cond_rhs, "==": cond_lhs == cond_rhs, "!=": cond_lhs != cond_rhs}[cmp]
         return a if ok else b
     return g.mk("select", tuple(g.lift(v) for v in vals), cmp)
+
+
+# ------------------------------------------------------------------------------------------
+# dual numbers (forward mode, the arithmetic of ForwardDiff.Dual) over traced values
+# ------------------------------------------------------------------------------------------
+class DualNode:
+    """A traced dual number: a value node ``v`` and P partial nodes ``d``. Each operation emits
+    the value and the partials as separate Float32 operations, in these forms (x, y dual, r real):
+    x +- y: (xv +- yv, xd +- yd); x * y: (xv yv, xd yv + yd xv); x / y: (xv / yv,
+    xd (1 / yv) + yd (-(xv / (yv yv)))); r * x: (r xv, r xd); x / r: (xv / r, xd / r);
+    r / y: (r / yv, yd (-(r / (yv yv)))); f(x): (f(xv), f'(xv) xd) with f' = cos, -sin, exp,
+    1 / x, 1 / (2 sqrt x), 1 - tanh^2, and for abs the sign of xv (negative partials where xv < 0)."""
+
+    __slots__ = ("v", "d")
+
+    def __init__(self, v: Node, d: Sequence[Node]):
+        self.v = v
+        self.d = tuple(d)
+
+    @property
+    def g(self) -> "Graph":
+        return self.v.g
+
+    @property
+    def P(self) -> int:
+        return len(self.d)
+
+    def _other(self, o):
+        if isinstance(o, DualNode):
+            if o.P != self.P:
+                raise TraceError("dual numbers with different numbers of partials")
+            return o
+        return self.g.lift(o)
+
+    def __add__(self, o):
+        o = self._other(o)
+        if isinstance(o, DualNode):
+            return DualNode(self.v + o.v, [a + b for a, b in zip(self.d, o.d)])
+        return DualNode(self.v + o, self.d)
+
+    def __radd__(self, o):
+        o = self.g.lift(o)
+        return DualNode(o + self.v, self.d)
+
+    def __sub__(self, o):
+        o = self._other(o)
+        if isinstance(o, DualNode):
+            return DualNode(self.v - o.v, [a - b for a, b in zip(self.d, o.d)])
+        return DualNode(self.v - o, self.d)
+
+    def __rsub__(self, o):
+        o = self.g.lift(o)
+        return DualNode(o - self.v, [-a for a in self.d])
+
+    def __mul__(self, o):
+        o = self._other(o)
+        if isinstance(o, DualNode):
+            return DualNode(self.v * o.v, [a * o.v + b * self.v for a, b in zip(self.d, o.d)])
+        return DualNode(self.v * o, [a * o for a in self.d])
+
+    def __rmul__(self, o):
+        o = self.g.lift(o)
+        return DualNode(o * self.v, [o * a for a in self.d])
+
+    def __truediv__(self, o):
+        o = self._other(o)
+        if isinstance(o, DualNode):
+            inv = 1.0 / o.v
+            c = -(self.v / (o.v * o.v))
+            return DualNode(self.v / o.v, [a * inv + b * c for a, b in zip(self.d, o.d)])
+        return DualNode(self.v / o, [a / o for a in self.d])
+
+    def __rtruediv__(self, o):
+        o = self.g.lift(o)
+        c = -(o / (self.v * self.v))
+        return DualNode(o / self.v, [b * c for b in self.d])
+
+    def __neg__(self):
+        return DualNode(-self.v, [-a for a in self.d])
+
+    def __pos__(self):
+        return self
+
+    def __pow__(self, e):
+        if isinstance(e, int) and not isinstance(e, bool):
+            if e == 0:
+                return _as_dual(self.g, 1.0, self.P)
+            if e == 1:
+                return self
+            if e == 2:
+                return self * self
+            if e == 3:
+                return (self * self) * self
+            if e == -1:
+                return 1.0 / self
+        return _dual_pow(self, e)
+
+    def __rpow__(self, b):
+        return _dual_pow(b, self)
+
+    def __bool__(self):
+        raise TraceError("a traced dual number has no truth value: use diffeqgpu_amd.math.ifelse")
+
+    def unary(self, name: str) -> "DualNode":
+        g, x = self.g, self.v
+        if name == "abs":
+            zero = g.const(0.0)
+            return DualNode(g.mk("abs", (x,)), [g.mk("select", (x, zero, -a, a), "<") for a in self.d])
+        fx = g.mk(name, (x,))
+        if name == "sin":
+            fp = g.mk("cos", (x,))
+        elif name == "cos":
+            fp = -g.mk("sin", (x,))
+        elif name == "exp":
+            fp = fx
+        elif name == "log":
+            fp = 1.0 / x
+        elif name == "sqrt":
+            fp = 1.0 / (2.0 * fx)
+        elif name == "tanh":
+            fp = 1.0 - fx * fx
+        else:  # pragma: no cover
+            raise TraceError(f"no dual rule for {name}")
+        return DualNode(fx, [fp * a for a in self.d])
+
+
+def _as_dual(g: "Graph", x, P: int) -> DualNode:
+    if isinstance(x, DualNode):
+        return x
+    zero = g.const(0.0)
+    return DualNode(g.lift(x), [zero] * P)
+
+
+def _dual_pow(x, y) -> DualNode:
+    """x^y with a dual base and/or exponent: (xv^yv, (yv xv^(yv - 1)) xd + (xv^yv log xv) yd)."""
+    ref = x if isinstance(x, DualNode) else y
+    g, P = ref.g, ref.P
+    if isinstance(x, DualNode) and not isinstance(y, DualNode):
+        yv = g.lift(y)
+        fp = yv * g.mk("pow", (x.v, yv - 1.0))
+        return DualNode(g.mk("pow", (x.v, yv)), [fp * a for a in x.d])
+    if not isinstance(x, DualNode):
+        xv = g.lift(x)
+        val = g.mk("pow", (xv, y.v))
+        fp = val * g.mk("log", (xv,))
+        return DualNode(val, [fp * b for b in y.d])
+    val = g.mk("pow", (x.v, y.v))
+    fa = y.v * g.mk("pow", (x.v, y.v - 1.0))
+    fb = val * g.mk("log", (x.v,))
+    return DualNode(val, [fa * a + fb * b for a, b in zip(x.d, y.d)])
+
+
+def emit_dual_rhs(f: Callable, n: int, np_: int, P: int) -> Tuple[str, int]:
+    """HIP C for the augmented right-hand side of a problem with dual inputs: deg_rhs over the
+    n (1 + P) state entries (values, then the partials of each component) with the np (1 + P)
+    parameters laid out the same way, and the macros DEG_DUAL_NV (n) and DEG_DUAL_P (P) that
+    select the dual error norm. Returns the source and the flop count of the value part."""
+    if P < 1:
+        raise TraceError("a dual problem needs at least one partial")
+    na, npa = n * (1 + P), np_ * (1 + P)
+    g, ua, pa, t = _new_inputs(na, npa)
+    u = [DualNode(ua[i], [ua[n + i * P + j] for j in range(P)]) for i in range(n)]
+    p = [DualNode(pa[k], [pa[np_ + k * P + j] for j in range(P)]) for k in range(np_)]
+    out = f(SymVec(u, "u"), SymVec(p, "p"), t)
+    if isinstance(out, (int, float, Node, DualNode)):
+        out = (out,)
+    out = list(out)
+    if len(out) != n:
+        raise TraceError(f"f must return {n} entries, got {len(out)}")
+    duals = [_as_dual(g, x, P) for x in out]
+    outs = [x.v for x in duals] + [x.d[j] for x in duals for j in range(P)]
+    body, _ = _emit_body(g, outs, "du")
+    # the value part alone, for the flop count
+    vtr = trace(f, n, np_, nout=n)
+    _vb, flops = _emit_body(vtr.g, vtr.outs, "du")
+    src = [f"#define DEG_DUAL_NV {n}", f"#define DEG_DUAL_P {P}",
+           "__device__ __forceinline__ void deg_rhs(const float* __restrict__ u, const float* __restrict__ p, float t,",
+           "                                        float* __restrict__ du) {",
+           "    (void)u; (void)p; (void)t;", *body, "}"]
+    return "\n".join(src) + "\n", flops
 
 
 # ------------------------------------------------------------------------------------------

@@ -158,6 +158,11 @@ int deg_set_cache_dir(const char *dir);
  * Rosenbrock / Rodas steps) is given as
  *   #define DEG_MASS 1   #define DEG_MASS_ENTRIES <n*n float literals, row-major>
  * a singular M makes the rows with zero mass algebraic (a DAE); without DEG_MASS, M is the identity.
+ * A problem with dual-number inputs (ForwardDiff duals in u0 / p, test/gpu_kernel_de/forward_diff.jl) is
+ * an augmented state of n values followed by the P partials of each component, n (1 + P) entries in all,
+ * with the parameters laid out the same way; the source then also defines
+ *   #define DEG_DUAL_NV <n>   #define DEG_DUAL_P <P>
+ * which selects the dual error norm (DEG_TSIT5, DEG_VERN7, DEG_VERN9 only, no callbacks).
  * With DEG_F_CALLBACKS it also defines the callbacks (GPUDiscreteCallback / GPUContinuousCallback,
  * src/ensemblegpukernel/callbacks.jl:1-124), j counting from 0 in CallbackSet order:
  *   #define DEG_NDCB <discrete callbacks>   #define DEG_NCCB <continuous callbacks>

@@ -58,7 +58,7 @@
 #include "tableaus.h"
 #include "systems.h"
 
-#define OMAXN 16
+#define OMAXN 24
 #define OMAXS 17
 
 enum { OM_TSIT5 = 0, OM_VERN7 = 1, OM_RB23 = 2, OM_VERN9 = 3, OM_RODAS4 = 4, OM_RODAS5P = 5, OM_KVAERNO3 = 6,
@@ -290,11 +290,46 @@ static void erk_attempt(const erk32 *T, const osys *S, const float *p, const flo
     }
 }
 
+/* The same error norm over a dual-number state (S->dualnv values with S->dualp partials each,
+ * component i's partials at dualnv + i dualp + j): the scaling e ./ (abstol .+ max.(|uprev|,|u|) *
+ * reltol) in dual arithmetic (abs negates the partials of a negative value; max picks by value, u only
+ * when larger; e / s has partials e' (1 / s) + s' (-(e / s^2))), then the root mean square over all
+ * values and partials, summed as value^2 + (partial_1^2 + ... + partial_P^2) per component. */
+static float dual_eest(int nv, int P, int n, const float *e, const float *uprev, const float *u, float abstol,
+                       float reltol) {
+    float acc = 0.0f;
+    for (int i = 0; i < nv; ++i) {
+        const float av = uprev[i], bv = u[i];
+        const float aa = fabsf(av), ba = fabsf(bv);
+        const int nan = (aa != aa) || (ba != ba);
+        const int pick_u = !nan && (ba > aa);
+        const float mv = nan ? aa + ba : (pick_u ? ba : aa);
+        const int neg = pick_u ? (bv < 0.0f) : (av < 0.0f);
+        const float sv = abstol + mv * reltol;
+        const float ev = e[i];
+        const float qv = ev / sv;
+        const float inv = 1.0f / sv;
+        const float c = -(ev / (sv * sv));
+        float ps = 0.0f;
+        for (int j = 0; j < P; ++j) {
+            const int q = nv + i * P + j;
+            const float md = pick_u ? u[q] : uprev[q];
+            const float sd = (neg ? -md : md) * reltol;
+            const float qd = e[q] * inv + sd * c;
+            ps = (j == 0) ? qd * qd : ps + qd * qd;
+        }
+        const float term = qv * qv + ps;
+        acc = (i == 0) ? term : acc + term;
+    }
+    return sqrtf(acc / (float)n);
+}
+
 /* scaled error  EEst = norm( dt*(sum btilde_j k_j) ./ (abstol .+ max.(|uprev|,|u|) * reltol) ) */
 static float erk_eest(const erk32 *T, int n, float dt, float k[][OMAXN], const float *uprev,
-                      const float *u, float abstol, float reltol) {
+                      const float *u, float abstol, float reltol, int dualnv, int dualp) {
     float e[OMAXN];
     weighted_sum(T->bt, T->btm, T->s, n, dt, k, e);
+    if (dualp > 0) return dual_eest(dualnv, dualp, n, e, uprev, u, abstol, reltol);
     for (int c = 0; c < n; ++c) {
         float sc = abstol + jl_max(fabsf(uprev[c]), fabsf(u[c])) * reltol;
         e[c] = e[c] / sc;
@@ -1334,7 +1369,7 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                     /* gpu_vern7_perform_step.jl:117, gpu_vern9_perform_step.jl:170: k1 in every attempt */
                     if (J->method == OM_VERN7 || J->method == OM_VERN9) S->f(s->uprev, s->p, t, k[1]);
                     erk_attempt(T, S, s->p, s->uprev, t, dt, k, unew);
-                    EEst = erk_eest(T, n, dt, k, s->uprev, unew, J->abstol, J->reltol);
+                    EEst = erk_eest(T, n, dt, k, s->uprev, unew, J->abstol, J->reltol, S->dualnv, S->dualp);
                 }
                 /* PI controller: gpu_tsit5_perform_step.jl:124-137 */
                 if (EEst == 0.0f) {
@@ -1420,6 +1455,10 @@ int deg_oracle_solve_cb(int system, int method, int adaptive, int64_t N,
     if (system < 0 || system >= OSYS_COUNT) return -1;
     if (method < OM_TSIT5 || method > OM_KVAERNO5) return -1;
     if (ntstops < 0 || ncb < 0 || (ntstops > 0 && !tstops) || (ncb > 0 && !cbs)) return -1;
+    /* a system without an analytic Jacobian (the dual Lorenz) is solved by the explicit methods */
+    if (!OSYSTEMS[system].jac && !(flags & DEG_O_FDJAC) &&
+        !(method == OM_TSIT5 || method == OM_VERN7 || method == OM_VERN9))
+        return -1;
     init_tabs();
     osys Sys = OSYSTEMS[system];
     Sys.fd = (flags & DEG_O_FDJAC) != 0;

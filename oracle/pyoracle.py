@@ -15,10 +15,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "libdeg_oracle.so")
 
 SYSTEM_IDS = {"lorenz": 0, "decay": 1, "growth": 2, "quad": 3, "robertson": 4, "vdp": 5, "tdep": 6, "rot": 7,
-              "ball": 8, "rober_dae": 9, "robertson_eye": 10, "dae2": 11}
+              "ball": 8, "rober_dae": 9, "robertson_eye": 10, "dae2": 11, "lorenz_dual": 12}
 SYSTEM_DIMS = {"lorenz": (3, 3), "decay": (1, 1), "growth": (1, 1), "quad": (1, 1), "robertson": (3, 3),
                "vdp": (2, 1), "tdep": (1, 1), "rot": (2, 1), "ball": (2, 1), "rober_dae": (3, 3),
-               "robertson_eye": (3, 3), "dae2": (2, 0)}
+               "robertson_eye": (3, 3), "dae2": (2, 0), "lorenz_dual": (21, 21)}
 METHOD_IDS = {"tsit5": 0, "vern7": 1, "rb23": 2, "vern9": 3, "rodas4": 4, "rodas5p": 5, "kvaerno3": 6,
               "kvaerno5": 7}
 KAHAN, DETPOW, FDJAC = 1, 2, 4

@@ -25,7 +25,7 @@
 
 enum { OSYS_LORENZ = 0, OSYS_DECAY = 1, OSYS_GROWTH = 2, OSYS_QUAD = 3, OSYS_ROBERTSON = 4,
        OSYS_VDP = 5, OSYS_TDEP = 6, OSYS_ROT = 7, OSYS_BALL = 8, OSYS_ROBER_DAE = 9, OSYS_ROBERTSON_EYE = 10,
-       OSYS_DAE2 = 11, OSYS_COUNT = 12 };
+       OSYS_DAE2 = 11, OSYS_LORENZ_DUAL = 12, OSYS_COUNT = 13 };
 
 typedef void (*orhs_fn)(const float *u, const float *p, float t, float *du);
 typedef void (*ojac_fn)(const float *u, const float *p, float t, float *J);
@@ -37,6 +37,7 @@ typedef struct {
     orhs_fn tgrad;
     const float *M;
     int fd; /* 1: jac and tgrad by forward differences of f (autodiff = false) */
+    int dualnv, dualp; /* dual-number state: dualnv values, each with dualp partials (0: plain) */
 } osys;
 
 static void lorenz_f(const float *u, const float *p, float t, float *du) {
@@ -185,19 +186,56 @@ static const float M_DIAG10[4] = {1.0f, 0.0f, 0.0f, 0.0f};
 static const float M_DIAG110[9] = {1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 static const float M_EYE3[9] = {1.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 1.0f};
 
+/* Lorenz over dual numbers with 6 partials: state and parameters are the 3 values followed by
+ * the partials of each component (u[3 + 6 i + j]). Each operation takes the form the tracer
+ * emits for it: x - y -> (xv - yv, xd - yd); x * y -> (xv yv, xd yv + yd xv). */
+#define OLD_P 6
+static void lorenz_dual_f(const float *u, const float *p, float t, float *du) {
+    (void)t;
+    const float *u0d = u + 3, *u1d = u + 3 + OLD_P, *u2d = u + 3 + 2 * OLD_P;
+    const float *sd = p + 3, *rd = p + 3 + OLD_P, *bd = p + 3 + 2 * OLD_P;
+    const float s = p[0], r = p[1], b = p[2];
+    /* du1 = s * (u1 - u0) */
+    const float d1 = u[1] - u[0];
+    du[0] = s * d1;
+    for (int j = 0; j < OLD_P; ++j) {
+        const float d1d = u1d[j] - u0d[j];
+        du[3 + j] = sd[j] * d1 + d1d * s;
+    }
+    /* du2 = u0 * (r - u2) - u1 */
+    const float e = r - u[2];
+    const float m = u[0] * e;
+    du[1] = m - u[1];
+    for (int j = 0; j < OLD_P; ++j) {
+        const float ed = rd[j] - u2d[j];
+        const float md = u0d[j] * e + ed * u[0];
+        du[3 + OLD_P + j] = md - u1d[j];
+    }
+    /* du3 = u0 * u1 - b * u2 */
+    const float a = u[0] * u[1];
+    const float bb = b * u[2];
+    du[2] = a - bb;
+    for (int j = 0; j < OLD_P; ++j) {
+        const float ad = u0d[j] * u[1] + u1d[j] * u[0];
+        const float bbd = bd[j] * u[2] + u2d[j] * b;
+        du[3 + 2 * OLD_P + j] = ad - bbd;
+    }
+}
+
 static const osys OSYSTEMS[OSYS_COUNT] = {
-    {3, 3, lorenz_f, lorenz_jac, zero_tgrad3, NULL, 0},
-    {1, 1, decay_f, decay_jac, zero_tgrad1, NULL, 0},
-    {1, 1, growth_f, growth_jac, zero_tgrad1, NULL, 0},
-    {1, 1, quad_f, quad_jac, zero_tgrad1, NULL, 0},
-    {3, 3, robertson_f, robertson_jac, zero_tgrad3, NULL, 0},
-    {2, 1, vdp_f, vdp_jac, zero_tgrad2, NULL, 0},
-    {1, 1, tdep_f, tdep_jac, tdep_tgrad, NULL, 0},
-    {2, 1, rot_f, rot_jac, zero_tgrad2, NULL, 0},
-    {2, 1, ball_f, ball_jac, zero_tgrad2, NULL, 0},
-    {3, 3, rober_dae_f, rober_dae_jac, zero_tgrad3, M_DIAG110, 0},
-    {3, 3, robertson_f, robertson_jac, zero_tgrad3, M_EYE3, 0},
-    {2, 0, dae2_f, dae2_jac, zero_tgrad2, M_DIAG10, 0},
+    {3, 3, lorenz_f, lorenz_jac, zero_tgrad3, NULL, 0, 0, 0},
+    {1, 1, decay_f, decay_jac, zero_tgrad1, NULL, 0, 0, 0},
+    {1, 1, growth_f, growth_jac, zero_tgrad1, NULL, 0, 0, 0},
+    {1, 1, quad_f, quad_jac, zero_tgrad1, NULL, 0, 0, 0},
+    {3, 3, robertson_f, robertson_jac, zero_tgrad3, NULL, 0, 0, 0},
+    {2, 1, vdp_f, vdp_jac, zero_tgrad2, NULL, 0, 0, 0},
+    {1, 1, tdep_f, tdep_jac, tdep_tgrad, NULL, 0, 0, 0},
+    {2, 1, rot_f, rot_jac, zero_tgrad2, NULL, 0, 0, 0},
+    {2, 1, ball_f, ball_jac, zero_tgrad2, NULL, 0, 0, 0},
+    {3, 3, rober_dae_f, rober_dae_jac, zero_tgrad3, M_DIAG110, 0, 0, 0},
+    {3, 3, robertson_f, robertson_jac, zero_tgrad3, M_EYE3, 0, 0, 0},
+    {2, 0, dae2_f, dae2_jac, zero_tgrad2, M_DIAG10, 0, 0, 0},
+    {21, 21, lorenz_dual_f, NULL, NULL, NULL, 0, 3, 6},
 };
 
 /* The same right-hand sides in double precision, for the FP64 oracle. */
@@ -248,6 +286,7 @@ static const orhs_fn_d OSYSTEMS_D[OSYS_COUNT] = {
     lorenz_fd, decay_fd, growth_fd, quad_fd, robertson_fd, vdp_fd, tdep_fd, rot_fd, ball_fd,
     robertson_fd, robertson_fd, /* the DAE's solution is the ODE form's (not integrated in FP64 here) */
     rot_fd,                     /* placeholder: the 2 x 2 DAE has no FP64 ODE form here */
+    rot_fd,                     /* placeholder: the dual Lorenz is not integrated in FP64 */
 };
 
 #endif

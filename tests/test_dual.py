@@ -1,0 +1,148 @@
+"""Dual-number (forward-mode) inputs: the host API, the JIT, and the oracle's dual Lorenz pinned by
+what fixes it.
+
+  values          fixed step: the value part of a dual solve is bit-identical to the plain solve
+                  (each dual operation computes its value exactly as the plain operation does)
+  partials        d u(t) / d (u0, p) at t = 1 against central differences of the FP64 Vern9 flow
+                  (rtol 1e-12) at the solve's end time, fixed and adaptive, for Tsit5, Vern7 and Vern9; a sign or index slip
+                  in a partial rule is off by O(1)
+  dual norm       with zero partials the dual error norm is the plain one divided by sqrt(1 + P);
+                  with abstol = 0 the plain norm scales as 1 / reltol, so an adaptive dual solve
+                  with zero seeds at reltol r takes the steps of the plain solve at r sqrt(1 + P)
+                  (up to the rounding of that product); the plain norm (values alone) at r takes
+                  more than 10 % more steps
+GPU parity with the oracle is tested in test_gpu_dual.py.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as P
+
+F32 = np.float32
+NP = 6  # partials: d/du0 (3) and d/dp (3)
+
+
+def _seeded(N, rng=None):
+    """Augmented (21, N) u0 and p of the dual Lorenz with the identity seeds."""
+    rng = rng or np.random.default_rng(5)
+    u0v = np.tile(np.array([[1.0], [0.0], [0.0]], F32), (1, N))
+    pv = (np.array([[10.0], [28.0], [8.0 / 3.0]], F32) * rng.uniform(0.3, 1.0, (1, N)).astype(F32)).astype(F32)
+    seeds = np.eye(NP, dtype=F32)
+    u0 = np.concatenate([u0v, np.repeat(seeds[:3].reshape(-1, 1), N, axis=1)]).astype(F32)
+    p = np.concatenate([pv, np.repeat(seeds[3:].reshape(-1, 1), N, axis=1)]).astype(F32)
+    return u0, p, u0v, pv
+
+
+def test_dual_api(deg):
+    D = deg.Dual
+    u0 = [D(1.0, (1, 0, 0, 0, 0, 0)), D(0.0, (0, 1, 0, 0, 0, 0)), D(0.0, (0, 0, 1, 0, 0, 0))]
+    p = [D(10.0, (0, 0, 0, 1, 0, 0)), D(28.0, (0, 0, 0, 0, 1, 0)), D(8 / 3, (0, 0, 0, 0, 0, 1))]
+    from diffeqgpu_amd import systems
+
+    prob = deg.ODEProblem(systems.lorenz, u0, (0.0, 10.0), p)
+    assert prob.is_dual and prob.u0.npartials == 6 and prob.u0[1] == u0[1]
+    cp = deg.make_prob_compatible(prob)
+    assert cp.dual == (3, 3, 6) and cp.u0.shape == (21,) and cp.p.shape == (21,)
+    assert cp.u0[3] == 1.0 and cp.u0[3 + 7] == 1.0 and cp.u0[3 + 14] == 1.0 and cp.u0[3:].sum() == 3.0
+    assert cp.p[2] == F32(8 / 3) and cp.p[3 + 6 * 0 + 3] == 1.0 and cp.p[3 + 6 * 2 + 5] == 1.0
+    # plain numbers mixed with duals get zero partials; remake keeps the duals
+    q = deg.remake(prob, p=[10.0, 28.0, D(2.0, (0, 0, 0, 0, 0, 1))])
+    assert q.is_dual and np.all(q.p.partials[:2] == 0) and q.p.partials[2, 5] == 1.0
+    with pytest.raises(ValueError):
+        deg.ODEProblem(systems.lorenz, [D(1.0, (1,)), D(0.0, (0, 1)), 0.0], (0.0, 1.0), [1.0, 2.0, 3.0])
+    # saved rows as (n, 1 + P)
+    from diffeqgpu_amd.dual import split_rows
+
+    rows = split_rows(np.arange(2 * 21, dtype=F32).reshape(2, 21), 3, 6)
+    assert rows.shape == (2, 3, 7) and rows[1, 2, 0] == 21 + 2 and rows[1, 2, 1] == 21 + 3 + 12
+    # the implicit methods and callbacks are refused before any device work
+    ep = deg.EnsembleProblem(prob)
+    with pytest.raises(NotImplementedError, match="dual"):
+        deg.solve(ep, deg.GPURosenbrock23(autodiff=False), deg.EnsembleGPUKernel(), trajectories=2, dt=0.01,
+                  adaptive=False)
+
+
+def test_dual_source_and_jit(deg):
+    from diffeqgpu_amd import _lib, systems
+
+    f, n, np_, _ = systems.SYSTEMS["lorenz_dual"]
+    src, flops = f.hip_source(n, np_, False)
+    assert "#define DEG_DUAL_NV 3" in src and "#define DEG_DUAL_P 6" in src and flops == 8
+    with pytest.raises(NotImplementedError):
+        f.hip_source(n, np_, True)
+    for m in (_lib.DEG_TSIT5, _lib.DEG_VERN7):
+        assert len(_lib.precompile(src, n, np_, m, _lib.F_PARITY)) == 16
+    for m in (_lib.DEG_RB23, _lib.DEG_KVAERNO5):
+        with pytest.raises(_lib.DegError, match="explicit"):
+            _lib.precompile(src, n, np_, m, _lib.F_PARITY)
+
+
+@pytest.mark.parametrize("method", ["tsit5", "vern7", "vern9"])
+def test_dual_fixed_values_equal_plain(oracle, method):
+    N = 64
+    u0, p, u0v, pv = _seeded(N)
+    d = P.solve("lorenz_dual", method, False, u0, p, (0.0, 2.0), 0.01, save_everystep=False)
+    o = P.solve("lorenz", method, False, u0v, pv, (0.0, 2.0), 0.01, save_everystep=False)
+    np.testing.assert_array_equal(d["us"][:, :3], o["us"])
+    np.testing.assert_array_equal(d["ts"], o["ts"])
+
+
+def _fd_jacobian(u0v, pv, T, h=1e-3):
+    """d u(T) / d (u0, p) of the exact flow by central differences of FP64 Vern9 (rtol 1e-12). The
+    oracle takes Float32 inputs, so the step is h (1 + |x|) and the quotient uses the difference of
+    the rounded inputs; the truncation error is O(h^2)."""
+    N = u0v.shape[1]
+    base = np.concatenate([u0v, pv]).astype(F32)
+    J = np.zeros((3, NP, N))
+    for j in range(NP):
+        up, dn = base.copy(), base.copy()
+        step = (h * (1.0 + np.abs(base[j].astype(np.float64))))
+        up[j] = (base[j] + step).astype(F32)
+        dn[j] = (base[j] - step).astype(F32)
+        ru, _, _ = P.vern9("lorenz", up[:3], up[3:], (0.0, T), saveat=np.array([T]))
+        rd, _, _ = P.vern9("lorenz", dn[:3], dn[3:], (0.0, T), saveat=np.array([T]))
+        J[:, j, :] = (ru[-1] - rd[-1]) / (up[j].astype(np.float64) - dn[j].astype(np.float64))
+    return J
+
+
+@pytest.mark.parametrize("method", ["tsit5", "vern7", "vern9"])
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_dual_partials_against_flow_derivative(oracle, method, adaptive):
+    N = 16
+    T = 1.0
+    u0, p, u0v, pv = _seeded(N)
+    kw = dict(abstol=1e-7, reltol=1e-7) if adaptive else {}
+    d = P.solve("lorenz_dual", method, adaptive, u0, p, (0.0, T), 0.001, save_everystep=False, **kw)
+    assert np.all(d["retcode"] == 0)
+    parts = d["us"][-1, 3:].astype(np.float64).reshape(3, NP, N)
+    # a fixed-step solve ends where the Float32 time accumulates to (1000 steps of 0.001f0: 1.00099)
+    te = float(d["ts"][-1, 0])
+    assert np.all(d["ts"][-1] == d["ts"][-1, 0]) and (adaptive or te > T)
+    J = _fd_jacobian(u0v, pv, te, h=1e-5)
+    scale = np.abs(J).max(axis=(0, 1))
+    err = np.abs(parts - J).max(axis=(0, 1)) / scale
+    assert err.max() < 1e-3, err.max()
+
+
+def test_dual_norm_zero_seeds(oracle):
+    """Zero partials: the dual norm is the plain norm / sqrt(1 + P). With abstol = 0 the scaled error is
+    e / (|u| reltol), so the dual solve at reltol r takes the steps of the plain solve at reltol
+    r sqrt(1 + P), up to the rounding of that product (step counts within 5 %, 1 % in total)."""
+    N = 64
+    rng = np.random.default_rng(9)
+    u0v = (np.array([[1.0], [0.5], [2.0]], F32) + rng.uniform(0, 1, (3, N)).astype(F32)).astype(F32)
+    pv = (np.array([[10.0], [28.0], [8.0 / 3.0]], F32) * rng.uniform(0.3, 0.5, (1, N)).astype(F32)).astype(F32)
+    u0 = np.concatenate([u0v, np.zeros((18, N), F32)])
+    p = np.concatenate([pv, np.zeros((18, N), F32)])
+    r = 1e-6
+    d = P.solve("lorenz_dual", "tsit5", True, u0, p, (0.0, 1.0), 0.01, abstol=0.0, reltol=r, save_everystep=False)
+    o = P.solve("lorenz", "tsit5", True, u0v, pv, (0.0, 1.0), 0.01, abstol=0.0, reltol=float(F32(r * np.sqrt(7.0))),
+                save_everystep=False)
+    na_d, na_o = d["naccept"] + d["nreject"], o["naccept"] + o["nreject"]
+    assert np.all(d["us"][-1, 3:] == 0.0)
+    assert np.abs(na_d.astype(float) / na_o - 1).max() < 0.05, (na_d[:8], na_o[:8])
+    assert np.abs(na_d.sum() / na_o.sum() - 1) < 0.01
+    np.testing.assert_allclose(d["us"][-1, :3], o["us"][-1], rtol=1e-4, atol=1e-5)
+    # and not the plain solve at the unscaled tolerance
+    o2 = P.solve("lorenz", "tsit5", True, u0v, pv, (0.0, 1.0), 0.01, abstol=0.0, reltol=r, save_everystep=False)
+    assert 1.1 * na_d.sum() < (o2["naccept"] + o2["nreject"]).sum()

@@ -1,0 +1,109 @@
+"""GPU parity of dual-number (forward-mode) solves against the oracle, and the reference's
+forward_diff.jl replayed through the high-level API.
+
+Bars:
+  * no contraction, deterministic controller pow: bit-identical ts, us (values and partials), return
+    codes and step counts with the oracle's dual Lorenz (6 partials), fixed and adaptive
+    (persistent grid), endpoint and saveat, for GPUTsit5, GPUVern7 and GPUVern9;
+  * forward_diff.jl: u0 and p of Lorenz as duals seeded with the identity, trajectories = 2, fixed
+    dt = 0.01 (endpoint) and adaptive dt = 0.01 for the explicit methods: every trajectory succeeds,
+    the values equal the plain solve's (fixed step: bit for bit in parity mode), the partials are
+    finite and d u(t0) / d u0 is the identity. The reference also runs the implicit methods with
+    autodiff = false there; here they raise NotImplementedError (a W factor with dual entries).
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from test_gpu_parity import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+ALGS = {"tsit5": "GPUTsit5", "vern7": "GPUVern7", "vern9": "GPUVern9"}
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+    return torch
+
+
+def _inputs(N):
+    rng = np.random.default_rng(17)
+    u0v = (np.array([[1.0], [0.0], [0.0]], F32) + rng.uniform(0, 0.5, (3, N)).astype(F32)).astype(F32)
+    pv = (np.array([[10.0], [28.0], [8.0 / 3.0]], F32) * rng.uniform(0.3, 1.0, (1, N)).astype(F32)).astype(F32)
+    seeds = np.eye(6, dtype=F32)
+    u0 = np.concatenate([u0v, np.repeat(seeds[:3].reshape(-1, 1), N, axis=1)]).astype(F32)
+    p = np.concatenate([pv, np.repeat(seeds[3:].reshape(-1, 1), N, axis=1)]).astype(F32)
+    return u0, p
+
+
+@pytest.mark.parametrize("method", ["tsit5", "vern7", "vern9"])
+@pytest.mark.parametrize("adaptive", [False, True])
+@pytest.mark.parametrize("mode", ["endpoint", "saveat"])
+def test_dual_parity_bitwise(deg, torch_cuda, method, adaptive, mode):
+    from diffeqgpu_amd import systems
+
+    N = 256
+    u0, p = _inputs(N)
+    f, n, np_, _ = systems.SYSTEMS["lorenz_dual"]
+    tspan, dt = (0.0, 2.0), 0.01
+    kw = dict(abstol=1e-6, reltol=1e-6) if adaptive else {}
+    okw = dict(kw)
+    if mode == "saveat":
+        sv = np.linspace(0.0, 2.0, 9).astype(F32)
+        kw["saveat"] = sv
+        okw["saveat"] = sv
+    else:
+        kw["save_everystep"] = False
+    if adaptive:
+        dev = deg.HIPDevice(contract="off", powmode="det", persistent=True)
+    else:
+        dev = deg.HIPDevice(contract="off")
+    probs = deg.HIPProbVector.from_arrays(f, u0, p, tspan, dev)
+    prob = deg.ODEProblem(systems.lorenz, u0[:3, 0], tspan, p[:3, 0])
+    alg = getattr(deg, ALGS[method])()
+    print(f"[gpu] dual lorenz {method} adaptive={adaptive} {mode} N={N}", flush=True)
+    fn = deg.vectorized_asolve if adaptive else deg.vectorized_solve
+    res = fn(probs, prob, alg, dt=dt, **kw)
+    torch_cuda.cuda.synchronize()
+    g = dict(ts=res.ts.cpu().numpy(), us=res.us_soa.cpu().numpy(), retcode=res.retcode.cpu().numpy(),
+             naccept=res.naccept.cpu().numpy(), nreject=res.nreject.cpu().numpy())
+    o = pyoracle.solve("lorenz_dual", method, adaptive, u0, p, tspan, dt, detpow=adaptive, **okw)
+    assert_bitwise(g, o)
+    assert (g["retcode"] == 0).all()
+
+
+def test_reference_forward_diff(deg, torch_cuda):
+    """forward_diff.jl:1-58 for the explicit methods."""
+    from diffeqgpu_amd import systems
+
+    D = deg.Dual
+    u0 = [D(1.0, (1, 0, 0, 0, 0, 0)), D(0.0, (0, 1, 0, 0, 0, 0)), D(0.0, (0, 0, 1, 0, 0, 0))]
+    p = [D(10.0, (0, 0, 0, 1, 0, 0)), D(28.0, (0, 0, 0, 0, 1, 0)), D(8.0 / 3.0, (0, 0, 0, 0, 0, 1))]
+    prob = deg.ODEProblem(systems.lorenz, u0, (0.0, 10.0), p)
+    ep = deg.EnsembleProblem(prob, prob_func=lambda pr, ctx: deg.remake(pr, p=p), safetycopy=False)
+    plain = deg.ODEProblem(systems.lorenz, [1.0, 0.0, 0.0], (0.0, 10.0), [10.0, 28.0, 8.0 / 3.0])
+    for method, name in ALGS.items():
+        alg = getattr(deg, name)()
+        print(f"[gpu] reference forward_diff {method}", flush=True)
+        dev = deg.HIPDevice(contract="off")
+        sol = deg.solve(ep, alg, deg.EnsembleGPUKernel(dev), trajectories=2, save_everystep=False,
+                        adaptive=False, dt=0.01)
+        ref = deg.solve(deg.EnsembleProblem(plain), alg, deg.EnsembleGPUKernel(dev), trajectories=2,
+                        save_everystep=False, adaptive=False, dt=0.01)
+        for s, r in zip(sol.u, ref.u):
+            assert s.retcode == "Success" and s.u.shape == (2, 3, 7)
+            np.testing.assert_array_equal(s.u[:, :, 0], r.u)
+            np.testing.assert_array_equal(s.u[0, :, 1:4], np.eye(3, dtype=F32))
+            assert np.all(np.isfinite(s.u))
+        asol = deg.solve(ep, alg, deg.EnsembleGPUKernel(), trajectories=2, adaptive=True, dt=0.01)
+        for s in asol.u:
+            assert s.retcode == "Success" and s.u.shape[1:] == (3, 7) and np.all(np.isfinite(s.u))
+    for name in ("GPURosenbrock23", "GPURodas4", "GPURodas5P", "GPUKvaerno3", "GPUKvaerno5"):
+        with pytest.raises(NotImplementedError, match="dual"):
+            deg.solve(ep, getattr(deg, name)(autodiff=False), deg.EnsembleGPUKernel(), trajectories=2,
+                      save_everystep=False, adaptive=False, dt=0.01)
