@@ -347,9 +347,9 @@ __device__ __forceinline__ float v7q(float th, float r2, float r3, float r4, flo
     return __builtin_fmaf(th, __builtin_fmaf(th, __builtin_fmaf(th, __builtin_fmaf(th, __builtin_fmaf(th, s5, s4), s3), s2), s1), c0);
 }
 #define V7Q(i) v7q(th, V7_r##i##2, V7_r##i##3, V7_r##i##4, V7_r##i##5, V7_r##i##6, V7_r##i##7)
-__device__ __noinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
-                                           const float (&y1)[DEG_N],
-                                        const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
+__device__ __noinline__ void deg_interp_ool(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                               const float (&y1)[DEG_N],
+                                               const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
     (void)y1;
     float tmp[DEG_N], k11[DEG_N], k12[DEG_N], k13[DEG_N], k14[DEG_N], k15[DEG_N], k16[DEG_N];
     DEG_FOR(c) tmp[c] = y0[c] + dt * (V7_a1101 * k[1][c] + V7_a1104 * k[4][c] + V7_a1105 * k[5][c] + V7_a1106 * k[6][c] +
@@ -458,9 +458,9 @@ __device__ __forceinline__ float V9H8(float th, float r1, float r2, float r3, fl
 // Vern9 dense output (interpolants.jl:192-370): extra stages 17..26 from uprev and the step's
 // stages 1, 8..15, then y0 + dt * (sum of b_i(th) k_i). Scalar muladds (stage times, Horner
 // steps) are fused; vector sums are unfused left folds.
-__device__ __noinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
-                                           const float (&y1)[DEG_N],
-                                        const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
+__device__ __noinline__ void deg_interp_ool(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                               const float (&y1)[DEG_N],
+                                               const float (&k)[DEG_NK + 1][DEG_N], float (&out)[DEG_N]) {
     (void)y1;
     float tmp[DEG_N], x[10][DEG_N];  // extra stages 17..26
     DEG_FOR(c) tmp[c] = y0[c] + dt * (V9_a1701 * k[1][c] + V9_a1708 * k[8][c] + V9_a1709 * k[9][c] + V9_a1710 * k[10][c] + V9_a1711 * k[11][c] + V9_a1712 * k[12][c] + V9_a1713 * k[13][c] + V9_a1714 * k[14][c] + V9_a1715 * k[15][c]);
@@ -599,6 +599,24 @@ __device__ __forceinline__ void w_solve(const double (&W)[DEG_N][DEG_N], const i
 #endif
     }
     DEG_FOR(i) x[i] = (float)y[i];
+}
+#endif
+
+#if DEG_METHOD == 1 || DEG_METHOD == 3
+// The Vern7 / Vern9 dense output is out of line (it recomputes 6 / 10 extra stages) and takes the
+// stages by reference. Handing it a copy made here keeps the step's own stage array out of
+// memory: the copy is written to scratch only on the (rare) path that interpolates, instead of
+// every attempt storing its stages for a call that may never come.
+__device__ __forceinline__ void deg_interp(float th, float dt, float tprev, const float* p, const float (&y0)[DEG_N],
+                                           const float (&y1)[DEG_N], const float (&k)[DEG_NK + 1][DEG_N],
+                                           float (&out)[DEG_N]) {
+    float y0c[DEG_N], y1c[DEG_N], kc[DEG_NK + 1][DEG_N];
+    DEG_FOR(c) {
+        y0c[c] = y0[c];
+        y1c[c] = y1[c];
+    }
+    _Pragma("unroll") for (int i = 0; i <= DEG_NK; ++i) DEG_FOR(c) kc[i][c] = k[i][c];
+    deg_interp_ool(th, dt, tprev, p, y0c, y1c, kc, out);
 }
 #endif
 
