@@ -46,6 +46,8 @@ class Node:
 
     # -- arithmetic (left operand first, exactly as written) --
     def _bin(self, op, other, rev=False):
+        if isinstance(other, DualNode):
+            return NotImplemented  # a real times a dual: the dual's reflected operator
         o = self.g.lift(other)
         return self.g.mk(op, (o, self) if rev else (self, o))
 
@@ -63,6 +65,8 @@ class Node:
     def __pow__(self, e):
         # Julia lowers x^2, x^3 with literal integer exponents to repeated multiplication
         # (Base.literal_pow); other exponents call pow.
+        if isinstance(e, DualNode):
+            return NotImplemented
         if isinstance(e, int) and not isinstance(e, bool):
             if e == 0:
                 return self.g.const(1.0)
@@ -498,11 +502,10 @@ def _dual_pow(x, y) -> DualNode:
     return DualNode(val, [fa * a + fb * b for a, b in zip(x.d, y.d)])
 
 
-def emit_dual_rhs(f: Callable, n: int, np_: int, P: int) -> Tuple[str, int]:
-    """HIP C for the augmented right-hand side of a problem with dual inputs: deg_rhs over the
-    n (1 + P) state entries (values, then the partials of each component) with the np (1 + P)
-    parameters laid out the same way, and the macros DEG_DUAL_NV (n) and DEG_DUAL_P (P) that
-    select the dual error norm. Returns the source and the flop count of the value part."""
+def trace_dual(f: Callable, n: int, np_: int, P: int) -> "Trace":
+    """Trace f on dual inputs: a Trace over the n (1 + P) state and np (1 + P) parameter entries
+    (values, then the partials of each component) whose outputs are the n values followed by the
+    partials of each output component."""
     if P < 1:
         raise TraceError("a dual problem needs at least one partial")
     na, npa = n * (1 + P), np_ * (1 + P)
@@ -517,7 +520,16 @@ def emit_dual_rhs(f: Callable, n: int, np_: int, P: int) -> Tuple[str, int]:
         raise TraceError(f"f must return {n} entries, got {len(out)}")
     duals = [_as_dual(g, x, P) for x in out]
     outs = [x.v for x in duals] + [x.d[j] for x in duals for j in range(P)]
-    body, _ = _emit_body(g, outs, "du")
+    return Trace(g, ua, pa, t, outs)
+
+
+def emit_dual_rhs(f: Callable, n: int, np_: int, P: int) -> Tuple[str, int]:
+    """HIP C for the augmented right-hand side of a problem with dual inputs: deg_rhs over the
+    n (1 + P) state entries (values, then the partials of each component) with the np (1 + P)
+    parameters laid out the same way, and the macros DEG_DUAL_NV (n) and DEG_DUAL_P (P) that
+    select the dual error norm. Returns the source and the flop count of the value part."""
+    tr = trace_dual(f, n, np_, P)
+    body, _ = _emit_body(tr.g, tr.outs, "du")
     # the value part alone, for the flop count
     vtr = trace(f, n, np_, nout=n)
     _vb, flops = _emit_body(vtr.g, vtr.outs, "du")

@@ -146,3 +146,46 @@ def test_dual_norm_zero_seeds(oracle):
     # and not the plain solve at the unscaled tolerance
     o2 = P.solve("lorenz", "tsit5", True, u0v, pv, (0.0, 1.0), 0.01, abstol=0.0, reltol=r, save_everystep=False)
     assert 1.1 * na_d.sum() < (o2["naccept"] + o2["nreject"]).sum()
+
+
+def _elementary(u, p, t):
+    """Every dual rule: division (dual / dual, real / dual, dual / real), powers (integer, dual base,
+    dual exponent, both), sin, cos, exp, log, sqrt, tanh, abs, ifelse, and t."""
+    from diffeqgpu_amd import math as m
+    from diffeqgpu_amd.tracer import SVector
+
+    a = m.sin(u[0]) * m.exp(p[0] * u[1]) / (1.0 + u[2] ** 2) + 2.0 / (1.0 + p[0] * p[0]) - u[1] / 3.0
+    b = m.log(2.0 + u[0] * u[0]) - m.sqrt(1.0 + p[1] * u[1] ** 2) + m.tanh(u[0] - p[0]) + m.cos(t * u[2])
+    c = (m.pow(1.5 + u[1] ** 2, p[1]) + m.pow(u[2] * u[2] + 1.0, 0.7) + m.pow(2.0, u[1])
+         + m.fabs(u[0] - 0.3) * u[2] + m.ifelse(u[0], ">", 0.5, u[1] * u[2], -u[2]) - u[0] ** 3)
+    return SVector(a, b, c)
+
+
+@pytest.mark.parametrize("point", [0, 1, 2])
+def test_dual_rules_against_finite_differences(point):
+    """The traced dual evaluation (interpreted in float64) against central differences of the plain
+    function along each seed: a wrong derivative factor or sign in any rule fails."""
+    from diffeqgpu_amd import tracer
+    from test_host import _eval_nodes
+
+    rng = np.random.default_rng(point)
+    u = rng.uniform(-1.0, 1.5, 3)
+    p = rng.uniform(0.3, 1.2, 2)
+    t = 0.4 + 0.3 * point
+    n, npp, P = 3, 2, 5
+    tr = tracer.trace_dual(_elementary, n, npp, P)
+    seeds = np.eye(P)
+    ua = np.concatenate([u, seeds[:3].reshape(-1)])
+    pa = np.concatenate([p, seeds[3:].reshape(-1)])
+    out = np.array(_eval_nodes(tr, tr.outs, ua, pa, t))
+    vals, parts = out[:n], out[n:].reshape(n, P)
+    np.testing.assert_allclose(vals, tracer.evaluate(_elementary, u, p, t), rtol=1e-12)
+    x0 = np.concatenate([u, p])
+    h = 1e-6
+    for j in range(P):
+        xp, xm = x0.copy(), x0.copy()
+        xp[j] += h
+        xm[j] -= h
+        fd = (np.array(tracer.evaluate(_elementary, xp[:3], xp[3:], t))
+              - np.array(tracer.evaluate(_elementary, xm[:3], xm[3:], t))) / (2 * h)
+        np.testing.assert_allclose(parts[:, j], fd, rtol=1e-6, atol=1e-7)
