@@ -143,6 +143,50 @@ def solve_kw(cfg):
     return kw
 
 
+def host_cpu():
+    """(threads for the CPU baseline, online CPUs, CPU model name) of the host. The threads are what
+    `nproc` reports for this process: the CPUs it may run on, limited by OMP_NUM_THREADS when that is
+    set (a shared host gives each GPU job its share that way)."""
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        threads = os.cpu_count() or 1
+    try:
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        omp = 0
+    if omp > 0:
+        threads = min(threads, omp)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, os.cpu_count() or threads, model
+
+
+def oracle_fp64_error(name, cfg, pyoracle):
+    """The same error statistic for the CPU restatement of the reference's algorithm (Float32, the
+    reference's operation sequence with the deterministic controller power) on the first 1,000
+    trajectories: the reference's own error at these tolerances."""
+    method = {"GPUTsit5": "tsit5", "GPUVern7": "vern7", "GPURosenbrock23": "rb23"}[WORKLOADS[name]["alg"]]
+    m = min(cfg["N"], 1000)
+    kw = dict(abstol=cfg["abstol"], reltol=cfg["reltol"], kahan=bool(cfg.get("kahan")), detpow=True)
+    sv = cfg.get("saveat")
+    if sv is not None:
+        kw["saveat"] = (np.arange(11, dtype=np.float32) if np.ndim(sv) == 0 else np.asarray(sv, np.float32))
+    else:
+        kw["save_everystep"] = False
+    if cfg.get("tfs") is not None:
+        kw["tfs"] = cfg["tfs"][:m]
+    r = pyoracle.solve(cfg["system"], method, True, cfg["u0"][:, :m], cfg["p"][:, :m], cfg["tspan"], cfg["dt"], **kw)
+    return fp64_error(name, r["us"])
+
+
 def cpu_baseline(name, cfg, threads, sample):
     """The oracle (CPU Float32 restatement of the same algorithm) on the host cores, same inputs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -164,11 +208,14 @@ def cpu_baseline(name, cfg, threads, sample):
         pyoracle.solve(*args, **kw)
         reps += 1
         el = time.perf_counter() - t0
-        if el > 10.0 or reps >= 50:
+        if el > 10.0 or reps >= 400:
             break
+    _, nproc, model = host_cpu()
     return dict(value=reps * n_sample / el, unit="trajectories/s", cores=threads, kind="port",
                 sample=f"{reps} x {n_sample} trajectories of the same workload (first {n_sample} inputs), "
-                       f"oracle/deg_oracle.c Float32 restatement, OpenMP {threads} threads, {el:.1f} s")
+                       f"oracle/deg_oracle.c Float32 restatement, OpenMP {threads} threads, {el:.1f} s",
+                nproc=threads, online_cpus=nproc, cpu_model=model,
+                fp64_oracle_error=oracle_fp64_error(name, cfg, pyoracle) if cfg["N"] >= 1000 else None)
 
 
 def end_to_end(deg, torch, f, cfg, prob, alg, dev, kw, reps=3):
@@ -205,12 +252,12 @@ def jit_compile_seconds(deg, f, n, np_, alg, dev):
             L.deg_set_cache_dir(_lib.CACHE_DIR.encode())
 
 
-def learned_leg(deg, torch, name, f, cfg, prob, alg, dev, kw, every, steps=10):
+def learned_leg(deg, torch, name, f, cfg, prob, alg, dev, kw, every, steps=10, train_n=50_000):
     """The same workload with the learned step-size proposer: logs from a held-out ensemble
     (different seed) -> GPU training -> the learned kernel. Reported beside the PI value."""
     from diffeqgpu_amd import learned, workloads
 
-    train = workloads.config(name, N=min(cfg["N"], 50_000), seed=9001)
+    train = workloads.config(name, N=min(cfg["N"], train_n), seed=9001)
     tprobs = deg.HIPProbVector.from_arrays(f, train["u0"], train["p"], train["tspan"], dev)
     t0 = time.perf_counter()
     rows, _ = learned.collect_steplog(tprobs, prob, alg, 12_000_000, **kw)
@@ -234,6 +281,47 @@ def learned_leg(deg, torch, name, f, cfg, prob, alg, dev, kw, every, steps=10):
                 train_s=train_s, rho=0.2, mlp_every=every,
                 fp64_oracle_error=fp64_error(name, res.us_soa[:, :, :1000].cpu().numpy()) if cfg["N"] >= 1000 else None,
                 note="value = trajectories / median kernel time (HIP events), same inputs and tolerances as the PI line")
+
+
+def learned_vs_pi(deg, torch, name, every, steps=8):
+    """A second learned-vs-PI comparison on its own workload (the 2M Van der Pol ensemble): the PI
+    kernel in its tuned configuration and the learned kernel on the same inputs and tolerances,
+    with attempts, rejections and the FP64 error of both on the fixture's 1,000 trajectories."""
+    from diffeqgpu_amd import systems, workloads
+
+    cfg = workloads.config(name)
+    f, n, np_, _ = systems.SYSTEMS[cfg["system"]]
+    alg = getattr(deg, WORKLOADS[name]["alg"])()
+    prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
+    kw = solve_kw(cfg)
+    dev = deg.HIPDevice(0, contract="fast", powmode="fast", persistent=True)
+    probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev)
+    dev = deg.autotune.tune(probs, prob, alg, **kw)
+    probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev)
+    res = deg.vectorized_asolve(probs, prob, alg, **kw)
+    stream = torch.cuda.current_stream()
+    for _ in range(2):
+        res.relaunch()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a_, b_ in evs:
+        a_.record(stream)
+        res.relaunch()
+        b_.record(stream)
+    torch.cuda.synchronize()
+    pi_ms = float(np.median([a_.elapsed_time(b_) for a_, b_ in evs]))
+    att = int((res.naccept.to(torch.int64) + res.nreject.to(torch.int64)).sum().item())
+    pi = dict(value=cfg["N"] / (pi_ms / 1e3), kernel_ms=pi_ms, attempts=att,
+              rejections=int(res.nreject.sum().item()),
+              retcodes=np.bincount(res.retcode.cpu().numpy(), minlength=4).tolist(),
+              fp64_oracle_error=fp64_error(name, res.us_soa[:, :, :1000].cpu().numpy()),
+              launch=dict(block_size=dev.block_size, waves=dev.waves, refill=dev.refill, chunk=dev.chunk,
+                          start_order=dev.start_order))
+    del res, probs
+    lrn = learned_leg(deg, torch, name, f, cfg, prob, alg, dev, kw, every, steps=steps, train_n=200_000)
+    lrn["speedup_vs_pi"] = pi_ms / lrn["kernel_ms"]
+    lrn["attempts_vs_pi"] = lrn["attempts"] / att
+    return dict(workload=WORKLOADS[name]["desc"], trajectories=cfg["N"], pi=pi, learned=lrn)
 
 
 def fp64_error(name, us):
@@ -346,7 +434,7 @@ def main_sde(args, name):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
 
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_cpu()[0]
         ns = min(N, spec["cpu_sample"])
         meth = "em" if spec["alg"] == "GPUEM" else "siea"
         reps, c0 = 0, time.perf_counter()
@@ -357,9 +445,11 @@ def main_sde(args, name):
             el = time.perf_counter() - c0
             if el > 10.0 or reps >= 50:
                 break
+        _, nproc, model = host_cpu()
         out["cpu_baseline"] = dict(value=reps * ns / el, unit="trajectories/s", cores=threads, kind="port",
                                    sample=f"{reps} x {ns} trajectories of the same workload, oracle/deg_oracle_sde.c "
-                                          f"Float32 restatement, OpenMP {threads} threads, {el:.1f} s")
+                                          f"Float32 restatement, OpenMP {threads} threads, {el:.1f} s",
+                                   nproc=threads, online_cpus=nproc, cpu_model=model)
     print(json.dumps(out))
 
 
@@ -507,9 +597,16 @@ def main():
         lrn["speedup_vs_pi"] = pi_ms / lrn["kernel_ms"]
         lrn["attempts_vs_pi"] = lrn["attempts"] / attempts
         out["learned"] = lrn
+    if not args.no_learned and name == "lorenz_adaptive":
+        out["learned_c5"] = learned_vs_pi(deg, torch, "vdp", args.mlp_every)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_cpu()[0]
         out["cpu_baseline"] = cpu_baseline(name, cfg, threads, spec["cpu_sample"])
+        if err is not None and out["cpu_baseline"].get("fp64_oracle_error"):
+            ref_err = out["cpu_baseline"]["fp64_oracle_error"]
+            out["fp64_oracle_error"]["reference_algorithm"] = {
+                "median": ref_err["median"], "p99": ref_err["p99"], "max": ref_err["max"],
+                "what": "the CPU restatement of the reference's Float32 algorithm on the same 1,000 trajectories"}
     print(json.dumps(out))
 
 

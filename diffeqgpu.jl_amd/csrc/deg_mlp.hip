@@ -12,7 +12,8 @@
 //   deg_mlp_grad   one wavefront per block, 64-row tiles staged in LDS; every thread owns a
 //                  fixed set of parameters and accumulates their gradient over the block's
 //                  rows; one partial gradient row per block (deterministic, no atomics)
-//   deg_mlp_adam   sums the partial rows in block order and applies one Adam step
+//   deg_mlp_adam   sums the partial rows (four interleaved partial sums per column, added in a
+//                  fixed order) and applies one Adam step
 //   deg_mlp_forward  FP32 forward pass (evaluation)
 //   deg_mlp_pack   FP32 master parameters -> BF16 MFMA fragments (struct deg_mlp_packed)
 //   deg_mlp_infer  the solver's MFMA evaluation (deg_mlp_eval) over an array of feature rows
@@ -172,26 +173,48 @@ __global__ void __launch_bounds__(64) deg_mlp_grad_kernel(const float* __restric
     }
 }
 
+// One block per 64 columns of the partial rows (parameters, then the loss sum and the row count):
+// each of the block's four wavefronts sums every fourth partial row of its columns, then the four
+// sums are added in a fixed order. Deterministic for a given nblocks, and the loads of one column
+// are independent of each other (no serial chain of nblocks dependent loads).
+constexpr int kAdamSeg = 4;
 __global__ void __launch_bounds__(256) deg_mlp_adam_kernel(float* __restrict__ P, float* __restrict__ M,
                                                            float* __restrict__ V, const float* __restrict__ partial,
                                                            int nblocks, float lr, float beta1, float beta2, float eps,
                                                            float bc1, float bc2, float* __restrict__ loss_out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < kNP) {
-        float g = 0.0f;
-        for (int b = 0; b < nblocks; ++b) g += partial[(long long)b * kPartial + p];
-        const float m = beta1 * M[p] + (1.0f - beta1) * g;
-        const float v = beta2 * V[p] + (1.0f - beta2) * g * g;
-        M[p] = m;
-        V[p] = v;
-        const float mh = m / bc1, vh = v / bc2;
-        P[p] -= lr * mh / (__builtin_sqrtf(vh) + eps);
-    } else if (p == kNP && loss_out) {
-        float s = 0.0f, c = 0.0f;
-        for (int b = 0; b < nblocks; ++b) {
-            s += partial[(long long)b * kPartial + kNP];
-            c += partial[(long long)b * kPartial + kNP + 1];
+    __shared__ float sg[kAdamSeg][64];
+    __shared__ float tot[64];
+    const int pl = threadIdx.x & 63, seg = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + pl;
+    float g = 0.0f;
+    if (p < kPartial) {
+        int b = seg;
+        for (; b + 3 * kAdamSeg < nblocks; b += 4 * kAdamSeg) {
+            const float a0 = partial[(long long)b * kPartial + p];
+            const float a1 = partial[(long long)(b + kAdamSeg) * kPartial + p];
+            const float a2 = partial[(long long)(b + 2 * kAdamSeg) * kPartial + p];
+            const float a3 = partial[(long long)(b + 3 * kAdamSeg) * kPartial + p];
+            g = (((g + a0) + a1) + a2) + a3;
         }
+        for (; b < nblocks; b += kAdamSeg) g += partial[(long long)b * kPartial + p];
+    }
+    sg[seg][pl] = g;
+    __syncthreads();
+    if (seg == 0) {
+        const float gs = ((sg[0][pl] + sg[1][pl]) + sg[2][pl]) + sg[3][pl];
+        tot[pl] = gs;
+        if (p < kNP) {
+            const float m = beta1 * M[p] + (1.0f - beta1) * gs;
+            const float v = beta2 * V[p] + (1.0f - beta2) * gs * gs;
+            M[p] = m;
+            V[p] = v;
+            const float mh = m / bc1, vh = v / bc2;
+            P[p] -= lr * mh / (__builtin_sqrtf(vh) + eps);
+        }
+    }
+    __syncthreads();
+    if (seg == 0 && p == kNP && loss_out) {
+        const float s = tot[pl], c = tot[pl + 1];  // kNP and kNP + 1 fall in the same block
         loss_out[0] = c > 0.0f ? s / c : 0.0f;
     }
 }
@@ -319,7 +342,7 @@ int deg_mlp_adam(float* params, float* m, float* v, const float* partial, int32_
         b1t *= beta1;
         b2t *= beta2;
     }
-    hipLaunchKernelGGL(deg_mlp_adam_kernel, dim3((kNP + 1 + 255) / 256), dim3(256), 0, (hipStream_t)stream, params, m, v,
+    hipLaunchKernelGGL(deg_mlp_adam_kernel, dim3((kPartial + 63) / 64), dim3(256), 0, (hipStream_t)stream, params, m, v,
                        partial, (int)nblocks, lr, beta1, beta2, eps, (float)(1.0 - b1t), (float)(1.0 - b2t), loss_out);
     return launch_status("deg_mlp_adam launch");
 }

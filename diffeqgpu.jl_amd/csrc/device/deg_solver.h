@@ -135,9 +135,11 @@ __device__ __forceinline__ float rms_norm(const float (&x)[DEG_N]) {
 #if DEG_DUAL_P
 // Dual-number state (DEG_DUAL_NV components, each with DEG_DUAL_P partials; the values are
 // entries 0..NV-1 and component i's partials are NV + i P .. NV + i P + P - 1). The error
-// scaling e ./ (abstol .+ max.(abs.(uprev), abs.(u)) * reltol) is formed in dual arithmetic:
-// abs negates the partials of a negative value, max picks by value (u only when its value is
-// larger, as jl_max), x * reltol scales the partials, e / s takes partials e' (1 / s) + s' (-(e / s^2)).
+// scaling e ./ (abstol .+ max.(abs.(uprev), abs.(u)) * reltol) is formed in dual arithmetic with
+// DiffRules' derivative rules (ForwardDiff's source of them): abs negates the partials where the
+// value's sign bit is set (-0.0 included), max(x, y) takes x's partials only when x > y and y's
+// otherwise (a tie or a NaN takes |u|'s), x * reltol scales the partials, e / s takes partials
+// e' (1 / s) + s' (-(e / s^2)).
 // The squared norm sums value^2 + (partial_1^2 + ... + partial_P^2) over the components, over
 // NV (1 + P) entries in all (ODE_DEFAULT_NORM of a dual array). rcp: reciprocal multiply instead
 // of the IEEE division of the value.
@@ -150,9 +152,9 @@ __device__ __forceinline__ float dual_scaled_error2(const float (&e)[DEG_N], con
         const float av = uprev[i], bv = u[i];
         const float aa = __builtin_fabsf(av), ba = __builtin_fabsf(bv);
         const bool nan = (aa != aa) || (ba != ba);
-        const bool pick_u = !nan && (ba > aa);
+        const bool pick_u = !(aa > ba);
         const float mv = nan ? aa + ba : (pick_u ? ba : aa);
-        const bool neg = pick_u ? (bv < 0.0f) : (av < 0.0f);
+        const bool neg = pick_u ? __builtin_signbit(bv) : __builtin_signbit(av);
         const float sv = abstol + mv * reltol;
         const float ev = e[i];
         const float qv = rcp ? ev * __builtin_amdgcn_rcpf(sv) : ev / sv;

@@ -456,8 +456,8 @@ class DualNode:
     def unary(self, name: str) -> "DualNode":
         g, x = self.g, self.v
         if name == "abs":
-            zero = g.const(0.0)
-            return DualNode(g.mk("abs", (x,)), [g.mk("select", (x, zero, -a, a), "<") for a in self.d])
+            # DiffRules: abs'(x) = signbit(x) ? -1 : 1 (so -0.0 negates the partials too)
+            return DualNode(g.mk("abs", (x,)), [g.mk("sbsel", (x, -a, a)) for a in self.d])
         fx = g.mk(name, (x,))
         if name == "sin":
             fp = g.mk("cos", (x,))
@@ -483,23 +483,48 @@ def _as_dual(g: "Graph", x, P: int) -> DualNode:
     return DualNode(g.lift(x), [zero] * P)
 
 
+def _all_zero(g: "Graph", parts) -> Node:
+    """A boolean node: every partial is zero (ForwardDiff's iszero(partials) / isconstant)."""
+    zero = g.const(0.0)
+    acc = None
+    for d in parts:
+        c = g.mk("cmp", (d, zero), "==")
+        acc = c if acc is None else g.mk("and", (acc, c))
+    return acc if acc is not None else g.lift_bool(True)
+
+
 def _dual_pow(x, y) -> DualNode:
-    """x^y with a dual base and/or exponent: (xv^yv, (yv xv^(yv - 1)) xd + (xv^yv log xv) yd)."""
+    """x^y with a dual base and/or exponent, by ForwardDiff's rules for ^ (ForwardDiff 1.3.3, not
+    vendored in the reference), whose guards keep log(x) out where it would give NaN:
+      dual ^ real:  partials 0 if y == 0 or x's partials are all 0, else (x' y) x^(y - 1);
+      real ^ dual:  partials (x == 0 and y > 0 ? 0 : x^y log x) y';
+      dual ^ dual:  x' (y x^(y - 1)) + y' L with L = 1 if y's partials are all 0, else 0 if x == 0 and
+                    y > 0, else x^y log x."""
     ref = x if isinstance(x, DualNode) else y
     g, P = ref.g, ref.P
+    zero = g.const(0.0)
     if isinstance(x, DualNode) and not isinstance(y, DualNode):
+        if isinstance(y, (int, float)) and not isinstance(y, bool) and y == 0:
+            return DualNode(g.mk("pow", (x.v, g.lift(y))), [zero] * P)
         yv = g.lift(y)
-        fp = yv * g.mk("pow", (x.v, yv - 1.0))
-        return DualNode(g.mk("pow", (x.v, yv)), [fp * a for a in x.d])
+        pw = g.mk("pow", (x.v, yv - 1.0))
+        const_x = _all_zero(g, x.d)
+        ynz = g.mk("cmp", (yv, zero), "==")
+        dead = g.mk("or", (ynz, const_x))
+        return DualNode(g.mk("pow", (x.v, yv)), [g.mk("bselect", (dead, zero, (a * yv) * pw)) for a in x.d])
     if not isinstance(x, DualNode):
         xv = g.lift(x)
         val = g.mk("pow", (xv, y.v))
-        fp = val * g.mk("log", (xv,))
+        lg = val * g.mk("log", (xv,))
+        guard = g.mk("and", (g.mk("cmp", (xv, zero), "=="), g.mk("cmp", (y.v, zero), ">")))
+        fp = g.mk("bselect", (guard, zero, lg))
         return DualNode(val, [fp * b for b in y.d])
     val = g.mk("pow", (x.v, y.v))
     fa = y.v * g.mk("pow", (x.v, y.v - 1.0))
-    fb = val * g.mk("log", (x.v,))
-    return DualNode(val, [fa * a + fb * b for a, b in zip(x.d, y.d)])
+    lg = val * g.mk("log", (x.v,))
+    guard = g.mk("and", (g.mk("cmp", (x.v, zero), "=="), g.mk("cmp", (y.v, zero), ">")))
+    fb = g.mk("bselect", (_all_zero(g, y.d), g.const(1.0), g.mk("bselect", (guard, zero, lg))))
+    return DualNode(val, [a * fa + b * fb for a, b in zip(x.d, y.d)])
 
 
 def trace_dual(f: Callable, n: int, np_: int, P: int) -> "Trace":
@@ -663,18 +688,26 @@ class _Diff:
             da = self.d(a[0])
             return None if da is None else (g.const(1.0) - x * x) * da
         if op == "abs":
+            # DiffRules: abs'(x) = signbit(x) ? -1 : 1
             da = self.d(a[0])
             if da is None:
                 return None
-            return g.mk("select", (a[0], g.const(0.0), da, -da), ">=")
+            return g.mk("sbsel", (a[0], -da, da))
         if op == "pow":
+            # ForwardDiff's ^ rules with their guards: the base term vanishes for a zero exponent;
+            # the log term is 0 where the base is 0 and the exponent positive
             base, ex = a
             db, de = self.d(base), self.d(ex)
+            zero = g.const(0.0)
             r = None
             if db is not None:
-                r = (ex * g.mk("pow", (base, ex - g.const(1.0)))) * db
+                bt = (ex * g.mk("pow", (base, ex - g.const(1.0)))) * db
+                if de is None:
+                    bt = g.mk("bselect", (g.mk("cmp", (ex, zero), "=="), zero, bt))
+                r = bt
             if de is not None:
-                r = self.add(r, (x * g.mk("log", (base,))) * de)
+                guard = g.mk("and", (g.mk("cmp", (base, zero), "=="), g.mk("cmp", (ex, zero), ">")))
+                r = self.add(r, g.mk("bselect", (guard, zero, x * g.mk("log", (base,)))) * de)
             return r
         if op == "select":
             da, db = self.d(a[2]), self.d(a[3])
@@ -786,6 +819,8 @@ def _emit_values(g: Graph, outs: List[Node], indent="    "):
             expr = f"({a[0]} {x.val} {a[1]}) ? {a[2]} : {a[3]}"
         elif op == "bselect":
             expr = f"{a[0]} ? {a[1]} : {a[2]}"
+        elif op == "sbsel":
+            expr = f"__builtin_signbit({a[0]}) ? {a[1]} : {a[2]}"
         else:  # pragma: no cover
             raise TraceError(f"unknown op {op}")
         v = f"v{x.idx}"

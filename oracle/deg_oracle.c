@@ -292,9 +292,11 @@ static void erk_attempt(const erk32 *T, const osys *S, const float *p, const flo
 
 /* The same error norm over a dual-number state (S->dualnv values with S->dualp partials each,
  * component i's partials at dualnv + i dualp + j): the scaling e ./ (abstol .+ max.(|uprev|,|u|) *
- * reltol) in dual arithmetic (abs negates the partials of a negative value; max picks by value, u only
- * when larger; e / s has partials e' (1 / s) + s' (-(e / s^2))), then the root mean square over all
- * values and partials, summed as value^2 + (partial_1^2 + ... + partial_P^2) per component. */
+ * reltol) in dual arithmetic with the derivative rules of DiffRules (a ForwardDiff 1.3.3 dependency,
+ * not vendored in the reference): abs negates the partials where signbit(value) (so -0.0 negates too),
+ * max(x, y) takes x's partials only when x > y and y's otherwise (ties and NaN take y = |u|); e / s
+ * has partials e' (1 / s) + s' (-(e / s^2)) (ForwardDiff's _div_partials); then the root mean square
+ * over all values and partials, summed as value^2 + (partial_1^2 + ... + partial_P^2) per component. */
 static float dual_eest(int nv, int P, int n, const float *e, const float *uprev, const float *u, float abstol,
                        float reltol) {
     float acc = 0.0f;
@@ -302,9 +304,9 @@ static float dual_eest(int nv, int P, int n, const float *e, const float *uprev,
         const float av = uprev[i], bv = u[i];
         const float aa = fabsf(av), ba = fabsf(bv);
         const int nan = (aa != aa) || (ba != ba);
-        const int pick_u = !nan && (ba > aa);
+        const int pick_u = !(aa > ba);
         const float mv = nan ? aa + ba : (pick_u ? ba : aa);
-        const int neg = pick_u ? (bv < 0.0f) : (av < 0.0f);
+        const int neg = pick_u ? (signbit(bv) != 0) : (signbit(av) != 0);
         const float sv = abstol + mv * reltol;
         const float ev = e[i];
         const float qv = ev / sv;
@@ -1649,6 +1651,12 @@ int deg_oracle_vern9(int system, int64_t N, const float *u0, const float *p, con
 /* Introspection used by the order-condition and helper tests                           */
 /* ------------------------------------------------------------------------------------ */
 float deg_oracle_det_pow(float x, float y) { return det_pow(x, y); }
+
+/* the dual error norm on one hand-made state (host checks of its abs and max rules) */
+float deg_oracle_dual_eest(int nv, int P, int n, const float *e, const float *uprev, const float *u, float abstol,
+                           float reltol) {
+    return dual_eest(nv, P, n, e, uprev, u, abstol, reltol);
+}
 
 /* which: 0 Tsit5, 1 Vern7, 2 Vern9. c[18], A[18*18], b[18], bt[18] in double. Returns s. */
 int deg_oracle_tableau(int which, double *c, double *A, double *b, double *bt) {

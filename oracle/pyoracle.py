@@ -101,6 +101,9 @@ def lib():
         L.deg_oracle_jac.restype = ctypes.c_int
         L.deg_oracle_det_pow.argtypes = [ctypes.c_float, ctypes.c_float]
         L.deg_oracle_det_pow.restype = ctypes.c_float
+        L.deg_oracle_dual_eest.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp, _fp, _fp, ctypes.c_float,
+                                           ctypes.c_float]
+        L.deg_oracle_dual_eest.restype = ctypes.c_float
         L.deg_oracle_tableau.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
         L.deg_oracle_tableau.restype = ctypes.c_int
         L.deg_oracle_tsit5_btheta.argtypes = [ctypes.c_float, _fp]
@@ -211,6 +214,12 @@ def vern9(system: str, u0, p, tspan, *, abstol=1e-12, reltol=1e-12, saveat=None,
 
 def det_pow(x: float, y: float) -> float:
     return float(lib().deg_oracle_det_pow(float(x), float(y)))
+
+
+def dual_eest(nv: int, P: int, n: int, e, uprev, u, abstol: float, reltol: float) -> float:
+    """The oracle's dual error norm (EEst) of one state."""
+    arrs = [np.ascontiguousarray(x, dtype=np.float32) for x in (e, uprev, u)]
+    return float(lib().deg_oracle_dual_eest(nv, P, n, *(_p(a, _fp) for a in arrs), float(abstol), float(reltol)))
 
 
 def tableau(which: str):

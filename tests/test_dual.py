@@ -189,3 +189,70 @@ def test_dual_rules_against_finite_differences(point):
         fd = (np.array(tracer.evaluate(_elementary, xp[:3], xp[3:], t))
               - np.array(tracer.evaluate(_elementary, xm[:3], xm[3:], t))) / (2 * h)
         np.testing.assert_allclose(parts[:, j], fd, rtol=1e-6, atol=1e-7)
+
+
+def _hill(u, p, t):
+    """A Hill feedback term and a power with a dual exponent: u' = -u + u^p / (1 + u^p)."""
+    from diffeqgpu_amd import math as m
+    from diffeqgpu_amd.tracer import SVector
+
+    h = m.pow(u[0], p[0])
+    return SVector(-u[0] + h / (1.0 + h))
+
+
+@pytest.mark.parametrize("uv,pv,seed,want", [
+    (0.0, 2.0, (1.0, 0.0), -1.0),        # zero base, exponent with zero partials: d/du = -1 (not NaN)
+    (0.0, 2.0, (0.0, 1.0), 0.0),         # zero base, positive dual exponent: the log term is 0
+    (-1.5, 3.0, (1.0, 0.0), None),       # negative base, integer-valued exponent with zero partials
+])
+def test_dual_pow_guards(uv, pv, seed, want):
+    """ForwardDiff's guards of the power rule: where x = 0 (exponent > 0) or where the exponent's
+    partials are all zero, the log(x) term must not turn the partials into NaN. Checked against
+    the derivative by hand and against a central difference of the plain function."""
+    from diffeqgpu_amd import tracer
+    from test_host import _eval_nodes
+
+    tr = tracer.trace_dual(_hill, 1, 1, 1)
+    out = _eval_nodes(tr, tr.outs, [uv, seed[0]], [pv, seed[1]], 0.0)
+    assert all(np.isfinite(out)), out
+    x0 = np.array([uv, pv])
+    h = 1e-6
+    d = np.array(seed)
+    fd = (tracer.evaluate(_hill, [uv + h * d[0]], [pv + h * d[1]], 0.0)[0]
+          - tracer.evaluate(_hill, [uv - h * d[0]], [pv - h * d[1]], 0.0)[0]) / (2 * h)
+    np.testing.assert_allclose(out[1], fd, rtol=1e-6, atol=1e-7)
+    if want is not None:
+        assert out[1] == want
+    else:
+        # d/du (u^3 / (1 + u^3)) - 1 at u = -1.5: 3u^2 / (1 + u^3)^2 - 1
+        assert abs(out[1] - (3 * uv ** 2 / (1 + uv ** 3) ** 2 - 1.0)) < 1e-12
+    del x0
+
+
+def test_dual_abs_and_max_tie_rules(oracle):
+    """The dual error norm's abs and max follow DiffRules: abs'(x) = signbit(x) ? -1 : 1 and
+    max(x, y) takes y's partials unless x > y. A state with one constant component ties
+    |uprev| = |u| on every step: the norm then takes u's partials. Checked with the oracle's norm
+    on hand-made states (values and partials chosen so that each rule changes the result)."""
+    nv, P_, n = 1, 1, 2
+    e = np.array([1e-3, 2e-3], F32)
+    # tie |uprev| == |u| = 1 with different partials: the rule takes u's partial (0.5), not uprev's (4)
+    uprev = np.array([1.0, 4.0], F32)
+    u = np.array([1.0, 0.5], F32)
+    got = P.dual_eest(nv, P_, n, e, uprev, u, 0.0, 1.0)
+
+    def norm(md, neg):
+        sv = 1.0 * 1.0
+        qv = F32(e[0]) / F32(sv)
+        sd = F32((-md if neg else md) * 1.0)
+        qd = F32(e[1] * (1.0 / sv) + sd * (-(e[0] / (sv * sv))))
+        return float(np.sqrt(np.float32(qv * qv + qd * qd) / 2))
+
+    assert abs(got - norm(0.5, False)) < 1e-9 and abs(got - norm(4.0, False)) > 1e-6
+    # -0.0 carries the sign bit: its partials are negated (u wins the tie at value 0)
+    got0 = P.dual_eest(nv, P_, n, e, np.array([0.0, 0.3], F32), np.array([-0.0, 0.7], F32), 1e-3, 1.0)
+    sv = 1e-3
+    qv = e[0] / F32(sv)
+    sd = F32(-0.7 * 1.0)
+    qd = F32(e[1] * F32(1.0 / F32(sv)) + sd * F32(-(e[0] / F32(F32(sv) * F32(sv)))))
+    assert abs(got0 - float(np.sqrt(F32(qv * qv + qd * qd) / 2))) < 1e-6 * got0

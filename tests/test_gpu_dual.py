@@ -107,3 +107,24 @@ def test_reference_forward_diff(deg, torch_cuda):
         with pytest.raises(NotImplementedError, match="dual"):
             deg.solve(ep, getattr(deg, name)(autodiff=False), deg.EnsembleGPUKernel(), trajectories=2,
                       save_everystep=False, adaptive=False, dt=0.01)
+
+
+def test_dual_hill_at_zero(deg, torch_cuda):
+    """A Hill feedback u' = -u + u^p / (1 + u^p) from u0 = 0 with p = 2, u0 and p seeded: u stays 0,
+    so d u / d u0 = exp(-t) and d u / d p = 0. The power's partials at a zero base go through
+    ForwardDiff's guards (the log term is 0 there); without them they are NaN."""
+    from test_dual import _hill
+
+    D = deg.Dual
+    prob = deg.ODEProblem(_hill, [D(0.0, (1.0, 0.0))], (0.0, 2.0), [D(2.0, (0.0, 1.0))])
+    sv = np.linspace(0.0, 2.0, 5).astype(F32)
+    for name in ("GPUTsit5", "GPUVern7"):
+        print(f"[gpu] dual hill at zero {name}", flush=True)
+        sol = deg.solve(deg.EnsembleProblem(prob), getattr(deg, name)(), deg.EnsembleGPUKernel(), trajectories=3,
+                        adaptive=True, dt=0.01, abstol=1e-8, reltol=1e-8, saveat=sv)
+        for s in sol.u:
+            assert s.retcode == "Success" and s.u.shape == (5, 1, 3)
+            assert np.all(np.isfinite(s.u))
+            np.testing.assert_array_equal(s.u[:, 0, 0], 0.0)
+            np.testing.assert_allclose(s.u[:, 0, 1], np.exp(-sv.astype(np.float64)), rtol=1e-5)
+            np.testing.assert_array_equal(s.u[:, 0, 2], 0.0)

@@ -90,6 +90,25 @@ def test_tracer_keeps_association_order(deg):
     assert "u[0] + u[1]" in src2 and "+ p[0]" in src2
 
 
+def _pow64(x, y):
+    """powf in float64: a negative base with a non-integer exponent is NaN, as in C."""
+    if x < 0 and y != math.floor(y):
+        return math.nan
+    try:
+        return float(x) ** float(y)
+    except ZeroDivisionError:
+        return math.inf
+
+
+def _log64(x):
+    """log in float64 with C's special values (log 0 = -inf, log of a negative number = NaN)."""
+    if x == 0:
+        return -math.inf
+    if x < 0 or x != x:
+        return math.nan
+    return math.log(x)
+
+
 def _eval_nodes(tr, outs, u, p, t):
     """Interpret a trace in float64 (host check of emitted expressions)."""
     vals = {}
@@ -107,10 +126,14 @@ def _eval_nodes(tr, outs, u, p, t):
             a = [vals[y.idx] for y in x.args]
             v = {"+": lambda: a[0] + a[1], "-": lambda: a[0] - a[1], "*": lambda: a[0] * a[1],
                  "/": lambda: a[0] / a[1], "neg": lambda: -a[0], "sin": lambda: math.sin(a[0]),
-                 "cos": lambda: math.cos(a[0]), "exp": lambda: math.exp(a[0]), "log": lambda: math.log(a[0]),
+                 "cos": lambda: math.cos(a[0]), "exp": lambda: math.exp(a[0]), "log": lambda: _log64(a[0]),
                  "sqrt": lambda: math.sqrt(a[0]), "tanh": lambda: math.tanh(a[0]), "abs": lambda: abs(a[0]),
-                 "pow": lambda: a[0] ** a[1],
-                 "select": lambda: a[2] if eval(f"{a[0]} {x.val} {a[1]}") else a[3]}[op]()
+                 "pow": lambda: _pow64(a[0], a[1]),
+                 "select": lambda: a[2] if eval(f"{a[0]} {x.val} {a[1]}") else a[3],
+                 "cmp": lambda: bool(eval(f"{a[0]} {x.val} {a[1]}")),
+                 "and": lambda: a[0] and a[1], "or": lambda: a[0] or a[1], "not": lambda: not a[0],
+                 "bconst": lambda: bool(x.val), "bselect": lambda: a[1] if a[0] else a[2],
+                 "sbsel": lambda: a[1] if math.copysign(1.0, a[0]) < 0 else a[2]}[op]()
         vals[x.idx] = v
     return [vals[o.idx] for o in outs]
 
