@@ -131,6 +131,13 @@ __device__ __forceinline__ float rms_norm(const float (&x)[DEG_N]) {
 #ifndef DEG_DUAL_P
 #define DEG_DUAL_P 0
 #endif
+// columns of the Jacobian deg_jac writes (J[r * DEG_JC + c], DEG_N rows): the values of a dual-number
+// state (rows DEG_JC + r P + j then hold partial j of row r), else every state component
+#if DEG_DUAL_P
+#define DEG_JC DEG_DUAL_NV
+#else
+#define DEG_JC DEG_N
+#endif
 
 #if DEG_DUAL_P
 // Dual-number state (DEG_DUAL_NV components, each with DEG_DUAL_P partials; the values are
@@ -536,6 +543,7 @@ __device__ __noinline__ void deg_interp_ool(float th, float dt, float tprev, con
 #ifndef DEG_MASS
 #define DEG_MASS 0
 #endif
+static_assert(!(DEG_MASS && DEG_DUAL_P), "dual-number states take no mass matrix");
 #if DEG_METHOD == 2 || DEG_METHOD == 4 || DEG_METHOD == 5 || DEG_METHOD == 6 || DEG_METHOD == 7
 #if DEG_MASS
 // The constant mass matrix M of the traced source (DEG_MASS_ENTRIES, row-major): out = M v, each
@@ -553,25 +561,49 @@ __device__ __forceinline__ void deg_mass_mul(const float (&v)[DEG_N], float (&ou
     }
 }
 #endif
+// The factor of W: LU with partial pivoting of W (FP64), or over a dual-number state of W's values
+// (DEG_JC x DEG_JC) together with what the partials of W need: W = alpha M + beta J, so partial j
+// of W is beta times partial j of J, the rows DEG_JC + r P + j of the dual Jacobian deg_jac writes.
+struct deg_wf {
+    double W[DEG_JC][DEG_JC];
+    int piv[DEG_JC];
+#if DEG_DUAL_P
+    double beta;
+    float jp[DEG_N - DEG_JC][DEG_JC];
+#endif
+};
+// Record the dual part of a factor from the dual Jacobian J (DEG_N rows of DEG_JC columns)
+__device__ __forceinline__ void w_dual(deg_wf& F, double beta, const float (&J)[DEG_N * DEG_JC]) {
+#if DEG_DUAL_P
+    F.beta = beta;
+    _Pragma("unroll") for (int r = DEG_JC; r < DEG_N; ++r)
+        _Pragma("unroll") for (int c = 0; c < DEG_JC; ++c) F.jp[r - DEG_JC][c] = J[r * DEG_JC + c];
+#else
+    (void)F;
+    (void)beta;
+    (void)J;
+#endif
+}
 // LU with partial pivoting; row swaps are applied with compile-time indices so W stays
 // in VGPRs. With DEG_PARITY every multiplier and back-substitution step divides by its pivot in
 // IEEE double, as the oracle does; otherwise each pivot is inverted once and the diagonal of the
-// factor holds 1 / U_kk, so a factorisation costs n FP64 divisions and a solve none.
-__device__ __forceinline__ void w_lu(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N]) {
-    _Pragma("unroll") for (int kk = 0; kk < DEG_N; ++kk) {
+// factor holds 1 / U_kk, so a factorisation costs DEG_JC FP64 divisions and a solve none.
+__device__ __forceinline__ void w_lu(deg_wf& F) {
+    auto& W = F.W;
+    _Pragma("unroll") for (int kk = 0; kk < DEG_JC; ++kk) {
         int pk = kk;
         double mx = __builtin_fabs(W[kk][kk]);
-        _Pragma("unroll") for (int i = kk + 1; i < DEG_N; ++i) {
+        _Pragma("unroll") for (int i = kk + 1; i < DEG_JC; ++i) {
             double v = __builtin_fabs(W[i][kk]);
             if (v > mx) {
                 mx = v;
                 pk = i;
             }
         }
-        piv[kk] = pk;
-        _Pragma("unroll") for (int i = kk + 1; i < DEG_N; ++i) {
+        F.piv[kk] = pk;
+        _Pragma("unroll") for (int i = kk + 1; i < DEG_JC; ++i) {
             if (pk == i) {
-                _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) {
+                _Pragma("unroll") for (int c = 0; c < DEG_JC; ++c) {
                     double tv = W[kk][c];
                     W[kk][c] = W[i][c];
                     W[i][c] = tv;
@@ -581,48 +613,68 @@ __device__ __forceinline__ void w_lu(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N
 #if !DEG_PARITY
         const double rp = 1.0 / W[kk][kk];
 #endif
-        _Pragma("unroll") for (int i = kk + 1; i < DEG_N; ++i) {
+        _Pragma("unroll") for (int i = kk + 1; i < DEG_JC; ++i) {
 #if DEG_PARITY
             double l = W[i][kk] / W[kk][kk];
 #else
             double l = W[i][kk] * rp;
 #endif
             W[i][kk] = l;
-            _Pragma("unroll") for (int c = kk + 1; c < DEG_N; ++c) W[i][c] = W[i][c] - l * W[kk][c];
+            _Pragma("unroll") for (int c = kk + 1; c < DEG_JC; ++c) W[i][c] = W[i][c] - l * W[kk][c];
         }
 #if !DEG_PARITY
         W[kk][kk] = rp;
 #endif
     }
 }
-__device__ __forceinline__ void w_solve(const double (&W)[DEG_N][DEG_N], const int (&piv)[DEG_N],
-                                        const float (&b)[DEG_N], float (&x)[DEG_N]) {
-    double y[DEG_N];
-    DEG_FOR(i) y[i] = (double)b[i];
-    _Pragma("unroll") for (int kk = 0; kk < DEG_N; ++kk) {
-        _Pragma("unroll") for (int i = kk + 1; i < DEG_N; ++i) {
-            if (piv[kk] == i) {
+// y = LU \ y in FP64
+__device__ __forceinline__ void lu_solve_d(const deg_wf& F, double (&y)[DEG_JC]) {
+    const auto& W = F.W;
+    _Pragma("unroll") for (int kk = 0; kk < DEG_JC; ++kk) {
+        _Pragma("unroll") for (int i = kk + 1; i < DEG_JC; ++i) {
+            if (F.piv[kk] == i) {
                 double tv = y[kk];
                 y[kk] = y[i];
                 y[i] = tv;
             }
         }
     }
-    DEG_FOR(i) {
+    _Pragma("unroll") for (int i = 0; i < DEG_JC; ++i) {
         double v = y[i];
         _Pragma("unroll") for (int j = 0; j < i; ++j) v = v - W[i][j] * y[j];
         y[i] = v;
     }
-    _Pragma("unroll") for (int i = DEG_N - 1; i >= 0; --i) {
+    _Pragma("unroll") for (int i = DEG_JC - 1; i >= 0; --i) {
         double v = y[i];
-        _Pragma("unroll") for (int j = i + 1; j < DEG_N; ++j) v = v - W[i][j] * y[j];
+        _Pragma("unroll") for (int j = i + 1; j < DEG_JC; ++j) v = v - W[i][j] * y[j];
 #if DEG_PARITY
         y[i] = v / W[i][i];
 #else
         y[i] = v * W[i][i];  // the factor keeps 1 / U_ii
 #endif
     }
-    DEG_FOR(i) x[i] = (float)y[i];
+}
+// x = W \ b. Over a dual-number state the solve is the dual one, the value LU plus one solve per
+// partial: x_v = W_v \ b_v, x_j = W_v \ (b_j - W_j x_v) with W_j = beta J_j and x_v kept in FP64
+// for the product (the derivative of W^-1 b is W^-1 (b' - W' W^-1 b)).
+__device__ __forceinline__ void w_solve(const deg_wf& F, const float (&b)[DEG_N], float (&x)[DEG_N]) {
+    double y[DEG_JC];
+    _Pragma("unroll") for (int i = 0; i < DEG_JC; ++i) y[i] = (double)b[i];
+    lu_solve_d(F, y);
+    _Pragma("unroll") for (int i = 0; i < DEG_JC; ++i) x[i] = (float)y[i];
+#if DEG_DUAL_P
+    _Pragma("unroll") for (int j = 0; j < DEG_DUAL_P; ++j) {
+        double z[DEG_JC];
+        _Pragma("unroll") for (int r = 0; r < DEG_JC; ++r) {
+            const int q = DEG_JC + r * DEG_DUAL_P + j;
+            double v = (double)b[q];
+            _Pragma("unroll") for (int c = 0; c < DEG_JC; ++c) v = v - (F.beta * (double)F.jp[q - DEG_JC][c]) * y[c];
+            z[r] = v;
+        }
+        lu_solve_d(F, z);
+        _Pragma("unroll") for (int r = 0; r < DEG_JC; ++r) x[DEG_JC + r * DEG_DUAL_P + j] = (float)z[r];
+    }
+#endif
 }
 #endif
 
@@ -647,17 +699,17 @@ __device__ __forceinline__ void deg_interp(float th, float dt, float tprev, cons
 #if DEG_METHOD == 2
 #define DEG_NK 2
 // W = M - gamma J (M = I without a mass matrix)
-__device__ __forceinline__ void w_factor(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N], float gam,
-                                         const float (&J)[DEG_N * DEG_N]) {
+__device__ __forceinline__ void w_factor(deg_wf& F, float gam, const float (&J)[DEG_N * DEG_JC]) {
     const double g = (double)gam;
-    DEG_FOR(r) {
+    _Pragma("unroll") for (int r = 0; r < DEG_JC; ++r) {
 #if DEG_MASS
-        _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (double)deg_mass_at(r, c) - g * (double)J[r * DEG_N + c];
+        _Pragma("unroll") for (int c = 0; c < DEG_JC; ++c) F.W[r][c] = (double)deg_mass_at(r, c) - g * (double)J[r * DEG_JC + c];
 #else
-        _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (r == c ? 1.0 : 0.0) - g * (double)J[r * DEG_N + c];
+        _Pragma("unroll") for (int c = 0; c < DEG_JC; ++c) F.W[r][c] = (r == c ? 1.0 : 0.0) - g * (double)J[r * DEG_JC + c];
 #endif
     }
-    w_lu(W, piv);
+    w_lu(F);
+    w_dual(F, -g, J);
 }
 #define RB23_D (1.0f / (2.0f + __builtin_sqrtf(2.0f)))
 #define RB23_E32 (6.0f + __builtin_sqrtf(2.0f))
@@ -670,15 +722,14 @@ __device__ __forceinline__ float deg_rb23(const float (&uprev)[DEG_N], const flo
     const float d = RB23_D;
     const float gam = dt * d;
     const float dto2 = dt / 2.0f;
-    float Jm[DEG_N * DEG_N], dT[DEG_N], F0[DEG_N], F1[DEG_N], rhs[DEG_N], tmp[DEG_N], x[DEG_N];
+    float Jm[DEG_N * DEG_JC], dT[DEG_N], F0[DEG_N], F1[DEG_N], rhs[DEG_N], tmp[DEG_N], x[DEG_N];
     deg_jac(uprev, p, t, Jm);
     deg_tgrad(uprev, p, t, dT);
-    double W[DEG_N][DEG_N];
-    int piv[DEG_N];
-    w_factor(W, piv, gam, Jm);
+    deg_wf W;
+    w_factor(W, gam, Jm);
     deg_rhs(uprev, p, t, F0);
     DEG_FOR(c) rhs[c] = F0[c] + gam * dT[c];
-    w_solve(W, piv, rhs, k[1]);
+    w_solve(W, rhs, k[1]);
     DEG_FOR(c) tmp[c] = uprev[c] + dto2 * k[1][c];
     deg_rhs(tmp, p, t + dto2, F1);
 #if DEG_MASS
@@ -687,7 +738,7 @@ __device__ __forceinline__ float deg_rb23(const float (&uprev)[DEG_N], const flo
 #else
     DEG_FOR(c) rhs[c] = F1[c] - k[1][c];
 #endif
-    w_solve(W, piv, rhs, x);
+    w_solve(W, rhs, x);
     DEG_FOR(c) k[2][c] = x[c] + k[1][c];
     DEG_FOR(c) unew[c] = uprev[c] + dt * k[2][c];
     if (!want_error) return 0.0f;
@@ -706,7 +757,7 @@ __device__ __forceinline__ float deg_rb23(const float (&uprev)[DEG_N], const flo
 #else
     DEG_FOR(c) rhs[c] = ((F2[c] - e32 * (k[2][c] - F1[c])) - 2.0f * (k[1][c] - F0[c])) + dt * dT[c];
 #endif
-    w_solve(W, piv, rhs, k3);
+    w_solve(W, rhs, k3);
     DEG_FOR(c) e[c] = dto6 * ((k[1][c] - 2.0f * k[2][c]) + k3[c]);
 #if DEG_FASTPOW
     return scaled_error2_fast(e, uprev, unew, abstol, reltol);
@@ -767,7 +818,7 @@ template <bool want_error>
 __device__ __forceinline__ float deg_rodas(const float (&uprev)[DEG_N], const float* p, float t, float dt,
                                            float (&k)[DEG_NK + 1][DEG_N], float (&unew)[DEG_N], float abstol,
                                            float reltol) {
-    float Jm[DEG_N * DEG_N], dT[DEG_N], du[DEG_N], lt[DEG_N], u[DEG_N];
+    float Jm[DEG_N * DEG_JC], dT[DEG_N], du[DEG_N], lt[DEG_N], u[DEG_N];
     deg_jac(uprev, p, t, Jm);
     deg_tgrad(uprev, p, t, dT);
     const float dtC21 = RO(C21) / dt, dtC31 = RO(C31) / dt, dtC32 = RO(C32) / dt;
@@ -777,52 +828,52 @@ __device__ __forceinline__ float deg_rodas(const float (&uprev)[DEG_N], const fl
     const float dtC65 = RO(C65) / dt;
     const float dtd1 = dt * RO(d1), dtd2 = dt * RO(d2), dtd3 = dt * RO(d3), dtd4 = dt * RO(d4);
     const float dtgamma = dt * RO(gamma);
-    double W[DEG_N][DEG_N];
-    int piv[DEG_N];
+    deg_wf W;
     {
         // W = J - M inv(dt gamma) (gpu_rodas4_perform_step.jl:66-67), the product with M in Float32
         const float igf = 1.0f / dtgamma;
         const double ig = (double)igf;
-        DEG_FOR(r) {
+        _Pragma("unroll") for (int r = 0; r < DEG_JC; ++r) {
 #if DEG_MASS
-            _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (double)Jm[r * DEG_N + c] - (double)(deg_mass_at(r, c) * igf);
+            _Pragma("unroll") for (int c = 0; c < DEG_JC; ++c) W.W[r][c] = (double)Jm[r * DEG_JC + c] - (double)(deg_mass_at(r, c) * igf);
 #else
-            _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (double)Jm[r * DEG_N + c] - (r == c ? ig : 0.0);
+            _Pragma("unroll") for (int c = 0; c < DEG_JC; ++c) W.W[r][c] = (double)Jm[r * DEG_JC + c] - (r == c ? ig : 0.0);
 #endif
         }
-        w_lu(W, piv);
+        w_lu(W);
+        w_dual(W, 1.0, Jm);
     }
     float K1[DEG_N], K2[DEG_N], K3[DEG_N], K4[DEG_N], K5[DEG_N], K6[DEG_N];
     deg_rhs(uprev, p, t, du);
     // stage 1
     DEG_FOR(c) lt[c] = -(du[c] + dtd1 * dT[c]);
-    w_solve(W, piv, lt, K1);
+    w_solve(W, lt, K1);
     DEG_FOR(c) u[c] = uprev[c] + RO(a21) * K1[c];
     deg_rhs(u, p, t + RO(c2) * dt, du);
     // stage 2
     DEG_RODAS_LT((du[c] + dtd2 * dT[c]), dtC21 * K1[c])
-    w_solve(W, piv, lt, K2);
+    w_solve(W, lt, K2);
     DEG_FOR(c) u[c] = (uprev[c] + RO(a31) * K1[c]) + RO(a32) * K2[c];
     deg_rhs(u, p, t + RO(c3) * dt, du);
     // stage 3
     DEG_RODAS_LT((du[c] + dtd3 * dT[c]), (dtC31 * K1[c] + dtC32 * K2[c]))
-    w_solve(W, piv, lt, K3);
+    w_solve(W, lt, K3);
     DEG_FOR(c) u[c] = ((uprev[c] + RO(a41) * K1[c]) + RO(a42) * K2[c]) + RO(a43) * K3[c];
     deg_rhs(u, p, t + RO(c4) * dt, du);
     // stage 4
     DEG_RODAS_LT((du[c] + dtd4 * dT[c]), ((dtC41 * K1[c] + dtC42 * K2[c]) + dtC43 * K3[c]))
-    w_solve(W, piv, lt, K4);
+    w_solve(W, lt, K4);
     DEG_FOR(c) u[c] = (((uprev[c] + RO(a51) * K1[c]) + RO(a52) * K2[c]) + RO(a53) * K3[c]) + RO(a54) * K4[c];
 #if DEG_METHOD == 4
     deg_rhs(u, p, t + dt, du);
     // stage 5 (its C terms in the reference's order 2, 4, 1, 3)
     DEG_RODAS_LT(du[c], (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]))
-    w_solve(W, piv, lt, K5);
+    w_solve(W, lt, K5);
     DEG_FOR(c) u[c] = u[c] + K5[c];
     deg_rhs(u, p, t + dt, du);
     // stage 6 (order 1, 2, 5, 4, 3)
     DEG_RODAS_LT(du[c], ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC65 * K5[c]) + dtC64 * K4[c]) + dtC63 * K3[c]))
-    w_solve(W, piv, lt, K6);
+    w_solve(W, lt, K6);
     DEG_FOR(c) unew[c] = u[c] + K6[c];
     DEG_FOR(c) {
         k[1][c] = (((RO(h21) * K1[c] + RO(h22) * K2[c]) + RO(h23) * K3[c]) + RO(h24) * K4[c]) + RO(h25) * K5[c];
@@ -839,22 +890,22 @@ __device__ __forceinline__ float deg_rodas(const float (&uprev)[DEG_N], const fl
     deg_rhs(u, p, t + RO(c5) * dt, du);
     // stage 5 (its C terms in the reference's order 2, 4, 1, 3)
     DEG_RODAS_LT((du[c] + dtd5 * dT[c]), (((dtC52 * K2[c] + dtC54 * K4[c]) + dtC51 * K1[c]) + dtC53 * K3[c]))
-    w_solve(W, piv, lt, K5);
+    w_solve(W, lt, K5);
     DEG_FOR(c) u[c] = ((((uprev[c] + RO(a61) * K1[c]) + RO(a62) * K2[c]) + RO(a63) * K3[c]) + RO(a64) * K4[c]) + RO(a65) * K5[c];
     deg_rhs(u, p, t + dt, du);
     // stage 6
     DEG_RODAS_LT(du[c], ((((dtC61 * K1[c] + dtC62 * K2[c]) + dtC63 * K3[c]) + dtC64 * K4[c]) + dtC65 * K5[c]))
-    w_solve(W, piv, lt, K6);
+    w_solve(W, lt, K6);
     DEG_FOR(c) u[c] = u[c] + K6[c];
     deg_rhs(u, p, t + dt, du);
     // stage 7
     DEG_RODAS_LT(du[c], (((((dtC71 * K1[c] + dtC72 * K2[c]) + dtC73 * K3[c]) + dtC74 * K4[c]) + dtC75 * K5[c]) + dtC76 * K6[c]))
-    w_solve(W, piv, lt, K7);
+    w_solve(W, lt, K7);
     DEG_FOR(c) u[c] = u[c] + K7[c];
     deg_rhs(u, p, t + dt, du);
     // stage 8
     DEG_RODAS_LT(du[c], ((((((dtC81 * K1[c] + dtC82 * K2[c]) + dtC83 * K3[c]) + dtC84 * K4[c]) + dtC85 * K5[c]) + dtC86 * K6[c]) + dtC87 * K7[c]))
-    w_solve(W, piv, lt, K8);
+    w_solve(W, lt, K8);
     DEG_FOR(c) unew[c] = u[c] + K8[c];
     DEG_FOR(c) {
         k[1][c] = ((((((RO(h21) * K1[c] + RO(h22) * K2[c]) + RO(h23) * K3[c]) + RO(h24) * K4[c]) + RO(h25) * K5[c]) + RO(h26) * K6[c]) + RO(h27) * K7[c]) + RO(h28) * K8[c];
@@ -903,19 +954,19 @@ __device__ __forceinline__ void deg_interp(float th, float dt, float tprev, cons
 #define DEG_KV_NEWTON_MAX 30
 #define DEG_KV_NEWTON_TOL (100.0f * 1.1920928955078125e-7f)  // 100 eps(Float32)
 // W = -M + (gamma dt) J(x, tc) (M = I without a mass matrix), LU-factored in FP64
-__device__ __forceinline__ void kv_w(double (&W)[DEG_N][DEG_N], int (&piv)[DEG_N], float gdt,
-                                     const float (&x)[DEG_N], const float* p, float tc) {
-    float Jm[DEG_N * DEG_N];
+__device__ __forceinline__ void kv_w(deg_wf& F, float gdt, const float (&x)[DEG_N], const float* p, float tc) {
+    float Jm[DEG_N * DEG_JC];
     deg_jac(x, p, tc, Jm);
     const double g = (double)gdt;
-    DEG_FOR(r) {
+    _Pragma("unroll") for (int r = 0; r < DEG_JC; ++r) {
 #if DEG_MASS
-        _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = -(double)deg_mass_at(r, c) + g * (double)Jm[r * DEG_N + c];
+        _Pragma("unroll") for (int c = 0; c < DEG_JC; ++c) F.W[r][c] = -(double)deg_mass_at(r, c) + g * (double)Jm[r * DEG_JC + c];
 #else
-        _Pragma("unroll") for (int c = 0; c < DEG_N; ++c) W[r][c] = (r == c ? -1.0 : 0.0) + g * (double)Jm[r * DEG_N + c];
+        _Pragma("unroll") for (int c = 0; c < DEG_JC; ++c) F.W[r][c] = (r == c ? -1.0 : 0.0) + g * (double)Jm[r * DEG_JC + c];
 #endif
     }
-    w_lu(W, piv);
+    w_lu(F);
+    w_dual(F, g, Jm);
 }
 // Newton on z for the stage x = tmp + gamma z at tc = tnl + c dtnl: z -= W \ (dtnl f(x) - z), at
 // most 30 iterations, until ||dtnl f(tmp + gamma z) - z|| < 100 eps(Float32)
@@ -926,9 +977,8 @@ __device__ __forceinline__ void kv_newton(float (&z)[DEG_N], const float (&tmp)[
     for (int it = 0; it < DEG_KV_NEWTON_MAX; ++it) {
         float x[DEG_N], fe[DEG_N], r[DEG_N], dz[DEG_N];
         DEG_FOR(q) x[q] = tmp[q] + KV(gamma) * z[q];
-        double W[DEG_N][DEG_N];
-        int piv[DEG_N];
-        kv_w(W, piv, gdt, x, p, tc);
+        deg_wf W;
+        kv_w(W, gdt, x, p, tc);
         deg_rhs(x, p, tc, fe);
 #if DEG_MASS
         float mz[DEG_N];
@@ -937,7 +987,7 @@ __device__ __forceinline__ void kv_newton(float (&z)[DEG_N], const float (&tmp)[
 #else
         DEG_FOR(q) r[q] = dtnl * fe[q] - z[q];
 #endif
-        w_solve(W, piv, r, dz);
+        w_solve(W, r, dz);
         DEG_FOR(q) z[q] = z[q] - dz[q];
         DEG_FOR(q) x[q] = tmp[q] + KV(gamma) * z[q];
         deg_rhs(x, p, tc, fe);
@@ -947,7 +997,14 @@ __device__ __forceinline__ void kv_newton(float (&z)[DEG_N], const float (&tmp)[
 #else
         DEG_FOR(q) r[q] = dtnl * fe[q] - z[q];
 #endif
+#if DEG_DUAL_P
+        // diffeqgpunorm (utils.jl:1-4): over a dual array, the values alone
+        float acc = r[0] * r[0];
+        _Pragma("unroll") for (int q = 1; q < DEG_JC; ++q) acc = acc + r[q] * r[q];
+        if (__builtin_sqrtf(acc / (float)DEG_JC) < DEG_KV_NEWTON_TOL) break;
+#else
         if (rms_norm(r) < DEG_KV_NEWTON_TOL) break;
+#endif
     }
 }
 // One Kvaerno step from (uprev, t) with step dt and first stage slope k1 (z1 = dt k1); the
@@ -1005,11 +1062,10 @@ __device__ __forceinline__ float deg_kvaerno(const float (&uprev)[DEG_N], const 
     DEG_FOR(q) ev[q] = ((((KV(btilde1) * z1[q] + KV(btilde3) * z3[q]) + KV(btilde4) * z4[q]) + KV(btilde5) * z5[q]) +
                         KV(btilde6) * z6[q]) + KV(btilde7) * zl[q];
 #endif
-    double W[DEG_N][DEG_N];
-    int piv[DEG_N];
+    deg_wf W;
     float e[DEG_N];
-    kv_w(W, piv, KV(gamma) * dt, unew, p, t + 1.0f * dt);
-    w_solve(W, piv, ev, e);
+    kv_w(W, KV(gamma) * dt, unew, p, t + 1.0f * dt);
+    w_solve(W, ev, e);
 #if DEG_FASTPOW
     return scaled_error2_fast(e, uprev, unew, abstol, reltol);
 #else

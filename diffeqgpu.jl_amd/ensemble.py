@@ -16,7 +16,7 @@ from typing import Any, List, Optional
 import numpy as np
 
 from . import _lib
-from .algorithms import EnsembleGPUKernel, GPUODEAlgorithm, GPUODEImplicitAlgorithm, GPUSDEAlgorithm
+from .algorithms import EnsembleGPUKernel, GPUODEAlgorithm, GPUSDEAlgorithm
 from .callbacks import merged_callback
 from .dual import split_rows
 from .lowerlevel import check_mass, vectorized_asolve, vectorized_solve
@@ -163,9 +163,6 @@ def batch_solve_up_kernel(ensembleprob, probs, alg, ensemblealg, I, adaptive, **
         kwargs.pop("callback")
     if isinstance(alg, GPUODEAlgorithm):
         check_mass(probs[0].f, alg)
-        if probs[0].dual is not None and isinstance(alg, GPUODEImplicitAlgorithm):
-            raise NotImplementedError(f"{type(alg).__name__} with dual-number inputs: the implicit methods would need a "
-                                      "W factor with dual entries; use GPUTsit5, GPUVern7 or GPUVern9")
         if probs[0].dual is not None and kwargs.get("callback") is not None:
             raise NotImplementedError("callbacks with dual-number inputs are not supported")
     if adaptive and isinstance(alg, GPUSDEAlgorithm):

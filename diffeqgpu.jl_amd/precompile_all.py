@@ -63,6 +63,14 @@ def variants():
             for m in (_lib.DEG_TSIT5, _lib.DEG_VERN7, _lib.DEG_VERN9):
                 for dev, adaptive in more + [(HIPDevice(), True)]:
                     out.append((name, src, n, np_, m, dev.flags(adaptive)))
+            # the implicit methods: the dual Jacobian by forward differences (autodiff = false) and by
+            # forward mode (fixed step)
+            fd_src, _ = f.hip_source(n, np_, True, fd=True)
+            ad_src, _ = f.hip_source(n, np_, True, fd=False)
+            for m in _IMPLICIT:
+                for dev, adaptive in more + [(HIPDevice(), True)]:
+                    out.append((name, fd_src, n, np_, m, dev.flags(adaptive)))
+                out.append((name, ad_src, n, np_, m, HIPDevice(contract="off").flags(False)))
             continue
         for m in (_IMPLICIT if f.mass_matrix is not None else _METHODS_MORE):
             src, _ = f.hip_source(n, np_, m in _IMPLICIT)

@@ -117,8 +117,9 @@ def _as_tspan(tspan) -> tuple:
 class DualODEFunction(ODEFunction):
     """The augmented right-hand side of a problem whose u0 / p hold dual numbers: state and
     parameters are values followed by partials (:mod:`diffeqgpu_amd.dual`), and deg_rhs is the
-    traced dual evaluation of ``f``. Only the explicit methods solve it (the implicit methods
-    would need a W factor with dual entries)."""
+    traced dual evaluation of ``f``; for the implicit methods the source adds the dual Jacobian over
+    the value columns and the augmented time gradient (:func:`diffeqgpu_amd.tracer.emit_dual_rhs`),
+    and the kernel factors W over the values and solves each partial with that factor."""
 
     def __init__(self, f: ODEFunction, n: int, np_: int, P: int):
         if f.mass_matrix is not None:
@@ -128,14 +129,15 @@ class DualODEFunction(ODEFunction):
         self.n, self.np, self.P = int(n), int(np_), int(P)
 
     def hip_source(self, n: int, np_: int, with_jac: bool, fd: bool = False) -> tuple[str, int]:
-        if with_jac:
-            raise NotImplementedError("dual-number problems are solved by the explicit methods (GPUTsit5, GPUVern7, "
-                                      "GPUVern9); the implicit methods would need a W factor with dual entries")
+        if with_jac and (self.base.jac is not None or self.base.tgrad is not None):
+            raise NotImplementedError("dual-number problems take the Jacobian from the traced f: an ODEFunction "
+                                      "jac / tgrad is not evaluated on dual numbers here")
         if (n, np_) != (self.n * (1 + self.P), self.np * (1 + self.P)):
             raise ValueError("augmented sizes do not match the dual problem")
-        key = ("dual", n, np_)
+        key = ("dual", n, np_, bool(with_jac), bool(fd) and bool(with_jac))
         if key not in self._src_cache:
-            self._src_cache[key] = tracer.emit_dual_rhs(self.f, self.n, self.np, self.P)
+            self._src_cache[key] = tracer.emit_dual_rhs(self.f, self.n, self.np, self.P, with_jac=with_jac,
+                                                        fd=bool(fd) and bool(with_jac))
         return self._src_cache[key]
 
 

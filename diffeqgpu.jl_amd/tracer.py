@@ -548,11 +548,20 @@ def trace_dual(f: Callable, n: int, np_: int, P: int) -> "Trace":
     return Trace(g, ua, pa, t, outs)
 
 
-def emit_dual_rhs(f: Callable, n: int, np_: int, P: int) -> Tuple[str, int]:
+def emit_dual_rhs(f: Callable, n: int, np_: int, P: int, with_jac: bool = False,
+                  fd: bool = False) -> Tuple[str, int]:
     """HIP C for the augmented right-hand side of a problem with dual inputs: deg_rhs over the
     n (1 + P) state entries (values, then the partials of each component) with the np (1 + P)
     parameters laid out the same way, and the macros DEG_DUAL_NV (n) and DEG_DUAL_P (P) that
-    select the dual error norm. Returns the source and the flop count of the value part."""
+    select the dual error norm and the dual W factor. Returns the source and the flop count of the
+    value part.
+
+    ``with_jac`` (the implicit methods) adds the dual Jacobian over the value columns, deg_jac
+    writing J[r n + c] = d F_r / d u_c for the n (1 + P) augmented rows r and the n values c (rows n
+    + i P + j are partial j of the Jacobian's row i), and the augmented time gradient deg_tgrad: by
+    forward-mode differentiation of the traced dual graph, or with ``fd`` (``autodiff=False``) by
+    forward differences of the dual deg_rhs that perturb a value alone, as finite_diff_jac does on a
+    dual u (alg_utils.jl:17-27)."""
     tr = trace_dual(f, n, np_, P)
     body, _ = _emit_body(tr.g, tr.outs, "du")
     # the value part alone, for the flop count
@@ -562,6 +571,30 @@ def emit_dual_rhs(f: Callable, n: int, np_: int, P: int) -> Tuple[str, int]:
            "__device__ __forceinline__ void deg_rhs(const float* __restrict__ u, const float* __restrict__ p, float t,",
            "                                        float* __restrict__ du) {",
            "    (void)u; (void)p; (void)t;", *body, "}"]
+    if with_jac:
+        N = n * (1 + P)
+        if fd:
+            src += _fd_jac_lines(N, cols=n) + _fd_tgrad_lines(N)
+        else:
+            g = tr.g
+            zero = g.const(0.0)
+            J = []
+            for o in tr.outs:
+                for c in range(n):
+                    dd = _Diff(g, tr.u[c]).d(o)
+                    J.append(zero if dd is None else dd)
+            T = []
+            for o in tr.outs:
+                dd = _Diff(g, tr.t).d(o)
+                T.append(zero if dd is None else dd)
+            jbody, _ = _emit_body(g, J, "J")
+            tbody, _ = _emit_body(g, T, "dT")
+            src += ["__device__ __forceinline__ void deg_jac(const float* __restrict__ u, const float* __restrict__ p, float t,",
+                    "                                        float* __restrict__ J) {",
+                    "    (void)u; (void)p; (void)t;", *jbody, "}",
+                    "__device__ __forceinline__ void deg_tgrad(const float* __restrict__ u, const float* __restrict__ p, float t,",
+                    "                                          float* __restrict__ dT) {",
+                    "    (void)u; (void)p; (void)t;", *tbody, "}"]
     return "\n".join(src) + "\n", flops
 
 
@@ -838,20 +871,22 @@ def count_flops(tr: Trace) -> int:
 FD_STEP_BITS = 0x39B504F3
 
 
-def _fd_jac_lines(n: int):
+def _fd_jac_lines(n: int, cols: int | None = None):
     """deg_jac by forward differences (finite_diff_jac, alg_utils.jl:17-27): column i is
-    (f(u + dx e_i) - f(u)) / dx, with the perturbed entry rounded to Float32 and a true division."""
+    (f(u + dx e_i) - f(u)) / dx, with the perturbed entry rounded to Float32 and a true division.
+    ``cols``: only the first cols columns (the values of a dual-number state), J[r * cols + i]."""
+    m = n if cols is None else cols
     return [
         "__device__ __forceinline__ void deg_jac(const float* __restrict__ u, const float* __restrict__ p, float t,",
         "                                        float* __restrict__ J) {",
         f"    const float dx = __builtin_bit_cast(float, 0x{FD_STEP_BITS:08x}u);",
         f"    float f0[{n}], f1[{n}], ux[{n}];",
         "    deg_rhs(u, p, t, f0);",
-        f"    _Pragma(\"unroll\") for (int i = 0; i < {n}; ++i) {{",
+        f"    _Pragma(\"unroll\") for (int i = 0; i < {m}; ++i) {{",
         f"        _Pragma(\"unroll\") for (int r = 0; r < {n}; ++r) ux[r] = u[r];",
         "        ux[i] = u[i] + dx;",
         "        deg_rhs(ux, p, t, f1);",
-        f"        _Pragma(\"unroll\") for (int r = 0; r < {n}; ++r) J[r * {n} + i] = (f1[r] - f0[r]) / dx;",
+        f"        _Pragma(\"unroll\") for (int r = 0; r < {n}; ++r) J[r * {m} + i] = (f1[r] - f0[r]) / dx;",
         "    }",
         "}"]
 

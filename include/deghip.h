@@ -162,7 +162,10 @@ int deg_set_cache_dir(const char *dir);
  * an augmented state of n values followed by the P partials of each component, n (1 + P) entries in all,
  * with the parameters laid out the same way; the source then also defines
  *   #define DEG_DUAL_NV <n>   #define DEG_DUAL_P <P>
- * which selects the dual error norm (DEG_TSIT5, DEG_VERN7, DEG_VERN9 only, no callbacks).
+ * which selects the dual error norm (any ODE method; no callbacks, mass matrix, proposer or step logs). For
+ * the implicit methods deg_jac then writes the n (1 + P) x n Jacobian over the value columns, J[r*n + c]
+ * (rows n + i P + j: partial j of the Jacobian's row i), deg_tgrad all n (1 + P) entries, and the solver
+ * factors W over the n values and solves each partial with that factor.
  * With DEG_F_CALLBACKS it also defines the callbacks (GPUDiscreteCallback / GPUContinuousCallback,
  * src/ensemblegpukernel/callbacks.jl:1-124), j counting from 0 in CallbackSet order:
  *   #define DEG_NDCB <discrete callbacks>   #define DEG_NCCB <continuous callbacks>

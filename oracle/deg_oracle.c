@@ -63,7 +63,9 @@
 
 enum { OM_TSIT5 = 0, OM_VERN7 = 1, OM_RB23 = 2, OM_VERN9 = 3, OM_RODAS4 = 4, OM_RODAS5P = 5, OM_KVAERNO3 = 6,
        OM_KVAERNO5 = 7 };
-enum { DEG_O_KAHAN = 1, DEG_O_DETPOW = 2, DEG_O_FDJAC = 4 };
+/* DEG_O_AUGMENTED: a dual-number system solved as a plain system of its n (1 + P) entries (the full
+ * augmented Jacobian and W, the plain norm); a pin of the dual W factor, not a reference behaviour */
+enum { DEG_O_KAHAN = 1, DEG_O_DETPOW = 2, DEG_O_FDJAC = 4, DEG_O_AUGMENTED = 8 };
 
 /* autodiff = false: J by forward differences, column i = (f(u + dx e_i) - f(u)) / dx with the
  * perturbed entry rounded to Float32 (finite_diff_jac, alg_utils.jl:17-27), and the time gradient
@@ -76,20 +78,26 @@ static float fd_step(void) {
     return x;
 }
 
+/* Over a dual-number state (S->dualp > 0) J has the columns of the dualnv values only,
+ * J[r * dualnv + c] = d f_r / d value_c for all n rows: rows 0..dualnv-1 are the Jacobian's values and
+ * row dualnv + r P + j its partial j (the forward difference of the dual right-hand side is taken in
+ * dual arithmetic, finite_diff_jac on a dual u: the step dx perturbs the value alone). */
+static int jac_cols(const osys *S) { return S->dualp > 0 ? S->dualnv : S->n; }
+
 static void sys_jac(const osys *S, const float *u, const float *p, float t, float *J) {
     if (!S->fd) {
         S->jac(u, p, t, J);
         return;
     }
-    const int n = S->n;
+    const int n = S->n, m = jac_cols(S);
     const float dx = fd_step();
     float f0[OMAXN], f1[OMAXN], ux[OMAXN];
     S->f(u, p, t, f0);
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < m; ++i) {
         for (int r = 0; r < n; ++r) ux[r] = u[r];
         ux[i] = u[i] + dx;
         S->f(ux, p, t, f1);
-        for (int r = 0; r < n; ++r) J[r * n + i] = (f1[r] - f0[r]) / dx;
+        for (int r = 0; r < n; ++r) J[r * m + i] = (f1[r] - f0[r]) / dx;
     }
 }
 
@@ -326,6 +334,18 @@ static float dual_eest(int nv, int P, int n, const float *e, const float *uprev,
     return sqrtf(acc / (float)n);
 }
 
+/* EEst = norm(e ./ (abstol .+ max.(|uprev|,|u|) * reltol)), the dual norm over a dual state (e is scaled
+ * in place on the plain path) */
+static float scaled_eest(const osys *S, float *e, const float *uprev, const float *u, float abstol, float reltol) {
+    const int n = S->n;
+    if (S->dualp > 0) return dual_eest(S->dualnv, S->dualp, n, e, uprev, u, abstol, reltol);
+    for (int c = 0; c < n; ++c) {
+        float sc = abstol + jl_max(fabsf(uprev[c]), fabsf(u[c])) * reltol;
+        e[c] = e[c] / sc;
+    }
+    return rms_norm(e, n);
+}
+
 /* scaled error  EEst = norm( dt*(sum btilde_j k_j) ./ (abstol .+ max.(|uprev|,|u|) * reltol) ) */
 static float erk_eest(const erk32 *T, int n, float dt, float k[][OMAXN], const float *uprev,
                       const float *u, float abstol, float reltol, int dualnv, int dualp) {
@@ -487,7 +507,22 @@ static void vern9_interp(const osys *S, const float *p, float th, float dt, floa
 typedef struct {
     double lu[OMAXN][OMAXN];
     int perm[OMAXN];
+    /* dual-number state (P > 0): lu factors the nv x nv values of W = alpha M + beta J, and the
+     * partials of W are beta times rows nv.. of the dual Jacobian, kept in jp */
+    int nv, P;
+    double beta;
+    float jp[OMAXN][OMAXN];
 } olu;
+
+/* Record the dual part of a factor built from J (the n x nv dual Jacobian of sys_jac) */
+static void w_dual(olu *F, const osys *S, double beta, const float *J) {
+    F->nv = S->dualp > 0 ? S->dualnv : 0;
+    F->P = S->dualp;
+    F->beta = beta;
+    if (F->P == 0) return;
+    for (int r = F->nv; r < S->n; ++r)
+        for (int c = 0; c < F->nv; ++c) F->jp[r][c] = J[r * F->nv + c];
+}
 
 static void w_lu(olu *F, int n) {
     for (int r = 0; r < n; ++r) F->perm[r] = r;
@@ -533,27 +568,32 @@ static void mass_mul(const float *M, int n, const float *v, float *out) {
 }
 
 /* Rosenbrock23: W = M - gamma J (gpu_rosenbrock23_perform_step.jl:44-45), M = I without a mass matrix */
-static void w_factor(olu *F, int n, float gam, const float *J, const float *M) {
+static void w_factor(olu *F, const osys *S, float gam, const float *J, const float *M) {
+    const int m = jac_cols(S);
     double g = (double)gam;
-    for (int r = 0; r < n; ++r)
-        for (int c = 0; c < n; ++c)
-            F->lu[r][c] = (M ? (double)M[r * n + c] : (r == c ? 1.0 : 0.0)) - g * (double)J[r * n + c];
-    w_lu(F, n);
+    for (int r = 0; r < m; ++r)
+        for (int c = 0; c < m; ++c)
+            F->lu[r][c] = (M ? (double)M[r * m + c] : (r == c ? 1.0 : 0.0)) - g * (double)J[r * m + c];
+    w_lu(F, m);
+    w_dual(F, S, -g, J);
 }
 
 /* Rodas: W = J - M inv(dt gamma) (gpu_rodas4_perform_step.jl:66-67), the reciprocal and its product
  * with M taken in Float32 */
-static void w_factor_rodas(olu *F, int n, float invg, const float *J, const float *M) {
-    for (int r = 0; r < n; ++r)
-        for (int c = 0; c < n; ++c)
-            F->lu[r][c] = (double)J[r * n + c] - (M ? (double)(M[r * n + c] * invg) : (r == c ? (double)invg : 0.0));
-    w_lu(F, n);
+static void w_factor_rodas(olu *F, const osys *S, float invg, const float *J, const float *M) {
+    const int m = jac_cols(S);
+    for (int r = 0; r < m; ++r)
+        for (int c = 0; c < m; ++c)
+            F->lu[r][c] = (double)J[r * m + c] - (M ? (double)(M[r * m + c] * invg) : (r == c ? (double)invg : 0.0));
+    w_lu(F, m);
+    w_dual(F, S, 1.0, J);
 }
 
-static void w_solve(const olu *F, int n, const float *b, float *x) {
+/* y = LU \ b in FP64 (b in FP64, overwritten) */
+static void lu_solve_d(const olu *F, int n, double *b) {
     double y[OMAXN];
     for (int i = 0; i < n; ++i) {
-        double v = (double)b[F->perm[i]];
+        double v = b[F->perm[i]];
         for (int j = 0; j < i; ++j) v = v - F->lu[i][j] * y[j];
         y[i] = v;
     }
@@ -562,7 +602,29 @@ static void w_solve(const olu *F, int n, const float *b, float *x) {
         for (int j = i + 1; j < n; ++j) v = v - F->lu[i][j] * y[j];
         y[i] = v / F->lu[i][i];
     }
-    for (int i = 0; i < n; ++i) x[i] = (float)y[i];
+    for (int i = 0; i < n; ++i) b[i] = y[i];
+}
+
+/* x = W \ b. Over a dual-number state the solve is the dual one, value LU plus one solve per
+ * partial: x_v = W_v \ b_v and x_j = W_v \ (b_j - W_j x_v), with W_j = beta J_j and x_v kept in FP64
+ * for the product (the derivative of W^-1 b: W^-1 (b' - W' W^-1 b)). */
+static void w_solve(const olu *F, int n, const float *b, float *x) {
+    const int m = F->P > 0 ? F->nv : n;
+    double y[OMAXN];
+    for (int i = 0; i < m; ++i) y[i] = (double)b[i];
+    lu_solve_d(F, m, y);
+    for (int i = 0; i < m; ++i) x[i] = (float)y[i];
+    for (int j = 0; j < F->P; ++j) {
+        double z[OMAXN];
+        for (int r = 0; r < m; ++r) {
+            const int q = m + r * F->P + j;
+            double v = (double)b[q];
+            for (int c = 0; c < m; ++c) v = v - (F->beta * (double)F->jp[q][c]) * y[c];
+            z[r] = v;
+        }
+        lu_solve_d(F, m, z);
+        for (int r = 0; r < m; ++r) x[m + r * F->P + j] = (float)z[r];
+    }
 }
 
 /* ------------------------------------------------------------------------------------ */
@@ -588,7 +650,7 @@ static void rodas4_step(const osys *S, const float *p, const float *uprev, float
     const float dtd1 = dt * R4(d1), dtd2 = dt * R4(d2), dtd3 = dt * R4(d3), dtd4 = dt * R4(d4);
     const float dtgamma = dt * R4(gamma);
     olu F;
-    w_factor_rodas(&F, n, 1.0f / dtgamma, Jm, S->M);
+    w_factor_rodas(&F, S, 1.0f / dtgamma, Jm, S->M);
     S->f(uprev, p, t, du);
     for (int c = 0; c < n; ++c) lt[c] = -(du[c] + dtd1 * dT[c]);
     w_solve(&F, n, lt, K1);
@@ -650,7 +712,7 @@ static void rodas5p_step(const osys *S, const float *p, const float *uprev, floa
     const float dtd1 = dt * R5(d1), dtd2 = dt * R5(d2), dtd3 = dt * R5(d3), dtd4 = dt * R5(d4), dtd5 = dt * R5(d5);
     const float dtgamma = dt * R5(gamma);
     olu F;
-    w_factor_rodas(&F, n, 1.0f / dtgamma, Jm, S->M);
+    w_factor_rodas(&F, S, 1.0f / dtgamma, Jm, S->M);
     S->f(uprev, p, t, du);
     for (int c = 0; c < n; ++c) lt[c] = -(du[c] + dtd1 * dT[c]);
     w_solve(&F, n, lt, K1);
@@ -725,13 +787,14 @@ static void rodas_interp(int method, float th, int n, const float *y0, const flo
 /* ------------------------------------------------------------------------------------ */
 /* W = -M + (gamma dt) J(x, tc) (nlsolve/type.jl:129-140), M = I without a mass matrix, factored in FP64 */
 static void kv_w(olu *F, const osys *S, const float *p, float gdt, const float *x, float tc) {
-    const int n = S->n;
+    const int m = jac_cols(S);
     float Jm[OMAXN * OMAXN];
     sys_jac(S, x, p, tc, Jm);
-    for (int r = 0; r < n; ++r)
-        for (int c = 0; c < n; ++c)
-            F->lu[r][c] = (S->M ? -(double)S->M[r * n + c] : (r == c ? -1.0 : 0.0)) + (double)gdt * (double)Jm[r * n + c];
-    w_lu(F, n);
+    for (int r = 0; r < m; ++r)
+        for (int c = 0; c < m; ++c)
+            F->lu[r][c] = (S->M ? -(double)S->M[r * m + c] : (r == c ? -1.0 : 0.0)) + (double)gdt * (double)Jm[r * m + c];
+    w_lu(F, m);
+    w_dual(F, S, (double)gdt, Jm);
 }
 
 /* nlsolve (nlsolve/utils.jl:1-27): Newton on z for x = tmp + gamma z at tc = tnl + c dtnl */
@@ -754,7 +817,8 @@ static void kv_newton(const osys *S, const float *p, float gamma, float *z, cons
         S->f(x, p, tc, fe);
         mass_mul(S->M, n, z, mz);
         for (int q = 0; q < n; ++q) r[q] = dtnl * fe[q] - mz[q];
-        if (rms_norm(r, n) < 100.0f * 1.1920928955078125e-7f) break;
+        /* diffeqgpunorm (utils.jl:1-4): over a dual array, the values alone */
+        if (rms_norm(r, S->dualp > 0 ? S->dualnv : n) < 100.0f * 1.1920928955078125e-7f) break;
     }
 }
 
@@ -1240,7 +1304,7 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                 sys_jac(S, s->uprev, s->p, told, Jm);
                 sys_tgrad(S, s->uprev, s->p, told, dT);
                 olu F;
-                w_factor(&F, n, gam, Jm, S->M);
+                w_factor(&F, S, gam, Jm, S->M);
                 S->f(s->uprev, s->p, told, F0);
                 for (int c = 0; c < n; ++c) rhs[c] = F0[c] + gam * dT[c];
                 w_solve(&F, n, rhs, s->rk1);
@@ -1311,21 +1375,13 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                     S->f(s->uprev, s->p, t, k[1]);
                     kvaerno_step(J->method, S, s->p, s->uprev, t, dt, t, dt, k[1], zl, unew, e);
                     for (int c = 0; c < n; ++c) k[2][c] = zl[c] / dt;
-                    for (int c = 0; c < n; ++c) {
-                        float sc = J->abstol + jl_max(fabsf(s->uprev[c]), fabsf(unew[c])) * J->reltol;
-                        e[c] = e[c] / sc;
-                    }
-                    EEst = rms_norm(e, n);
+                    EEst = scaled_eest(S, e, s->uprev, unew, J->abstol, J->reltol);
                 } else if (J->method == OM_RODAS4 || J->method == OM_RODAS5P) {
                     /* gpu_rodas4_perform_step.jl:158-238, gpu_rodas5P_perform_step.jl:186-296 */
                     float e[OMAXN];
                     if (J->method == OM_RODAS4) rodas4_step(S, s->p, s->uprev, t, dt, k, unew, e);
                     else rodas5p_step(S, s->p, s->uprev, t, dt, k, unew, e);
-                    for (int c = 0; c < n; ++c) {
-                        float sc = J->abstol + jl_max(fabsf(s->uprev[c]), fabsf(unew[c])) * J->reltol;
-                        e[c] = e[c] / sc;
-                    }
-                    EEst = rms_norm(e, n);
+                    EEst = scaled_eest(S, e, s->uprev, unew, J->abstol, J->reltol);
                 } else if (J->method == OM_RB23) {
                     /* gpu_rosenbrock23_perform_step.jl:106-153 */
                     float Jm[OMAXN * OMAXN], dT[OMAXN], F0[OMAXN], F1[OMAXN], F2[OMAXN], rhs[OMAXN], tmp[OMAXN], x[OMAXN];
@@ -1335,7 +1391,7 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                     sys_jac(S, s->uprev, s->p, t, Jm);
                     sys_tgrad(S, s->uprev, s->p, t, dT);
                     olu F;
-                    w_factor(&F, n, gam, Jm, S->M);
+                    w_factor(&F, S, gam, Jm, S->M);
                     S->f(s->uprev, s->p, t, F0);
                     for (int c = 0; c < n; ++c) rhs[c] = F0[c] + gam * dT[c];
                     w_solve(&F, n, rhs, s->rk1);
@@ -1362,11 +1418,7 @@ static int solve_traj(const ojob *J, int64_t i, const float *u0, const float *p,
                     w_solve(&F, n, rhs, k3);
                     float e[OMAXN] = {0};
                     for (int c = 0; c < n; ++c) e[c] = dto6 * ((s->rk1[c] - 2.0f * s->rk2[c]) + k3[c]);
-                    for (int c = 0; c < n; ++c) {
-                        float sc = J->abstol + jl_max(fabsf(s->uprev[c]), fabsf(unew[c])) * J->reltol;
-                        e[c] = e[c] / sc;
-                    }
-                    EEst = rms_norm(e, n);
+                    EEst = scaled_eest(S, e, s->uprev, unew, J->abstol, J->reltol);
                 } else {
                     /* gpu_vern7_perform_step.jl:117, gpu_vern9_perform_step.jl:170: k1 in every attempt */
                     if (J->method == OM_VERN7 || J->method == OM_VERN9) S->f(s->uprev, s->p, t, k[1]);
@@ -1464,6 +1516,10 @@ int deg_oracle_solve_cb(int system, int method, int adaptive, int64_t N,
     init_tabs();
     osys Sys = OSYSTEMS[system];
     Sys.fd = (flags & DEG_O_FDJAC) != 0;
+    if (flags & DEG_O_AUGMENTED) {
+        Sys.dualnv = 0;
+        Sys.dualp = 0;
+    }
     const osys *S = &Sys;
     ojob J;
     J.S = S;

@@ -6,6 +6,10 @@ what fixes it.
   partials        d u(t) / d (u0, p) at t = 1 against central differences of the FP64 Vern9 flow
                   (rtol 1e-12) at the solve's end time, fixed and adaptive, for Tsit5, Vern7 and Vern9; a sign or index slip
                   in a partial rule is off by O(1)
+  implicit        the five implicit methods (autodiff = false): fixed-step values bit-identical to the
+                  plain solve; partials against the flow derivative as above; the dual W solve against
+                  the same method on the augmented system with its full Jacobian (which pins the
+                  partials of W: leaving them out misses by 10x the bar)
   dual norm       with zero partials the dual error norm is the plain one divided by sqrt(1 + P);
                   with abstol = 0 the plain norm scales as 1 / reltol, so an adaptive dual solve
                   with zero seeds at reltol r takes the steps of the plain solve at r sqrt(1 + P)
@@ -55,11 +59,12 @@ def test_dual_api(deg):
 
     rows = split_rows(np.arange(2 * 21, dtype=F32).reshape(2, 21), 3, 6)
     assert rows.shape == (2, 3, 7) and rows[1, 2, 0] == 21 + 2 and rows[1, 2, 1] == 21 + 3 + 12
-    # the implicit methods and callbacks are refused before any device work
+    # callbacks are refused before any device work
     ep = deg.EnsembleProblem(prob)
+    cb = deg.GPUDiscreteCallback(lambda u, t, integ: t == 1.0, lambda integ: None)
     with pytest.raises(NotImplementedError, match="dual"):
-        deg.solve(ep, deg.GPURosenbrock23(autodiff=False), deg.EnsembleGPUKernel(), trajectories=2, dt=0.01,
-                  adaptive=False)
+        deg.solve(ep, deg.GPUTsit5(), deg.EnsembleGPUKernel(), trajectories=2, dt=0.01, adaptive=False,
+                  callback=cb, merge_callbacks=True)
 
 
 def test_dual_source_and_jit(deg):
@@ -68,13 +73,19 @@ def test_dual_source_and_jit(deg):
     f, n, np_, _ = systems.SYSTEMS["lorenz_dual"]
     src, flops = f.hip_source(n, np_, False)
     assert "#define DEG_DUAL_NV 3" in src and "#define DEG_DUAL_P 6" in src and flops == 8
-    with pytest.raises(NotImplementedError):
-        f.hip_source(n, np_, True)
     for m in (_lib.DEG_TSIT5, _lib.DEG_VERN7):
         assert len(_lib.precompile(src, n, np_, m, _lib.F_PARITY)) == 16
+    # the implicit methods: the dual Jacobian over the 3 value columns of the 21 augmented rows
+    fd_src, _ = f.hip_source(n, np_, True, fd=True)
+    assert "for (int i = 0; i < 3; ++i)" in fd_src and "J[r * 3 + i]" in fd_src and "deg_tgrad" in fd_src
+    ad_src, _ = f.hip_source(n, np_, True, fd=False)
+    assert "J[62] = " in ad_src and "J[63] = " not in ad_src and "dT[20] = " in ad_src
     for m in (_lib.DEG_RB23, _lib.DEG_KVAERNO5):
-        with pytest.raises(_lib.DegError, match="explicit"):
-            _lib.precompile(src, n, np_, m, _lib.F_PARITY)
+        assert len(_lib.precompile(fd_src, n, np_, m, _lib.F_PARITY)) == 16
+    with pytest.raises(_lib.DegError, match="mass"):
+        _lib.precompile("#define DEG_MASS 1\n" + fd_src, n, np_, _lib.DEG_RB23, _lib.F_PARITY)
+    with pytest.raises(_lib.DegError, match="ODE methods|noise"):
+        _lib.precompile(src, n, np_, _lib.DEG_EM, _lib.F_PARITY)
 
 
 @pytest.mark.parametrize("method", ["tsit5", "vern7", "vern9"])
@@ -122,6 +133,64 @@ def test_dual_partials_against_flow_derivative(oracle, method, adaptive):
     scale = np.abs(J).max(axis=(0, 1))
     err = np.abs(parts - J).max(axis=(0, 1)) / scale
     assert err.max() < 1e-3, err.max()
+
+
+IMPLICIT = ["rb23", "rodas4", "rodas5p", "kvaerno3", "kvaerno5"]
+
+
+@pytest.mark.parametrize("method", IMPLICIT)
+def test_dual_implicit_values_equal_plain(oracle, method):
+    """The implicit methods with autodiff = false (the reference's forward_diff.jl settings): the W
+    factor of a dual state is built from the Jacobian's values alone, so the value part of a fixed-step
+    dual solve is bit-identical to the plain solve."""
+    N = 32
+    u0, p, u0v, pv = _seeded(N)
+    d = P.solve("lorenz_dual", method, False, u0, p, (0.0, 1.0), 0.005, save_everystep=False, fdjac=True)
+    o = P.solve("lorenz", method, False, u0v, pv, (0.0, 1.0), 0.005, save_everystep=False, fdjac=True)
+    np.testing.assert_array_equal(d["us"][:, :3], o["us"])
+    np.testing.assert_array_equal(d["ts"], o["ts"])
+
+
+@pytest.mark.parametrize("method", IMPLICIT)
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_dual_implicit_partials_against_flow_derivative(oracle, method, adaptive):
+    N = 16
+    T = 1.0
+    u0, p, u0v, pv = _seeded(N)
+    kw = dict(abstol=1e-7, reltol=1e-7) if adaptive else {}
+    d = P.solve("lorenz_dual", method, adaptive, u0, p, (0.0, T), 0.001, save_everystep=False, fdjac=True, **kw)
+    assert np.all(d["retcode"] == 0)
+    parts = d["us"][-1, 3:].astype(np.float64).reshape(3, NP, N)
+    te = float(d["ts"][-1, 0])
+    J = _fd_jacobian(u0v, pv, te, h=1e-5)
+    err = np.abs(parts - J).max(axis=(0, 1)) / np.abs(J).max(axis=(0, 1))
+    assert err.max() < 1e-3, err.max()
+
+
+# partials of the dual solve against the same method on the augmented system with its full Jacobian;
+# dropping the partials of W from the dual solve (x_j = W_v^-1 b_j) misses by 10x these bars or more
+W_BARS = {"rb23": 1e-3, "rodas4": 1e-3, "rodas5p": 2e-4, "kvaerno3": 2e-4, "kvaerno5": 1e-4}
+
+
+@pytest.mark.parametrize("method", IMPLICIT)
+def test_dual_implicit_w_partials_against_augmented_solve(oracle, method):
+    """The dual W solve x_j = W_v^-1 (b_j - beta J_j x_v) against the method applied to the 21-entry
+    augmented system as a plain system (its forward-difference Jacobian holds the partials of J in the
+    lower-left block and J again on the block diagonal; one 21 x 21 FP64 LU). Both differentiate the
+    discrete step, so at dt = 0.02 they agree to the forward-difference noise of the augmented
+    Jacobian's diagonal blocks; leaving out the W partials differs at first order in dt."""
+    N = 16
+    u0, p, _, _ = _seeded(N)
+    d = P.solve("lorenz_dual", method, False, u0, p, (0.0, 0.5), 0.02, save_everystep=False, fdjac=True)
+    g = P.solve("lorenz_dual", method, False, u0, p, (0.0, 0.5), 0.02, save_everystep=False, fdjac=True,
+                augmented=True)
+    pd = d["us"][-1, 3:].reshape(3, NP, N).astype(np.float64)
+    pg = g["us"][-1, 3:].reshape(3, NP, N).astype(np.float64)
+    rel = (np.abs(pd - pg).max(axis=(0, 1)) / np.abs(pg).max(axis=(0, 1))).max()
+    assert rel < W_BARS[method], rel
+    # the values: identical for the Rosenbrock methods; Kvaerno's Newton iterations stop on another norm
+    # (the values alone against every entry) and so a few iterations apart
+    np.testing.assert_allclose(d["us"][-1, :3], g["us"][-1, :3], rtol=1e-4)
 
 
 def test_dual_norm_zero_seeds(oracle):

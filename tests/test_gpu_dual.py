@@ -4,12 +4,16 @@ forward_diff.jl replayed through the high-level API.
 Bars:
   * no contraction, deterministic controller pow: bit-identical ts, us (values and partials), return
     codes and step counts with the oracle's dual Lorenz (6 partials), fixed and adaptive
-    (persistent grid), endpoint and saveat, for GPUTsit5, GPUVern7 and GPUVern9;
+    (persistent grid), endpoint and saveat, for GPUTsit5, GPUVern7 and GPUVern9, and for the implicit
+    methods GPURosenbrock23, GPURodas4, GPURodas5P, GPUKvaerno3 and GPUKvaerno5 with autodiff = false
+    (the dual W factor: the value LU plus one solve per partial);
+  * autodiff = true for the implicit methods (the dual Jacobian by forward mode over the traced dual
+    graph, which the oracle does not restate): the partials at t = 1 within 1e-3 of the exact flow's
+    d u / d (u0, p), as the oracle's own dual solves are pinned;
   * forward_diff.jl: u0 and p of Lorenz as duals seeded with the identity, trajectories = 2, fixed
-    dt = 0.01 (endpoint) and adaptive dt = 0.01 for the explicit methods: every trajectory succeeds,
-    the values equal the plain solve's (fixed step: bit for bit in parity mode), the partials are
-    finite and d u(t0) / d u0 is the identity. The reference also runs the implicit methods with
-    autodiff = false there; here they raise NotImplementedError (a W factor with dual entries).
+    dt = 0.01 (endpoint) and adaptive dt = 0.01 for all eight methods (the implicit ones with
+    autodiff = false): every trajectory succeeds, the values equal the plain solve's (fixed step: bit
+    for bit in parity mode), the partials are finite and d u(t0) / d u0 is the identity.
 """
 import numpy as np
 import pytest
@@ -20,6 +24,8 @@ from test_gpu_parity import assert_bitwise
 pytestmark = pytest.mark.gpu
 F32 = np.float32
 ALGS = {"tsit5": "GPUTsit5", "vern7": "GPUVern7", "vern9": "GPUVern9"}
+IMPLICIT = {"rb23": "GPURosenbrock23", "rodas4": "GPURodas4", "rodas5p": "GPURodas5P", "kvaerno3": "GPUKvaerno3",
+            "kvaerno5": "GPUKvaerno5"}
 
 
 @pytest.fixture(scope="module")
@@ -77,8 +83,57 @@ def test_dual_parity_bitwise(deg, torch_cuda, method, adaptive, mode):
     assert (g["retcode"] == 0).all()
 
 
+@pytest.mark.parametrize("method", list(IMPLICIT))
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_dual_implicit_parity_bitwise(deg, torch_cuda, method, adaptive):
+    from diffeqgpu_amd import systems
+
+    N = 128
+    u0, p = _inputs(N)
+    f, n, np_, _ = systems.SYSTEMS["lorenz_dual"]
+    tspan, dt = (0.0, 1.0), 0.01
+    sv = np.linspace(0.0, 1.0, 5).astype(F32)
+    kw = dict(abstol=1e-6, reltol=1e-6) if adaptive else {}
+    dev = deg.HIPDevice(contract="off", powmode="det", persistent=True) if adaptive else deg.HIPDevice(contract="off")
+    probs = deg.HIPProbVector.from_arrays(f, u0, p, tspan, dev)
+    prob = deg.ODEProblem(systems.lorenz, u0[:3, 0], tspan, p[:3, 0])
+    alg = getattr(deg, IMPLICIT[method])(autodiff=False)
+    print(f"[gpu] dual lorenz implicit {method} adaptive={adaptive} N={N}", flush=True)
+    fn = deg.vectorized_asolve if adaptive else deg.vectorized_solve
+    res = fn(probs, prob, alg, dt=dt, saveat=sv, **kw)
+    torch_cuda.cuda.synchronize()
+    g = dict(ts=res.ts.cpu().numpy(), us=res.us_soa.cpu().numpy(), retcode=res.retcode.cpu().numpy(),
+             naccept=res.naccept.cpu().numpy(), nreject=res.nreject.cpu().numpy())
+    o = pyoracle.solve("lorenz_dual", method, adaptive, u0, p, tspan, dt, detpow=adaptive, saveat=sv, fdjac=True, **kw)
+    assert_bitwise(g, o)
+    assert (g["retcode"] == 0).all()
+
+
+@pytest.mark.parametrize("method", ["rb23", "rodas5p", "kvaerno5"])
+def test_dual_implicit_autodiff_against_flow_derivative(deg, torch_cuda, method):
+    from diffeqgpu_amd import systems
+    from test_dual import _fd_jacobian
+
+    N = 16
+    u0, p = _inputs(N)
+    f, n, np_, _ = systems.SYSTEMS["lorenz_dual"]
+    dev = deg.HIPDevice(contract="off")
+    probs = deg.HIPProbVector.from_arrays(f, u0, p, (0.0, 1.0), dev)
+    prob = deg.ODEProblem(systems.lorenz, u0[:3, 0], (0.0, 1.0), p[:3, 0])
+    print(f"[gpu] dual lorenz implicit autodiff {method}", flush=True)
+    res = deg.vectorized_solve(probs, prob, getattr(deg, IMPLICIT[method])(), dt=0.001, save_everystep=False)
+    torch_cuda.cuda.synchronize()
+    us = res.us_soa.cpu().numpy()
+    te = float(res.ts.cpu().numpy()[-1, 0])
+    assert (res.retcode.cpu().numpy() == 0).all()
+    J = _fd_jacobian(u0[:3], p[:3], te, h=1e-5)
+    parts = us[-1, 3:].astype(np.float64).reshape(3, 6, N)
+    err = np.abs(parts - J).max(axis=(0, 1)) / np.abs(J).max(axis=(0, 1))
+    assert err.max() < 1e-3, err.max()
+
+
 def test_reference_forward_diff(deg, torch_cuda):
-    """forward_diff.jl:1-58 for the explicit methods."""
+    """forward_diff.jl:1-58: the explicit methods, and the implicit ones with autodiff = false."""
     from diffeqgpu_amd import systems
 
     D = deg.Dual
@@ -87,8 +142,9 @@ def test_reference_forward_diff(deg, torch_cuda):
     prob = deg.ODEProblem(systems.lorenz, u0, (0.0, 10.0), p)
     ep = deg.EnsembleProblem(prob, prob_func=lambda pr, ctx: deg.remake(pr, p=p), safetycopy=False)
     plain = deg.ODEProblem(systems.lorenz, [1.0, 0.0, 0.0], (0.0, 10.0), [10.0, 28.0, 8.0 / 3.0])
-    for method, name in ALGS.items():
-        alg = getattr(deg, name)()
+    algs = [(m, getattr(deg, nm)()) for m, nm in ALGS.items()]
+    algs += [(m, getattr(deg, nm)(autodiff=False)) for m, nm in IMPLICIT.items()]
+    for method, alg in algs:
         print(f"[gpu] reference forward_diff {method}", flush=True)
         dev = deg.HIPDevice(contract="off")
         sol = deg.solve(ep, alg, deg.EnsembleGPUKernel(dev), trajectories=2, save_everystep=False,
@@ -103,10 +159,6 @@ def test_reference_forward_diff(deg, torch_cuda):
         asol = deg.solve(ep, alg, deg.EnsembleGPUKernel(), trajectories=2, adaptive=True, dt=0.01)
         for s in asol.u:
             assert s.retcode == "Success" and s.u.shape[1:] == (3, 7) and np.all(np.isfinite(s.u))
-    for name in ("GPURosenbrock23", "GPURodas4", "GPURodas5P", "GPUKvaerno3", "GPUKvaerno5"):
-        with pytest.raises(NotImplementedError, match="dual"):
-            deg.solve(ep, getattr(deg, name)(autodiff=False), deg.EnsembleGPUKernel(), trajectories=2,
-                      save_everystep=False, adaptive=False, dt=0.01)
 
 
 def test_dual_hill_at_zero(deg, torch_cuda):
