@@ -97,7 +97,7 @@ EXPORTS = [
     "deg_last_error", "deg_version", "deg_solve_args_size", "deg_draw_uniform", "deg_graph_begin", "deg_graph_end",
     "deg_graph_launch", "deg_graph_num_nodes", "deg_graph_destroy", "deg_mlp_packed_size", "deg_mlp_partial_cols",
     "deg_mlp_grad", "deg_mlp_adam", "deg_mlp_forward", "deg_mlp_pack", "deg_mlp_infer", "deg_order_by_span",
-    "deg_draw_normal", "deg_start_order",
+    "deg_draw_normal", "deg_start_order", "deg_start_order_noalloc",
 ]
 STEPLOG_COLS = 12
 MLP_NPARAM = 1313
@@ -134,6 +134,7 @@ def _declare(lib):
                                      ctypes.c_int32, ctypes.c_float, ctypes.c_float, vp]
     lib.deg_order_by_span.argtypes = [vp, vp, ctypes.c_float, ctypes.c_float, ctypes.c_int64, vp, vp, vp]
     lib.deg_start_order.argtypes = [vp, ctypes.POINTER(SolveArgs), vp, vp, vp]
+    lib.deg_start_order_noalloc.argtypes = [vp, ctypes.POINTER(SolveArgs), vp]
     lib.deg_draw_normal.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32,
                                     ctypes.c_int32, vp]
     lib.deg_graph_begin.argtypes = [vp]
@@ -153,7 +154,8 @@ def _declare(lib):
                  "deg_kernel_destroy", "deg_kernel_attr", "deg_jit_precompile", "deg_solve", "deg_sync",
                  "deg_draw_uniform", "deg_graph_begin", "deg_graph_end", "deg_graph_launch", "deg_graph_num_nodes",
                  "deg_graph_destroy", "deg_mlp_grad", "deg_mlp_adam", "deg_mlp_forward", "deg_mlp_pack",
-                 "deg_mlp_infer", "deg_order_by_span", "deg_draw_normal", "deg_start_order"):
+                 "deg_mlp_infer", "deg_order_by_span", "deg_draw_normal", "deg_start_order",
+                 "deg_start_order_noalloc"):
         getattr(lib, name).restype = ctypes.c_int
 
 
@@ -245,6 +247,12 @@ class Kernel:
         args.struct_size = ctypes.sizeof(SolveArgs)
         check(lib().deg_start_order(self.handle, ctypes.byref(args), ctypes.c_void_p(key_ptr),
                                     ctypes.c_void_p(order_ptr), ctypes.c_void_p(work_ptr)), "deg_start_order")
+
+    def start_order_noalloc(self, args: SolveArgs, order_ptr: int):
+        """deg_start_order_noalloc: the same order with the library's static device scratch."""
+        args.struct_size = ctypes.sizeof(SolveArgs)
+        check(lib().deg_start_order_noalloc(self.handle, ctypes.byref(args), ctypes.c_void_p(order_ptr)),
+              "deg_start_order_noalloc")
 
 
 _contexts = {}

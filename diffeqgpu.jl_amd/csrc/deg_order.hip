@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 int deg_rng_fail(int code, const char* msg);  // thread-local last error (deghip.cpp)
@@ -98,6 +99,31 @@ __global__ void __launch_bounds__(256) scatter_kernel(const float* t0s, const fl
 }
 
 }  // namespace
+
+// Scratch of deg_start_order_noalloc: the cost keys of up to DEG_NOALLOC_MAX_TRAJ trajectories,
+// then the 2 x 2048 bucket counters of the counting sort (cleared by clear_kernel in every call).
+// One copy per device, held until the process ends. In host code `scratch` names a host-side
+// stand-in: its device address comes from hipGetSymbolAddress (deg_noalloc_scratch).
+static __device__ __align__(256) float scratch[DEG_NOALLOC_MAX_TRAJ + 2 * 2048];
+
+// The device address of `scratch` on `device` (which must be the current device), looked up
+// once per device. Called by deg_ctx_create.
+int deg_noalloc_scratch(int device, void** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* ptrs[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return deg_rng_fail(DEG_E_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptrs[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return deg_rng_fail(DEG_E_HIP, (std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)).c_str());
+        ptrs[device] = ptr;
+    }
+    *out = ptrs[device];
+    return DEG_OK;
+}
 
 extern "C" int deg_order_by_span(const float* t0s, const float* tfs, float t0, float tf, int64_t n, int32_t* order,
                                  int32_t* work, void* stream) {

@@ -260,6 +260,16 @@ int deg_order_by_span(const float *t0s, const float *tfs, float t0, float tf, in
  * the order. key: [n_traj] float, order: [n_traj] int32, device memory. */
 int deg_start_order(deg_kernel *k, const deg_solve_args *a, float *key, int32_t *order, int32_t *work);
 
+/* Allocation-free variant of deg_start_order (no counterpart in the reference; for callers that cannot
+ * allocate device memory: code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The cost keys and the counting sort's counters live in a static device array of the library, one
+ * per device, sized for at most DEG_NOALLOC_MAX_TRAJ trajectories (16 MiB + 16 KiB, held until the
+ * process ends); larger n_traj is rejected with DEG_E_INVALID. Writes the same order as
+ * deg_start_order (same keys, same buckets; order within a bucket is not fixed in either). Calls that
+ * overlap on one device share that array and race: order them on one stream. */
+#define DEG_NOALLOC_MAX_TRAJ 4194304
+int deg_start_order_noalloc(deg_kernel *k, const deg_solve_args *a, int32_t *order);
+
 typedef struct deg_graph deg_graph;
 /* Capture every launch made on `stream` (non-NULL) by this thread between begin and end --
  * deg_draw_uniform, deg_solve, and any other stream work -- into an instantiated HIP graph.
