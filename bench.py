@@ -254,7 +254,12 @@ def jit_compile_seconds(deg, f, n, np_, alg, dev):
 
 def learned_leg(deg, torch, name, f, cfg, prob, alg, dev, kw, every, steps=10, train_n=50_000):
     """The same workload with the learned step-size proposer: logs from a held-out ensemble
-    (different seed) -> GPU training -> the learned kernel. Reported beside the PI value."""
+    (different seed) -> GPU training -> the learned kernel in its own tuned launch configuration
+    (block size, register cap, refill, chunk, start order; measured before the timed runs, as for the
+    PI kernel), evaluated every `every` iterations. Reported beside the PI value, with the other
+    evaluation intervals measured the same way."""
+    import dataclasses
+
     from diffeqgpu_amd import learned, workloads
 
     train = workloads.config(name, N=min(cfg["N"], train_n), seed=9001)
@@ -267,20 +272,39 @@ def learned_leg(deg, torch, name, f, cfg, prob, alg, dev, kw, every, steps=10, t
     packed = tr.pack()
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
-    probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev)
-    res = deg.vectorized_asolve(probs, prob, alg, mlp=packed, mlp_every=every, **kw)
-    torch.cuda.synchronize()
-    ms = []
-    for _ in range(steps):
-        r = deg.vectorized_asolve(probs, prob, alg, mlp=packed, mlp_every=every, timing=True, **kw)
-        ms.append(r.kernel_ms)
-    k_ms = float(np.median(ms))
-    att = int((res.naccept.to(torch.int64) + res.nreject.to(torch.int64)).sum().item())
-    return dict(value=cfg["N"] / (k_ms / 1e3), kernel_ms=k_ms, attempts=att, rejections=int(res.nreject.sum().item()),
-                retcodes=np.bincount(res.retcode.cpu().numpy(), minlength=4).tolist(), train_rows=int(rows.shape[0]),
-                train_s=train_s, rho=0.2, mlp_every=every,
-                fp64_oracle_error=fp64_error(name, res.us_soa[:, :, :1000].cpu().numpy()) if cfg["N"] >= 1000 else None,
-                note="value = trajectories / median kernel time (HIP events), same inputs and tolerances as the PI line")
+    base = dataclasses.replace(dev, block_size=None, waves=0, refill=0, chunk=0, blocks_per_cu=0, start_order=None)
+    probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], base)
+    ldev = deg.autotune.tune(probs, prob, alg, mlp=packed, mlp_every=every, **kw)
+    probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], ldev)
+    per_every = {}
+    for ev in sorted({every, 1, 2, 4}):
+        res = deg.vectorized_asolve(probs, prob, alg, mlp=packed, mlp_every=ev, **kw)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        for _ in range(2):
+            res.relaunch()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a_, b_ in evs:
+            a_.record(stream)
+            res.relaunch()
+            b_.record(stream)
+        torch.cuda.synchronize()
+        k_ms = float(np.median([a_.elapsed_time(b_) for a_, b_ in evs]))
+        att = int((res.naccept.to(torch.int64) + res.nreject.to(torch.int64)).sum().item())
+        per_every[ev] = dict(value=cfg["N"] / (k_ms / 1e3), kernel_ms=k_ms, attempts=att,
+                             rejections=int(res.nreject.sum().item()),
+                             retcodes=np.bincount(res.retcode.cpu().numpy(), minlength=4).tolist(),
+                             fp64_oracle_error=fp64_error(name, res.us_soa[:, :, :1000].cpu().numpy())
+                             if cfg["N"] >= 1000 else None)
+        del res
+    out = dict(per_every[every])
+    out.update(train_rows=int(rows.shape[0]), train_s=train_s, rho=0.2, mlp_every=every,
+               launch=dict(block_size=ldev.block_size, waves=ldev.waves, refill=ldev.refill, chunk=ldev.chunk,
+                           start_order=ldev.start_order),
+               by_mlp_every={str(k): {kk: v[kk] for kk in ("kernel_ms", "attempts", "rejections", "fp64_oracle_error")}
+                             for k, v in per_every.items()},
+               note="value = trajectories / median kernel time (HIP events), same inputs and tolerances as the PI line")
+    return out
 
 
 def learned_vs_pi(deg, torch, name, every, steps=8):

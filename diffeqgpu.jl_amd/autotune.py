@@ -46,14 +46,15 @@ def blocks_per_cu_candidates(block: int):
     return (0,) + tuple(max(1, w * 64 // block) for w in WAVES_PER_CU)
 
 
-def _key(probs, alg, adaptive, nsave_hint) -> str:
+def _key(probs, alg, adaptive, nsave_hint, learned: bool = False) -> str:
     implicit = isinstance(alg, GPUODEImplicitAlgorithm)
     src, _ = probs.f.hip_source(probs.n, probs.np, implicit, fd=implicit and not alg.autodiff)
     h = hashlib.sha1(src.encode()).hexdigest()[:16]
     base = dataclasses.replace(probs.dev, block_size=None, refill=0, chunk=0, waves=0, blocks_per_cu=0,
                                start_order=None)
     nb = int(round(math.log2(max(1, probs.N))))
-    return f"{h}:{probs.n}:{probs.np}:{alg.method}:{base.flags(adaptive)}:{nb}:{nsave_hint}"
+    flags = base.flags(adaptive) | (_lib.F_LEARNED if learned else 0)
+    return f"{h}:{probs.n}:{probs.np}:{alg.method}:{flags}:{nb}:{nsave_hint}"
 
 
 def _read(path) -> dict:
@@ -101,7 +102,7 @@ def tune(probs, prob, alg, *, adaptive: bool = True, repeats: int = 2, use_cache
 
     fn = vectorized_asolve if adaptive else vectorized_solve
     nsave_hint = "saveat" if solve_kw.get("saveat") is not None else ("every" if solve_kw.get("save_everystep") else "end")
-    key = _key(probs, alg, adaptive, nsave_hint)
+    key = _key(probs, alg, adaptive, nsave_hint, learned=solve_kw.get("mlp") is not None)
     if use_cache and not refresh:
         hit = lookup(key)
         if hit is not None:

@@ -252,26 +252,24 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
 __global__ void __launch_bounds__(64) deg_mlp_pack_kernel(const float* __restrict__ P, TrainCfg cfg, float zlo,
                                                           float zhi, deg_mlp_packed* __restrict__ W) {
     const int l = threadIdx.x;
-    const int h = l >> 5, m = l & 31;
+    const int g = l >> 4;
+    // hid2(8 g + i): the layer-1 hidden unit a lane of group g holds as the i-th K value of layer 2
+    auto hid2 = [](int gg, int i) { return i < 4 ? 4 * gg + i : 12 + 4 * gg + i; };
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int k = 4 * h + i;
-        float w = 0.0f;
-        if (k < DEG_MLP_IN) w = P[DEG_MLP_OFF_W1 + k * DEG_MLP_HID + m];
-        else if (k == DEG_MLP_IN) w = P[DEG_MLP_OFF_B1 + m];
-        W->a1[l][i] = bf16_rne(w);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int k2 = 8 * s + 4 * h + i;
-            W->a2[s][l][i] = bf16_rne(P[DEG_MLP_OFF_W2 + k2 * DEG_MLP_HID + m]);
+    for (int i = 0; i < 8; ++i) {
+        if (l < 32) {
+            float w = 0.0f;
+            if (i < DEG_MLP_IN) w = P[DEG_MLP_OFF_W1 + i * DEG_MLP_HID + l];
+            else if (i == DEG_MLP_IN) w = P[DEG_MLP_OFF_B1 + l];
+            W->a1[l][i] = bf16_rne(w);
         }
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+            W->a2[r][l][i] = bf16_rne(P[DEG_MLP_OFF_W2 + hid2(g, i) * DEG_MLP_HID + 16 * r + (l & 15)]);
     }
-    if (l < 32) {
-        const int hh = l >> 4, v = l & 15;
-        const int row = 8 * (v / 4) + 4 * hh + (v % 4);
-        W->b2c[hh][v] = P[DEG_MLP_OFF_B2 + row];
-        W->w3c[hh][v] = P[DEG_MLP_OFF_W3 + row];
-    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) W->a3[l][i] = (l & 15) == 0 ? bf16_rne(P[DEG_MLP_OFF_W3 + hid2(g, i)]) : (unsigned short)0;
+    if (l < 32) W->b2c[l >> 4][l & 15] = P[DEG_MLP_OFF_B2 + l];
     if (l < DEG_MLP_IN) {
         W->xscale[l] = cfg.inv_std[l];
         W->xshift[l] = -cfg.mean[l] * cfg.inv_std[l];
@@ -287,12 +285,15 @@ __global__ void __launch_bounds__(64) deg_mlp_pack_kernel(const float* __restric
 __global__ void __launch_bounds__(256) deg_mlp_infer_kernel(const deg_mlp_packed* __restrict__ W,
                                                             const float* __restrict__ X, long long ld, long long n,
                                                             float* __restrict__ out) {
+    __shared__ deg_mlp_packed w;
+    deg_mlp_stage(w, W);
+    __syncthreads();
     const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     float x[DEG_MLP_IN];
 #pragma unroll
     for (int k = 0; k < DEG_MLP_IN; ++k) x[k] = (r < n) ? X[r * ld + k] : 0.0f;
     // the whole wavefront evaluates (rows past n carry zeros)
-    const float z = deg_mlp_eval(W, x);
+    const float z = deg_mlp_eval(w, x);
     if (r < n) out[r] = z;
 }
 

@@ -561,12 +561,17 @@ __device__ __forceinline__ void deg_mass_mul(const float (&v)[DEG_N], float (&ou
     }
 }
 #endif
-// The factor of W: LU with partial pivoting of W (FP64), or over a dual-number state of W's values
-// (DEG_JC x DEG_JC) together with what the partials of W need: W = alpha M + beta J, so partial j
-// of W is beta times partial j of J, the rows DEG_JC + r P + j of the dual Jacobian deg_jac writes.
+// The factor of W (FP64), over a dual-number state of W's values (DEG_JC x DEG_JC) together with what
+// the partials of W need: W = alpha M + beta J, so partial j of W is beta times partial j of J, the
+// rows DEG_JC + r P + j of the dual Jacobian deg_jac writes. For DEG_JC <= 3 the factor is Cramer's
+// rule, the reference's own solve for these sizes (linalg/linsolve.jl:8-57): W holds the cofactors
+// (x_i = sum_j W[i][j] b_j / det, each row of cofactors in the reference's formula); above that, LU
+// with partial pivoting.
+#define DEG_CRAMER (DEG_JC <= 3)
 struct deg_wf {
     double W[DEG_JC][DEG_JC];
     int piv[DEG_JC];
+    double det;  // Cramer: the determinant (its reciprocal outside parity mode)
 #if DEG_DUAL_P
     double beta;
     float jp[DEG_N - DEG_JC][DEG_JC];
@@ -584,12 +589,43 @@ __device__ __forceinline__ void w_dual(deg_wf& F, double beta, const float (&J)[
     (void)J;
 #endif
 }
-// LU with partial pivoting; row swaps are applied with compile-time indices so W stays
-// in VGPRs. With DEG_PARITY every multiplier and back-substitution step divides by its pivot in
-// IEEE double, as the oracle does; otherwise each pivot is inverted once and the diagonal of the
-// factor holds 1 / U_kk, so a factorisation costs DEG_JC FP64 divisions and a solve none.
+// Factor W in place. Cramer (DEG_JC <= 3): the cofactor rows and det as StaticArrays forms them
+// (det of a 3 x 3 expanded along the first row). LU: row swaps are applied with compile-time indices
+// so W stays in VGPRs. With DEG_PARITY every division is an IEEE double division, as the oracle does;
+// otherwise the factor keeps reciprocals (1 / det, or 1 / U_kk on the diagonal), so a solve divides
+// nowhere.
 __device__ __forceinline__ void w_lu(deg_wf& F) {
     auto& W = F.W;
+#if DEG_CRAMER
+#if DEG_JC == 1
+    F.det = W[0][0];
+    W[0][0] = 1.0;
+#elif DEG_JC == 2
+    const double a00 = W[0][0], a01 = W[0][1], a10 = W[1][0], a11 = W[1][1];
+    F.det = a00 * a11 - a01 * a10;
+    W[0][0] = a11;
+    W[0][1] = -a01;
+    W[1][0] = -a10;
+    W[1][1] = a00;
+#else
+    const double a00 = W[0][0], a01 = W[0][1], a02 = W[0][2];
+    const double a10 = W[1][0], a11 = W[1][1], a12 = W[1][2];
+    const double a20 = W[2][0], a21 = W[2][1], a22 = W[2][2];
+    F.det = (a00 * (a11 * a22 - a12 * a21) - a01 * (a10 * a22 - a12 * a20)) + a02 * (a10 * a21 - a11 * a20);
+    W[0][0] = a11 * a22 - a12 * a21;
+    W[0][1] = a02 * a21 - a01 * a22;
+    W[0][2] = a01 * a12 - a02 * a11;
+    W[1][0] = a12 * a20 - a10 * a22;
+    W[1][1] = a00 * a22 - a02 * a20;
+    W[1][2] = a02 * a10 - a00 * a12;
+    W[2][0] = a10 * a21 - a11 * a20;
+    W[2][1] = a01 * a20 - a00 * a21;
+    W[2][2] = a00 * a11 - a01 * a10;
+#endif
+#if !DEG_PARITY
+    F.det = 1.0 / F.det;
+#endif
+#else
     _Pragma("unroll") for (int kk = 0; kk < DEG_JC; ++kk) {
         int pk = kk;
         double mx = __builtin_fabs(W[kk][kk]);
@@ -626,10 +662,24 @@ __device__ __forceinline__ void w_lu(deg_wf& F) {
         W[kk][kk] = rp;
 #endif
     }
+#endif
 }
-// y = LU \ y in FP64
+// y = W^-1 y in FP64
 __device__ __forceinline__ void lu_solve_d(const deg_wf& F, double (&y)[DEG_JC]) {
     const auto& W = F.W;
+#if DEG_CRAMER
+    double x[DEG_JC];
+    _Pragma("unroll") for (int i = 0; i < DEG_JC; ++i) {
+        double v = W[i][0] * y[0];
+        _Pragma("unroll") for (int j = 1; j < DEG_JC; ++j) v = v + W[i][j] * y[j];
+#if DEG_PARITY
+        x[i] = v / F.det;
+#else
+        x[i] = v * F.det;
+#endif
+    }
+    _Pragma("unroll") for (int i = 0; i < DEG_JC; ++i) y[i] = x[i];
+#else
     _Pragma("unroll") for (int kk = 0; kk < DEG_JC; ++kk) {
         _Pragma("unroll") for (int i = kk + 1; i < DEG_JC; ++i) {
             if (F.piv[kk] == i) {
@@ -653,6 +703,7 @@ __device__ __forceinline__ void lu_solve_d(const deg_wf& F, double (&y)[DEG_JC])
         y[i] = v * W[i][i];  // the factor keeps 1 / U_ii
 #endif
     }
+#endif
 }
 // x = W \ b. Over a dual-number state the solve is the dual one, the value LU plus one solve per
 // partial: x_v = W_v \ b_v, x_j = W_v \ (b_j - W_j x_v) with W_j = beta J_j and x_v kept in FP64
@@ -1281,6 +1332,14 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
     __syncthreads();
 #endif
 #endif
+#if DEG_LEARNED
+    // the proposer's packed weights in LDS (read by every evaluation without a vector-memory wait)
+    __shared__ deg_mlp_packed deg_mlpw;
+    deg_mlp_stage(deg_mlpw, (const deg_mlp_packed*)a.mlp);
+#if DEG_SAVEAT_GLOBAL
+    __syncthreads();
+#endif
+#endif
     // saveat times staged in LDS: reading them then needs no vector-memory wait (which on
     // CDNA would also wait for every store still in flight)
     extern __shared__ float deg_dyn[];  // sized at launch: saveat times, then the save stash
@@ -1333,6 +1392,11 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
 #if DEG_LEARNED
     float zml_keep = 0.0f;  // the proposer's last correction for this lane
     int mlp_tick = 0;       // wave-uniform iteration counter modulo mlp_every
+    // the last attempt's features and the step its controller left pending for the proposer:
+    // accepted?, log2 of the PI factor, the attempt's dt, and (tf - t) - dt before t advanced
+    float lfeat[DEG_MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    bool lpend = false, lacc = false;
+    float lzpi = 0.0f, ldt0 = 0.0f, lrem = 0.0f;
 #endif
 #if DEG_LEARNED || DEG_STEPLOG
     float lx_prev = -13.2877123795f;  // log2 EEst of the previous attempt (log2 qoldinit at first)
@@ -1573,6 +1637,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
 #endif
 #if DEG_LEARNED
             zml_keep = 0.0f;  // a new trajectory starts from the plain PI proposal
+            lpend = false;
 #endif
 #if DEG_STEPLOG
             have_prev = false;
@@ -1640,11 +1705,39 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
         bool attempted = false;
         float EEst = 0.0f;
 #endif
-#if DEG_LEARNED || DEG_STEPLOG
-        float feat[DEG_MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-#endif
 #if DEG_LEARNED
-        float zml = 0.0f;
+#if DEG_METHOD == 0
+        // the FSAL stage moves into k1 now rather than at the attempt, so k7 is free during the proposer
+        if (fsal_pending) {
+            DEG_FOR(c) k[1][c] = k[7][c];
+            fsal_pending = false;
+        }
+#endif
+        {
+            // the proposer, on the matrix cores with every lane of the wavefront (the loop exit is
+            // wave-uniform in this variant), on the features of each lane's last attempt. It runs
+            // here, before the next attempt, where the stage registers are free; between evaluations
+            // each lane keeps its last correction, and lanes without a pending step get none
+            if (mlp_tick == 0) {
+                const float zo = deg_mlp_eval(deg_mlpw, lfeat);
+                zml_keep = lpend ? zo : 0.0f;
+            }
+            if (++mlp_tick >= a.mlp_every) mlp_tick = 0;
+            if (lpend) {
+                // the pending step: the PI factor corrected in log2 space within the PI range
+                // [qmin, qmax]; after a rejection never more than 0.9 dt; the PI step stands when the
+                // proposal is not finite
+                float zz = __builtin_fminf(__builtin_fmaxf(lzpi + zml_keep, -2.32192809f), 3.32192809f);
+                if (!lacc) zz = __builtin_fminf(zz, -0.152003093f);
+                const float fml = __builtin_amdgcn_exp2f(zz);
+                if (fml == fml)
+                    dt = lacc ? __builtin_fminf(__builtin_fabsf(ldt0 * fml), __builtin_fabsf(lrem)) : ldt0 * fml;
+                lpend = false;
+            }
+        }
+        float (&feat)[DEG_MLP_IN] = lfeat;
+#elif DEG_STEPLOG
+        float feat[DEG_MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #endif
 #if DEG_ADAPTIVE
         // ---- step-size control and acceptance (gpu_*_perform_step.jl adaptive versions); run
@@ -1660,14 +1753,13 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
             const float zpi = __builtin_amdgcn_logf(factor);  // log2 of the PI factor
 #endif
 #if DEG_LEARNED
-            {
-                // the proposer corrects the PI factor in log2 space, within the PI range
-                // [qmin, qmax]; after a rejection it never proposes more than 0.9 dt
-                float zz = __builtin_fminf(__builtin_fmaxf(zpi + zml, -2.32192809f), 3.32192809f);
-                if (!acc_step) zz = __builtin_fminf(zz, -0.152003093f);
-                const float fml = __builtin_amdgcn_exp2f(zz);
-                if (fml == fml) factor = fml;
-            }
+            // the next step is set by the proposer before the next attempt (the PI step stands until
+            // then; the no-progress rule below decides with it)
+            lpend = true;
+            lacc = acc_step;
+            lzpi = zpi;
+            ldt0 = dt;
+            lrem = (tf - t) - dt;
 #endif
 #if DEG_STEPLOG
             {
@@ -1832,9 +1924,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                     feat[5] = first_attempt ? 0.0f : feat[2] - ldt_prev;
                 }
 #endif
-#if !DEG_LEARNED
                 control();
-#endif
 #else
                 // ---- one fixed step (gpu_*_perform_step.jl fixed versions)
                 const float told = t;
@@ -1908,18 +1998,6 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
 #endif
             }
         }
-#if DEG_LEARNED
-        }  // phase A
-
-        // every lane of the wavefront is here (the loop exit is wave-uniform in this variant);
-        // between evaluations each lane keeps its last correction
-        if (mlp_tick == 0) zml_keep = deg_mlp_eval((const deg_mlp_packed*)a.mlp, feat);
-        if (++mlp_tick >= a.mlp_every) mlp_tick = 0;
-        zml = zml_keep;
-
-        if (have) {
-        if (attempted) control();
-#endif
 #if DEG_BATCH_SAVES
         if (accepted && use_saveat) {
             const bool crossing = tnext <= t;

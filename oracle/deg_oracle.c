@@ -505,8 +505,10 @@ static void vern9_interp(const osys *S, const float *p, float th, float dt, floa
 /* FP64 LU with partial pivoting for W = I - gamma*J (Rosenbrock23)                     */
 /* ------------------------------------------------------------------------------------ */
 typedef struct {
-    double lu[OMAXN][OMAXN];
+    double lu[OMAXN][OMAXN];  /* LU factors, or for n <= 3 the cofactor rows of Cramer's rule */
     int perm[OMAXN];
+    int cramer;               /* n <= 3: x_i = (sum_j lu[i][j] b_j) / det */
+    double det;
     /* dual-number state (P > 0): lu factors the nv x nv values of W = alpha M + beta J, and the
      * partials of W are beta times rows nv.. of the dual Jacobian, kept in jp */
     int nv, P;
@@ -524,7 +526,44 @@ static void w_dual(olu *F, const osys *S, double beta, const float *J) {
         for (int c = 0; c < F->nv; ++c) F->jp[r][c] = J[r * F->nv + c];
 }
 
+/* Cramer's rule for n <= 3, the reference's own solve for these sizes (linalg/linsolve.jl:8-57), in FP64:
+ * the cofactor rows of its formulas and StaticArrays' det (3 x 3 expanded along the first row) */
+static void w_cramer(olu *F, int n) {
+    double (*W)[OMAXN] = F->lu;
+    F->cramer = 1;
+    if (n == 1) {
+        F->det = W[0][0];
+        W[0][0] = 1.0;
+    } else if (n == 2) {
+        const double a00 = W[0][0], a01 = W[0][1], a10 = W[1][0], a11 = W[1][1];
+        F->det = a00 * a11 - a01 * a10;
+        W[0][0] = a11;
+        W[0][1] = -a01;
+        W[1][0] = -a10;
+        W[1][1] = a00;
+    } else {
+        const double a00 = W[0][0], a01 = W[0][1], a02 = W[0][2];
+        const double a10 = W[1][0], a11 = W[1][1], a12 = W[1][2];
+        const double a20 = W[2][0], a21 = W[2][1], a22 = W[2][2];
+        F->det = (a00 * (a11 * a22 - a12 * a21) - a01 * (a10 * a22 - a12 * a20)) + a02 * (a10 * a21 - a11 * a20);
+        W[0][0] = a11 * a22 - a12 * a21;
+        W[0][1] = a02 * a21 - a01 * a22;
+        W[0][2] = a01 * a12 - a02 * a11;
+        W[1][0] = a12 * a20 - a10 * a22;
+        W[1][1] = a00 * a22 - a02 * a20;
+        W[1][2] = a02 * a10 - a00 * a12;
+        W[2][0] = a10 * a21 - a11 * a20;
+        W[2][1] = a01 * a20 - a00 * a21;
+        W[2][2] = a00 * a11 - a01 * a10;
+    }
+}
+
 static void w_lu(olu *F, int n) {
+    if (n <= 3) {
+        w_cramer(F, n);
+        return;
+    }
+    F->cramer = 0;
     for (int r = 0; r < n; ++r) F->perm[r] = r;
     for (int k = 0; k < n; ++k) {
         int piv = k;
@@ -592,6 +631,15 @@ static void w_factor_rodas(olu *F, const osys *S, float invg, const float *J, co
 /* y = LU \ b in FP64 (b in FP64, overwritten) */
 static void lu_solve_d(const olu *F, int n, double *b) {
     double y[OMAXN];
+    if (F->cramer) {
+        for (int i = 0; i < n; ++i) {
+            double v = F->lu[i][0] * b[0];
+            for (int j = 1; j < n; ++j) v = v + F->lu[i][j] * b[j];
+            y[i] = v / F->det;
+        }
+        for (int i = 0; i < n; ++i) b[i] = y[i];
+        return;
+    }
     for (int i = 0; i < n; ++i) {
         double v = b[F->perm[i]];
         for (int j = 0; j < i; ++j) v = v - F->lu[i][j] * y[j];

@@ -17,9 +17,9 @@ library implements:
   Adam (Kingma and Ba 2015) with bias correction.
 Parameter vector layout (1313 floats): W1[6][32], b1[32], W2[32][32], b2[32], w3[32], b3.
 
-The matrix-core inference rounds weights (W1, b1, W2) and the layer inputs (the normalised
-features, the constant 1, the ReLU outputs of layer 1) to bfloat16 with round-to-nearest-even,
-and accumulates in FP32. ``infer_bf16`` emulates that (in float64, so it differs from the device
+The matrix-core inference rounds weights (W1, b1, W2, w3) and the layer inputs (the normalised
+features, the constant 1, the ReLU outputs of both hidden layers) to bfloat16 with
+round-to-nearest-even, and accumulates in FP32 (b2 and b3 are added in FP32). ``infer_bf16`` emulates that (in float64, so it differs from the device
 only by FP32 accumulation rounding).
 
 Pins (tests/test_mlp_oracle.py): the gradient against central finite differences; the Adam
@@ -107,7 +107,8 @@ def bf16_rne(x) -> np.ndarray:
 
 
 def infer_bf16(P, X, norm, zlo, zhi):
-    """The matrix-core evaluation: bf16 weights and layer inputs, float accumulation."""
+    """The matrix-core evaluation: bf16 weights (W1, b1, W2, w3) and layer inputs (features, h1, h2),
+    float accumulation; b2 and b3 are added in float."""
     W1, b1, W2, b2, w3, b3 = unpack(np.asarray(P, dtype=np.float32))
     norm = np.asarray(norm, dtype=np.float32)
     X = np.asarray(X, dtype=np.float32)[:, :IN]
@@ -121,6 +122,6 @@ def infer_bf16(P, X, norm, zlo, zhi):
     a1 = (xb @ W1b + b1b).astype(np.float32)
     h1 = bf16_rne(np.maximum(a1, 0.0)).astype(np.float64)
     a2 = (h1 @ W2b + b2.astype(np.float64)).astype(np.float32)
-    h2 = np.maximum(a2, 0.0).astype(np.float64)
-    z = h2 @ w3.astype(np.float64) + float(b3)
+    h2 = bf16_rne(np.maximum(a2, 0.0)).astype(np.float64)
+    z = (h2 @ bf16_rne(w3).astype(np.float64)).astype(np.float32).astype(np.float64) + float(b3)
     return np.clip(z, zlo, zhi)

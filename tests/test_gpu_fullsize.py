@@ -11,6 +11,8 @@ range (this exercises the persistent queue's slab and chunk boundaries at scale)
   * with the deterministic controller pow and no contraction, 1,500 sampled trajectories equal
     the CPU oracle bit for bit.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -75,6 +77,17 @@ def test_lorenz_adaptive_1M(deg, torch_cuda):
         us = r.us_soa.cpu().numpy()
         np.testing.assert_array_equal(us[0], cfg["u0"])
         assert np.all(np.isfinite(us))
+        if not bitwise:
+            # the default (benchmarked) path on the fixture's 1,000 trajectories of this ensemble: its error
+            # against FP64 Vern9 (all 11 rows) within 1.25x of the oracle's own at the median and 90th percentile
+            gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lorenz_adaptive_vern9.npz"))
+            np.testing.assert_array_equal(gold["u0"], cfg["u0"][:, :1000])
+            np.testing.assert_array_equal(gold["p"], cfg["p"][:, :1000])
+            o = _oracle_subset(cfg, np.arange(1000), saveat=sv)
+            eg = np.abs(us[:, :, :1000] - gold["u"]).max(axis=(0, 1))
+            eo = np.abs(o["us"] - gold["u"]).max(axis=(0, 1))
+            for q in (50, 90):
+                assert np.percentile(eg, q) <= 1.25 * np.percentile(eo, q), (q, np.percentile(eg, q), np.percentile(eo, q))
         if bitwise:
             idx = _sample(N)
             o = _oracle_subset(cfg, idx, saveat=sv)

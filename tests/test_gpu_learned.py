@@ -10,9 +10,10 @@ Bars:
     and rare bf16 rounding ties differ);
   * step logs: one row per attempt with a successor; within one trajectory the rows chain
     (x1 of row j+1 is x0 of row j, x2 + col6 gives the next log2 dt, col7 is the next x0);
-  * the learned solve: every trajectory succeeds and the FP64-oracle error stays within a
-    stated factor of the PI controller's. The test prints what the trained proposer gains
-    over the PI controller (attempts, rejections, kernel time) on held-out trajectories.
+  * the learned solve: every trajectory succeeds; trained for a target error ratio of 0.05 the
+    proposer removes at least 80 % of the PI controller's rejections on held-out trajectories, at a
+    99th-percentile FP64 error no higher than PI's and a median within 10 % (the Float32 floor); the
+    throughput target (1.5x PI at equal error) is an expected failure until it is met.
 """
 import numpy as np
 import pytest
@@ -153,51 +154,77 @@ def test_steplog_rows_chain(deg, torch_cuda):
     np.testing.assert_array_equal(res_c.naccept.cpu().numpy(), res.naccept.cpu().numpy())
 
 
-def test_learned_solve_vs_pi(deg, torch_cuda):
-    """Train on the solver's own step logs (VdP, 8,192 trajectories), then solve 16,384 held-out
-    trajectories with the PI controller and with the proposer."""
-    torch = torch_cuda
+def _train_and_compare(deg, torch, rho, every, n_test=16384):
+    """Train on the solver's own step logs (VdP, 8,192 trajectories), then solve held-out
+    trajectories with the PI controller and with the proposer (target error ratio rho, evaluated every
+    `every` iterations); FP64 errors on all of them against Vern9 at 1e-12."""
     from diffeqgpu_amd import learned, systems
 
     f, n, np_, _ = systems.SYSTEMS["vdp"]
     train = _vdp(8192, 1001)
-    test = _vdp(16384, 505)
+    test = _vdp(n_test, 505)
     dev = deg.HIPDevice()
     kw = dict(dt=0.01, abstol=1e-6, reltol=1e-6)
     prob = deg.ODEProblem(f, test["u0"][:, 0], test["tspan"], test["p"][:, 0])
-    print("[gpu] learned: collect logs", flush=True)
+    print(f"[gpu] learned rho={rho}: collect logs", flush=True)
     probs_tr = deg.HIPProbVector.from_arrays(f, train["u0"], train["p"], train["tspan"], dev)
     rows, _ = learned.collect_steplog(probs_tr, prob, deg.GPUTsit5(), 4_000_000, **kw)
     assert rows.shape[0] > 1_000_000
-    tr = learned.ProposerTrainer(rows, expo=0.2, rho=0.5, seed=0)
+    tr = learned.ProposerTrainer(rows, expo=0.2, rho=rho, seed=0)
     loss0 = tr.evaluate()
-    print("[gpu] learned: train", flush=True)
+    print(f"[gpu] learned rho={rho}: train", flush=True)
     tr.train(600, batch=16384, lr=3e-3)
     loss1 = tr.evaluate()
-    assert loss1 < 0.5 * loss0, (loss0, loss1)
     packed = tr.pack()
-
     probs = deg.HIPProbVector.from_arrays(f, test["u0"], test["p"], test["tspan"], dev)
     out = {}
     for name, mlp in (("pi", None), ("learned", packed)):
-        res = deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), mlp=mlp, **kw)
+        res = deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), mlp=mlp, mlp_every=every, **kw)
         torch.cuda.synchronize()
         ms = []
         for _ in range(3):
-            r2 = deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), mlp=mlp, timing=True, **kw)
+            r2 = deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), mlp=mlp, mlp_every=every, timing=True, **kw)
             ms.append(r2.kernel_ms)
         rc = res.retcode.cpu().numpy()
         assert np.all(rc == 0), np.bincount(rc)
         out[name] = dict(acc=int(res.naccept.sum().item()), rej=int(res.nreject.sum().item()), ms=min(ms),
-                         u=res.us_soa[-1, :, :300].cpu().numpy().astype(np.float64))
-    ref, st, _ = pyoracle.vern9("vdp", test["u0"][:, :300], test["p"][:, :300], test["tspan"])
+                         u=res.us_soa[-1].cpu().numpy().astype(np.float64))
+    ref, st, _ = pyoracle.vern9("vdp", test["u0"], test["p"], test["tspan"])
+    assert np.all(st == 0)
     for name in out:
         e = np.abs(out[name]["u"] - ref[0]).max(axis=0)
         out[name]["err_median"] = float(np.median(e))
         out[name]["err_p99"] = float(np.percentile(e, 99))
     pi, ml = out["pi"], out["learned"]
-    print(f"[learned vs PI] attempts {ml['acc'] + ml['rej']} vs {pi['acc'] + pi['rej']}, rejections {ml['rej']} vs "
-          f"{pi['rej']}, kernel {ml['ms']:.3f} ms vs {pi['ms']:.3f} ms, FP64 error median {ml['err_median']:.3g} vs "
-          f"{pi['err_median']:.3g}, p99 {ml['err_p99']:.3g} vs {pi['err_p99']:.3g}; training loss {loss0:.3f} -> "
-          f"{loss1:.3f}", flush=True)
-    assert ml["err_median"] <= 4.0 * pi["err_median"] + 1e-6
+    print(f"[learned rho={rho} every={every} vs PI] attempts {ml['acc'] + ml['rej']} vs {pi['acc'] + pi['rej']}, "
+          f"rejections {ml['rej']} vs {pi['rej']}, kernel {ml['ms']:.3f} ms vs {pi['ms']:.3f} ms, FP64 error median "
+          f"{ml['err_median']:.3g} vs {pi['err_median']:.3g}, p99 {ml['err_p99']:.3g} vs {pi['err_p99']:.3g}; "
+          f"training loss {loss0:.3f} -> {loss1:.3f}", flush=True)
+    assert loss1 < 0.5 * loss0, (loss0, loss1)
+    return pi, ml
+
+
+def test_learned_removes_rejections(deg, torch_cuda):
+    """What the trained proposer gains, stated and asserted: trained for a target error ratio of 0.05
+    and evaluated on every attempt, it removes at least 80 % of the PI controller's rejected attempts on
+    16,384 held-out Van der Pol trajectories (mu ~ logU(0.1, 30), Tsit5 at 1e-6), with a 99th-percentile
+    FP64 error (over all of them) no higher than the PI controller's and a median within 10 % of it. The
+    median error at this tolerance sits at the Float32 rounding floor of the state (about 8e-6 for
+    either controller, whatever the step sequence), where runs differ by a few per cent: the proposer's
+    training sees its step-log rows in the order the solver's atomics wrote them."""
+    pi, ml = _train_and_compare(deg, torch_cuda, rho=0.05, every=1)
+    assert ml["rej"] <= 0.2 * pi["rej"], (ml["rej"], pi["rej"])
+    assert ml["err_p99"] <= pi["err_p99"], (ml["err_p99"], pi["err_p99"])
+    assert ml["err_median"] <= 1.10 * pi["err_median"], (ml["err_median"], pi["err_median"])
+
+
+@pytest.mark.xfail(reason="the throughput target (>= 1.5x the PI kernel's trajectories/s at equal or lower FP64 "
+                          "error) is not met: the PI controller rejects about 3 % of attempts here and the proposer's "
+                          "evaluation costs more than the attempts it saves", strict=False)
+def test_learned_throughput_target(deg, torch_cuda):
+    """The north star's target for the proposer: at least 1.5x the PI kernel's trajectories/s (kernel
+    time) on held-out Van der Pol trajectories at equal or lower median and 99th-percentile FP64 error
+    (the bench's setting: target ratio 0.2, evaluated every 2nd iteration)."""
+    pi, ml = _train_and_compare(deg, torch_cuda, rho=0.2, every=2)
+    assert ml["err_median"] <= pi["err_median"] and ml["err_p99"] <= pi["err_p99"]
+    assert pi["ms"] >= 1.5 * ml["ms"], (ml["ms"], pi["ms"])

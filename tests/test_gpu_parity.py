@@ -217,12 +217,16 @@ def test_adaptive_powf_fp64_error(deg, torch_cuda, system, method, tspan, tol):
 # default fast path against the FP64 oracle
 # ---------------------------------------------------------------------------------------
 def test_fast_path_fp64_error_lorenz_adaptive(deg, torch_cuda):
-    """The benchmark's adaptive Lorenz workload (saveat 0:1:10, tol 1e-6) on a 1,000-trajectory
-    subset: endpoint error against FP64 Vern9 at 1e-12 is within 2x of the oracle's own Float32
-    error (median and 90th percentile), and every trajectory succeeds."""
+    """The benchmark's adaptive Lorenz workload (saveat 0:1:10, tol 1e-6) on 10,000 trajectories:
+    the error against FP64 Vern9 at 1e-12, the largest over the 11 save rows and 3 states of each
+    trajectory (so the dense output at interior save points counts), is within 1.25x of the oracle's
+    own Float32 error on the same trajectories at the median and the 90th percentile. The default path
+    differs from the oracle only by FMA contraction and the log2/exp2 controller, whose rounding moves
+    step sequences, not error levels; a wrong dense-output coefficient or a lost order would miss by
+    far more. Every trajectory succeeds and the saved times are the grid."""
     from diffeqgpu_amd import workloads
 
-    cfg = workloads.config("lorenz_adaptive", N=1000, seed=202)
+    cfg = workloads.config("lorenz_adaptive", N=10_000, seed=202)
     sv = np.arange(11, dtype=F32)
     ref, st, _ = pyoracle.vern9("lorenz", cfg["u0"], cfg["p"], (0.0, 10.0), saveat=sv.astype(np.float64))
     assert np.all(st == 0)
@@ -232,10 +236,10 @@ def test_fast_path_fp64_error_lorenz_adaptive(deg, torch_cuda):
                        saveat=sv)
     assert np.all(g["retcode"] == 0)
     np.testing.assert_array_equal(g["ts"], o["ts"])
-    eg = np.abs(g["us"][-1] - ref[-1]).max(axis=0)
-    eo = np.abs(o["us"][-1] - ref[-1]).max(axis=0)
-    assert np.median(eg) <= 2 * np.median(eo) + 1e-6
-    assert np.percentile(eg, 90) <= 2 * np.percentile(eo, 90) + 1e-6
+    eg = np.abs(g["us"] - ref).max(axis=(0, 1))
+    eo = np.abs(o["us"] - ref).max(axis=(0, 1))
+    for q in (50, 90):
+        assert np.percentile(eg, q) <= 1.25 * np.percentile(eo, q), (q, np.percentile(eg, q), np.percentile(eo, q))
 
 
 @pytest.mark.parametrize("method", ["rb23", "rodas4", "rodas5p", "kvaerno3", "kvaerno5"])
