@@ -36,6 +36,10 @@ Besides ``value`` (inputs and outputs resident in HBM) the line reports the end-
 through the C ABI from host arrays (SoA pack, H2D, solve, D2H of ts/us/retcode) and the cold
 hiprtc compile time of the workload's solver; neither is part of ``value``.
 
+Persistent adaptive workloads (not the sweeps) fit the start-order cost model (costmodel.py) on a
+200,000-trajectory held-out draw before the timed region, and start in its order when that measures
+faster than the tuned start order.
+
 Options: --gpus N (1 only: one GPU), --steps K, --warmup W, --workload, --n-traj, --no-autotune,
 --copy-probe (one 256 MiB device copy first, to calibrate HBM counters under rocprofv3),
 --no-cpu (skip the CPU baseline), --no-extras (skip the end-to-end and compile timings).
@@ -105,11 +109,17 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the end-to-end and JIT-compile timings")
     ap.add_argument("--no-order", action="store_true", help="lorenz_uneven: ascending ids instead of longest first")
-    ap.add_argument("--start-order", default="auto", choices=["auto", "cost", "none", "span"],
+    ap.add_argument("--start-order", default="auto", choices=["auto", "cost", "model", "none", "span"],
                     help="persistent grids: the tuned choice (auto), descending span / initial step estimate (cost), "
-                         "index order (none), or (lorenz_uneven) descending span (span)")
+                         "descending attempt count predicted by the fitted cost model (model), index order (none), "
+                         "or (lorenz_uneven) descending span (span)")
     ap.add_argument("--retune", action="store_true", help="measure the launch configuration again")
+    ap.add_argument("--no-cost-model", action="store_true",
+                    help="do not fit the start-order cost model on a held-out draw (no start_order='model')")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposer leg")
+    ap.add_argument("--order-study", action="store_true",
+                    help="also time the solve under other start orders (index, span / h0 key, cost model, tiles of "
+                         "16 consecutive ids, and the true attempt counts as a bound); adds 'start_orders'")
     ap.add_argument("--mlp-every", type=int, default=2, help="learned leg: evaluate the proposer every k iterations")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
@@ -134,6 +144,94 @@ def flop_model(method, n, fc):
     if method == "GPUVern7":
         return 116 * n + 24 + 10 * F, 0, 0
     return 25 * n + 8 + 3 * F + FJ + FT, n * n + n + (2 * n ** 3) // 3 + 6 * n * n, 4 * n + 8
+
+
+def faster_start(deg, torch, f, cfg, prob, alg, dev, kw, reps=5):
+    """The tuned device, or the same with start_order="model" when that is faster (median of reps
+    relaunches each, HIP events; outside the timed region)."""
+    best, best_ms = dev, None
+    for d in (dev, dataclasses.replace(dev, start_order="model")):
+        probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], d, tfs=cfg.get("tfs"))
+        res = deg.vectorized_asolve(probs, prob, alg, **kw)
+        st = torch.cuda.current_stream()
+        ms = []
+        for _ in range(reps):
+            a_, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a_.record(st)
+            res.relaunch()
+            b_.record(st)
+            b_.synchronize()
+            ms.append(a_.elapsed_time(b_))
+        m = float(np.median(ms))
+        if best_ms is None or m < best_ms:
+            best, best_ms = d, m
+        del res, probs
+    return best
+
+
+def _tile_order(tile_key, N, T=16, Q=8):
+    """Start order made of tiles of T consecutive trajectory ids, tiles in descending tile_key, dealt
+    round-robin over the Q queues so that each claimed chunk of T entries is one whole tile (queue q
+    serves order entries q, q + Q, q + 2Q, ...). N must be a multiple of T * Q."""
+    rank = np.argsort(-np.asarray(tile_key), kind="stable")
+    order = np.empty(N, np.int32)
+    j = np.arange(T)
+    for r, tile in enumerate(rank):
+        q, m = r % Q, r // Q
+        order[(T * m + j) * Q + q] = tile * T + j
+    return order
+
+
+def order_study(deg, torch, f, cfg, prob, alg, dev, kw, reps=7):
+    """Median kernel time (HIP events, reps relaunches, each including its ordering kernels) of the same
+    solve under different start orders, with the tuned launch, on the workload's first N - N % 128
+    trajectories (whole tiles of 16 ids for each of the 8 queues):
+      index        ids in order;
+      cost_key     descending span / h0 (deg_start_order);
+      cost_model   descending predicted attempts (deg_start_order_model), when a model is fitted;
+      tiles_by_key tiles of 16 consecutive ids -- what coalesced save-row writes would need -- in
+                   descending order of the tile's largest span / h0 key;
+      tiles_by_attempts, attempts
+                   the same tiles, or single trajectories, in descending order of their true attempt
+                   counts from an earlier solve: bounds, since the counts are known only after solving.
+    Also the Spearman rank correlation of each key with the attempt counts."""
+    N = cfg["N"] - cfg["N"] % 128
+    u0, p = cfg["u0"][:, :N], cfg["p"][:, :N]
+    base = dataclasses.replace(dev, start_order="none")
+    st = torch.cuda.current_stream()
+
+    def timed(d, order=None):
+        probs = deg.HIPProbVector.from_arrays(f, u0, p, cfg["tspan"], d)
+        o = None if order is None else torch.as_tensor(order, device="cuda")
+        res = deg.vectorized_asolve(probs, prob, alg, order=o, **kw)
+        ms = []
+        for _ in range(reps):
+            a_, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a_.record(st)
+            res.relaunch()
+            b_.record(st)
+            b_.synchronize()
+            ms.append(a_.elapsed_time(b_))
+        return res, float(np.median(ms))
+
+    def spearman(x, y):
+        rx, ry = np.argsort(np.argsort(x)), np.argsort(np.argsort(y))
+        return float(np.corrcoef(rx, ry)[0, 1])
+
+    out = {"trajectories": N}
+    res, out["index"] = timed(base)
+    att = (res.naccept.to(torch.int64) + res.nreject.to(torch.int64)).cpu().numpy()
+    res, out["cost_key"] = timed(dataclasses.replace(base, start_order="cost"))
+    key = res.order_key.cpu().numpy().astype(np.float64)
+    out["spearman_cost_key"] = spearman(key, att)
+    if dev.cost_model:
+        res, out["cost_model"] = timed(dataclasses.replace(base, start_order="model"))
+        out["spearman_cost_model"] = spearman(res.order_key.cpu().numpy().astype(np.float64), att)
+    _, out["tiles_by_key"] = timed(base, _tile_order(key.reshape(-1, 16).max(axis=1), N))
+    _, out["tiles_by_attempts"] = timed(base, _tile_order(att.reshape(-1, 16).max(axis=1), N))
+    _, out["attempts"] = timed(base, np.argsort(-att, kind="stable").astype(np.int32))
+    out["unit"] = "ms (median kernel time)"
+    return out
 
 
 def solve_kw(cfg):
@@ -510,7 +608,11 @@ def main():
     cfg = workloads.config(name, N=args.n_traj or None)
     N = cfg["N"]
     f, n, np_, _ = systems.SYSTEMS[cfg["system"]]
-    start = {"auto": None, "cost": "cost"}.get(args.start_order, "none")
+    start = {"auto": None, "cost": "cost", "model": "model"}.get(args.start_order, "none")
+    if start == "model" and (args.no_cost_model or not args.persistent or name == "lorenz_uneven"):
+        print("--start-order model needs the fitted cost model (persistent grids, not --no-cost-model or "
+              "lorenz_uneven)", file=sys.stderr)
+        sys.exit(2)
     if args.no_order:
         start = "none"
     dev = deg.HIPDevice(0, contract=args.contract, powmode=args.powmode, persistent=bool(args.persistent),
@@ -519,12 +621,27 @@ def main():
     alg = getattr(deg, spec["alg"])()
     prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
     kw = solve_kw(cfg)
+    cost_model = None
+    if not args.no_cost_model and name != "lorenz_uneven" and bool(args.persistent):
+        # the start-order cost model, fitted on a held-out draw of the workload (another seed), before
+        # the timed region like the tuning; the tuner then also measures start_order="model"
+        pilot = workloads.config(name, N=min(N, 200_000), seed=9001)
+        t_fit = time.perf_counter()
+        pprobs = deg.HIPProbVector.from_arrays(f, pilot["u0"], pilot["p"], pilot["tspan"], dev, tfs=pilot.get("tfs"))
+        w = deg.costmodel.fit(pprobs, prob, alg, **kw)
+        cost_model = dict(pilot_trajectories=pilot["N"], pilot_seed=9001, fit_s=time.perf_counter() - t_fit,
+                          weights=list(w))
+        dev = dataclasses.replace(dev, cost_model=w)
+        del pprobs
     tuned = None
     if not args.no_autotune and not (args.block_size or args.refill or args.chunk):
         tprobs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev, tfs=cfg.get("tfs"))
         dev = deg.autotune.tune(tprobs, prob, alg, refresh=args.retune, **kw)
         if start is not None or args.start_order == "span":
             dev = dataclasses.replace(dev, start_order=start or "none")  # an explicit choice beats the tuned one
+        elif cost_model is not None and dev.start_order != "model":
+            # a tuned configuration from the table predates the cost model: time both start orders
+            dev = faster_start(deg, torch, f, cfg, prob, alg, dev, kw)
         tuned = dict(block_size=dev.block_size, waves=dev.waves, refill=dev.refill, chunk=dev.chunk,
                      blocks_per_cu=dev.blocks_per_cu, start_order=dev.start_order)
         del tprobs
@@ -598,7 +715,8 @@ def main():
                    "persistent": bool(args.persistent), "block_size": dev.block_size or 256,
                    "waves": dev.waves, "refill": dev.refill or 4, "chunk": dev.chunk or 16, "autotuned": tuned,
                    "start_order": ("span" if name == "lorenz_uneven" and args.start_order == "span" and not args.no_order
-                                   else dev.start_order)},
+                                   else dev.start_order),
+                   "cost_model": cost_model},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
@@ -614,6 +732,8 @@ def main():
                              "what": "host SoA arrays -> H2D -> solve -> D2H of ts, us, retcode (PCIe-inclusive)"}
     if not args.no_extras:
         out["jit_compile_s"] = jit_compile_seconds(deg, f, n, np_, alg, dev)
+    if args.order_study and name != "lorenz_uneven" and bool(args.persistent):
+        out["start_orders"] = order_study(deg, torch, f, cfg, prob, alg, dev, kw)
     if not args.no_learned and spec["alg"] in ("GPUTsit5", "GPUVern7") and name != "lorenz_uneven":
         # learned vs PI on the same GPU, inputs and tolerances; the PI line above is `value`
         lrn = learned_leg(deg, torch, name, f, cfg, prob, alg, dev, kw, args.mlp_every)

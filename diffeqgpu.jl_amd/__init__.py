@@ -17,7 +17,7 @@ from .problems import (DualODEFunction, EnsembleProblem, HIPProbVector, Immutabl
                        ODEFunction, ODEProblem, SDEFunction, SDEProblem, adapt, get_backend, is_diagonal_noise, make_prob_compatible, remake)
 from .ranges import colon, linrange
 from .sweeps import Draw, ParameterSweep
-from . import autotune, learned
+from . import autotune, costmodel, learned
 from .tracer import SVector
 from . import tracer as math  # sin, cos, exp, log, sqrt, tanh, fabs, pow, ifelse on traced values
 
@@ -27,7 +27,7 @@ __all__ = [
     "HIPDevice", "alg_order", "EnsembleContext", "EnsembleSolution", "ODESolution", "solve", "SolveResult", "order_by_span",
     "vectorized_asolve", "vectorized_solve", "EnsembleProblem", "HIPProbVector", "ImmutableODEProblem",
     "ODEFunction", "ODEProblem", "adapt", "get_backend", "make_prob_compatible", "remake", "colon", "linrange",
-    "SVector", "math", "Draw", "ParameterSweep", "autotune", "learned", "CallbackSet", "ContinuousCallback",
+    "SVector", "math", "Draw", "ParameterSweep", "autotune", "costmodel", "learned", "CallbackSet", "ContinuousCallback",
     "DiscreteCallback", "GPUContinuousCallback", "GPUDiscreteCallback", "LeftRootFind", "NoRootFind", "RightRootFind",
     "terminate", "u_modified", "GPUSDEAlgorithm", "GPUEM", "GPUSIEA", "SDEFunction", "SDEProblem",
     "ImmutableSDEProblem", "is_diagonal_noise", "Dual", "DualArray", "DualODEFunction",

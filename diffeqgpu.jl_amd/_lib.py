@@ -97,7 +97,7 @@ EXPORTS = [
     "deg_last_error", "deg_version", "deg_solve_args_size", "deg_draw_uniform", "deg_graph_begin", "deg_graph_end",
     "deg_graph_launch", "deg_graph_num_nodes", "deg_graph_destroy", "deg_mlp_packed_size", "deg_mlp_partial_cols",
     "deg_mlp_grad", "deg_mlp_adam", "deg_mlp_forward", "deg_mlp_pack", "deg_mlp_infer", "deg_order_by_span",
-    "deg_draw_normal", "deg_start_order", "deg_start_order_noalloc",
+    "deg_draw_normal", "deg_start_order", "deg_start_order_noalloc", "deg_start_order_model",
 ]
 STEPLOG_COLS = 12
 MLP_NPARAM = 1313
@@ -135,6 +135,7 @@ def _declare(lib):
     lib.deg_order_by_span.argtypes = [vp, vp, ctypes.c_float, ctypes.c_float, ctypes.c_int64, vp, vp, vp]
     lib.deg_start_order.argtypes = [vp, ctypes.POINTER(SolveArgs), vp, vp, vp]
     lib.deg_start_order_noalloc.argtypes = [vp, ctypes.POINTER(SolveArgs), vp]
+    lib.deg_start_order_model.argtypes = [vp, ctypes.POINTER(SolveArgs), vp, ctypes.c_int32, vp, vp, vp]
     lib.deg_draw_normal.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32,
                                     ctypes.c_int32, vp]
     lib.deg_graph_begin.argtypes = [vp]
@@ -155,7 +156,7 @@ def _declare(lib):
                  "deg_draw_uniform", "deg_graph_begin", "deg_graph_end", "deg_graph_launch", "deg_graph_num_nodes",
                  "deg_graph_destroy", "deg_mlp_grad", "deg_mlp_adam", "deg_mlp_forward", "deg_mlp_pack",
                  "deg_mlp_infer", "deg_order_by_span", "deg_draw_normal", "deg_start_order",
-                 "deg_start_order_noalloc"):
+                 "deg_start_order_noalloc", "deg_start_order_model"):
         getattr(lib, name).restype = ctypes.c_int
 
 
@@ -253,6 +254,14 @@ class Kernel:
         args.struct_size = ctypes.sizeof(SolveArgs)
         check(lib().deg_start_order_noalloc(self.handle, ctypes.byref(args), ctypes.c_void_p(order_ptr)),
               "deg_start_order_noalloc")
+
+    def start_order_model(self, args: SolveArgs, model_ptr: int, nmodel: int, key_ptr: int, order_ptr: int,
+                          work_ptr: int):
+        """Cost-ordered start by a fitted cost model (deg_start_order_model)."""
+        args.struct_size = ctypes.sizeof(SolveArgs)
+        check(lib().deg_start_order_model(self.handle, ctypes.byref(args), ctypes.c_void_p(model_ptr), int(nmodel),
+                                          ctypes.c_void_p(key_ptr), ctypes.c_void_p(order_ptr),
+                                          ctypes.c_void_p(work_ptr)), "deg_start_order_model")
 
 
 _contexts = {}

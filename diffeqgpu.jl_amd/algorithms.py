@@ -165,8 +165,11 @@ class HIPDevice:
                   and leave, so fewer, fuller waves finish the tail.
       start_order persistent adaptive grids: "cost" starts the trajectories in descending order of
                   span / initial step estimate (deg_start_order, on the device before every solve),
-                  "none" in index order; None: "none" (the autotuner measures both). An explicit
-                  ``order=`` overrides it. Results do not depend on the order.
+                  "model" in descending order of the attempt count ``cost_model`` predicts
+                  (deg_start_order_model), "none" in index order; None: "none" (the autotuner
+                  measures them). An explicit ``order=`` overrides it. Results do not depend on the order.
+      cost_model  the 3 + n + np weights of a start-order cost model (costmodel.fit), for
+                  start_order="model".
     """
 
     device: int = 0
@@ -183,14 +186,18 @@ class HIPDevice:
     compact: bool = False
     batch_saves: bool = False
     start_order: str | None = None
+    cost_model: tuple | None = None
 
     def cost_order(self, adaptive: bool) -> bool:
-        """Whether solves on this device start their trajectories in cost order."""
+        """Whether solves on this device start their trajectories in cost order (by the span / initial
+        step key or by the cost model)."""
         mode = self.start_order or "none"
-        if mode not in ("cost", "none"):
-            raise ValueError("start_order must be 'cost' or 'none'")
+        if mode not in ("cost", "model", "none"):
+            raise ValueError("start_order must be 'cost', 'model' or 'none'")
+        if mode == "model" and not self.cost_model:
+            raise ValueError("start_order='model' needs cost_model weights (costmodel.fit)")
         persistent = self.persistent if self.persistent is not None else adaptive
-        return mode == "cost" and adaptive and persistent
+        return mode in ("cost", "model") and adaptive and persistent
 
     def sde_flags(self) -> int:
         """Compile flags of the fixed-step SDE kernels: contraction (default "off") and the

@@ -8,7 +8,8 @@ lowest measured kernel time:
   refill batch     persistent grids: idle lanes per wave that trigger a refill;
   chunk            persistent grids: trajectory ids claimed per queue atomic;
   blocks per CU    persistent grids: resident blocks per compute unit (0 = as many as fit);
-  start order      persistent adaptive grids: index order or the cost order of deg_start_order
+  start order      persistent adaptive grids: index order, the cost order of deg_start_order, or (when the
+                   device carries a fitted cost model) the order of deg_start_order_model
                    (which helps where the initial step estimate predicts the work, and can hurt
                    where it points the wrong way).
 The LDS depth of the saveat cache needs no tuning: it is sized to the save grid at every launch.
@@ -140,7 +141,7 @@ def tune(probs, prob, alg, *, adaptive: bool = True, repeats: int = 2, use_cache
     if persistent:
         axes += [("refill", REFILL), ("chunk", CHUNK), ("blocks_per_cu", None)]
     if persistent and adaptive:
-        axes += [("start_order", ("none", "cost"))]
+        axes += [("start_order", ("none", "cost", "model") if probs.dev.cost_model else ("none", "cost"))]
     for name, values in axes:
         if values is None:
             values = blocks_per_cu_candidates(cfg["block_size"])

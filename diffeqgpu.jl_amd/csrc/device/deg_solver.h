@@ -2262,14 +2262,20 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
     }
 }
 
-// Clears the persistent work counter before a launch (a kernel, not a memset, so the
-// sequence can be captured in a graph).
 // Start-order cost key of each trajectory: its span over the initial step estimate of Hairer,
 // Norsett and Wanner (Solving Ordinary Differential Equations I, II.4): with sc = abstol + |u0| reltol,
 // d0 = ||u0 / sc||, d1 = ||f(u0, p, t0) / sc|| (RMS norms), h0 = 0.01 d0 / d1, or 1e-6 when d0 or d1
-// is below 1e-5. Sorting by it (descending) starts the trajectories that are likely to take the most
+// is below 1e-5. With a cost model (model != NULL) the key is instead the predicted attempt count
+// 2^(w . x) over the features x = [1, log2(span / h0), log2(|u0_c| + 2^-20) for each c,
+// log2(|p_j| + 2^-20) for each j, log2 span], DEG_COST_NFEAT = 3 + DEG_N + DEG_NP weights, summed in
+// that order. Sorting by the key (descending) starts the trajectories that are likely to take the most
 // steps first; a non-finite key becomes 0 (started last).
-extern "C" __global__ void __launch_bounds__(256) DEG_COST_KEY(const deg_kargs a, float* __restrict__ key) {
+#define DEG_COST_NFEAT (3 + DEG_N + DEG_NP)
+__device__ __forceinline__ float deg_cost_log2(float x) {
+    return __builtin_amdgcn_logf(__builtin_fabsf(x) + 9.5367431640625e-07f);  // |x| + 2^-20
+}
+extern "C" __global__ void __launch_bounds__(256) DEG_COST_KEY(const deg_kargs a, float* __restrict__ key,
+                                                               const float* __restrict__ model) {
     const long long li = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= a.n_traj) return;
     const long long i = a.id_offset + li;
@@ -2291,10 +2297,20 @@ extern "C" __global__ void __launch_bounds__(256) DEG_COST_KEY(const deg_kargs a
     d0 = __builtin_sqrtf(d0 / (float)DEG_N);
     d1 = __builtin_sqrtf(d1 / (float)DEG_N);
     const float h0 = (d0 < 1e-5f || d1 < 1e-5f) ? 1e-6f : 0.01f * d0 / d1;
-    const float k = __builtin_fabsf(tf - t0) / h0;
+    const float span = __builtin_fabsf(tf - t0);
+    float k = span / h0;
+    if (model != nullptr) {
+        float z = model[0] + model[1] * __builtin_amdgcn_logf(__builtin_fmaxf(k, 1e-30f));
+        DEG_FOR(c) z = z + model[2 + c] * deg_cost_log2(u[c]);
+        _Pragma("unroll") for (int j = 0; j < DEG_NP; ++j) z = z + model[2 + DEG_N + j] * deg_cost_log2(p[j]);
+        z = z + model[2 + DEG_N + DEG_NP] * __builtin_amdgcn_logf(__builtin_fmaxf(span, 1e-30f));
+        k = __builtin_amdgcn_exp2f(__builtin_fminf(z, 126.0f));
+    }
     key[li] = (k == k && k < 3.0e38f) ? k : (k == k ? 3.0e38f : 0.0f);
 }
 
+// Clears the persistent work counters before a launch (a kernel, not a memset, so the
+// sequence can be captured in a graph).
 extern "C" __global__ void DEG_QUEUE_RESET(unsigned int* q) {
     if (blockIdx.x == 0 && threadIdx.x < DEG_NQUEUE) q[threadIdx.x * DEG_QSTRIDE] = 0u;
 }

@@ -250,10 +250,16 @@ def _launch(probs: HIPProbVector, alg, adaptive: bool, *, dt, abstol, reltol, sa
     if mlp is not None:
         a.mlp = mlp.data_ptr()
         a.mlp_every = int(mlp_every)
-    okey = owork = None
+    okey = owork = omodel = None
     if order is None and not sde and dev.cost_order(adaptive) and N > 0:
         # cost-ordered start, recomputed on the device before every launch of this solve
         okey, order, owork = (outputs.get(k) for k in ("order_key", "order_auto", "order_work"))
+        if dev.start_order == "model":
+            if len(dev.cost_model) != 3 + n + probs.np:
+                raise ValueError("cost_model needs 3 + n + np weights for this problem")
+            omodel = outputs.get("cost_model")
+            if omodel is None:
+                omodel = torch.as_tensor(np.asarray(dev.cost_model, np.float32)).to(device)
         if okey is None:
             okey = torch.empty(N, dtype=torch.float32, device=device)
         if order is None:
@@ -284,8 +290,15 @@ def _launch(probs: HIPProbVector, alg, adaptive: bool, *, dt, abstol, reltol, sa
         a.seeds = probs.seeds.data_ptr() if probs.seeds is not None else None
         a.seed = int(probs.seed) & 0xFFFFFFFFFFFFFFFF
         a.traj_base = int(traj_base)
-    if okey is not None:
-        kern.start_order(a, okey.data_ptr(), order.data_ptr(), owork.data_ptr())
+
+    def order_start():
+        if omodel is not None:
+            kern.start_order_model(a, omodel.data_ptr(), omodel.numel(), okey.data_ptr(), order.data_ptr(),
+                                   owork.data_ptr())
+        elif okey is not None:
+            kern.start_order(a, okey.data_ptr(), order.data_ptr(), owork.data_ptr())
+
+    order_start()
     kern.solve(a)
     if timing:
         ms = float(box.value)
@@ -293,15 +306,14 @@ def _launch(probs: HIPProbVector, alg, adaptive: bool, *, dt, abstol, reltol, sa
 
     def relaunch():
         # same arguments, same output buffers: recomputes the start order and every output
-        if okey is not None:
-            kern.start_order(a, okey.data_ptr(), order.data_ptr(), owork.data_ptr())
+        order_start()
         kern.solve(a)
 
     # the result keeps the argument block and every tensor it points to alive
     return SolveResult(ts, us_soa.permute(0, 2, 1), retcode=retcode, naccept=nacc, nreject=nrej, us_soa=us_soa,
                        kernel_ms=ms, saveat=sv, kernel=kern, flags=flags, relaunch=relaunch, args=a,
                        probs=probs, tstops=tst, order_key=okey, order_auto=order if okey is not None else None,
-                       order_work=owork)
+                       order_work=owork, cost_model=omodel)
 
 
 def vectorized_solve(probs, prob, alg, *, dt, saveat=None, save_everystep=True, debug=False, callback=None,

@@ -260,6 +260,14 @@ int deg_order_by_span(const float *t0s, const float *tfs, float t0, float tf, in
  * the order. key: [n_traj] float, order: [n_traj] int32, device memory. */
 int deg_start_order(deg_kernel *k, const deg_solve_args *a, float *key, int32_t *order, int32_t *work);
 
+/* Cost-ordered start with a cost model (no counterpart in the reference): the key is the predicted
+ * attempt count 2^(w . x) of a linear model on log2 features, x = [1, log2 key, log2(|u0_c| + 2^-20)
+ * (c < n), log2(|p_j| + 2^-20) (j < np), log2 span] with key = span / h0 as in deg_start_order; model
+ * holds the nmodel = 3 + n + np weights (device), fitted by the host on a pilot ensemble's attempt
+ * counts (diffeqgpu.jl_amd/costmodel.py). Otherwise as deg_start_order. */
+int deg_start_order_model(deg_kernel *k, const deg_solve_args *a, const float *model, int32_t nmodel, float *key,
+                          int32_t *order, int32_t *work);
+
 /* Allocation-free variant of deg_start_order (no counterpart in the reference; for callers that cannot
  * allocate device memory: code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
  * The cost keys and the counting sort's counters live in a static device array of the library, one

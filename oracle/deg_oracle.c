@@ -64,8 +64,9 @@
 enum { OM_TSIT5 = 0, OM_VERN7 = 1, OM_RB23 = 2, OM_VERN9 = 3, OM_RODAS4 = 4, OM_RODAS5P = 5, OM_KVAERNO3 = 6,
        OM_KVAERNO5 = 7 };
 /* DEG_O_AUGMENTED: a dual-number system solved as a plain system of its n (1 + P) entries (the full
- * augmented Jacobian and W, the plain norm); a pin of the dual W factor, not a reference behaviour */
-enum { DEG_O_KAHAN = 1, DEG_O_DETPOW = 2, DEG_O_FDJAC = 4, DEG_O_AUGMENTED = 8 };
+ * augmented Jacobian and W, the plain norm); a pin of the dual W factor, not a reference behaviour.
+ * DEG_O_PIVLU: W by LU with partial pivoting also for n <= 3; a pin of the Cramer solve, likewise */
+enum { DEG_O_KAHAN = 1, DEG_O_DETPOW = 2, DEG_O_FDJAC = 4, DEG_O_AUGMENTED = 8, DEG_O_PIVLU = 16 };
 
 /* autodiff = false: J by forward differences, column i = (f(u + dx e_i) - f(u)) / dx with the
  * perturbed entry rounded to Float32 (finite_diff_jac, alg_utils.jl:17-27), and the time gradient
@@ -558,8 +559,8 @@ static void w_cramer(olu *F, int n) {
     }
 }
 
-static void w_lu(olu *F, int n) {
-    if (n <= 3) {
+static void w_lu(olu *F, int n, int pivlu) {
+    if (n <= 3 && !pivlu) {
         w_cramer(F, n);
         return;
     }
@@ -613,7 +614,7 @@ static void w_factor(olu *F, const osys *S, float gam, const float *J, const flo
     for (int r = 0; r < m; ++r)
         for (int c = 0; c < m; ++c)
             F->lu[r][c] = (M ? (double)M[r * m + c] : (r == c ? 1.0 : 0.0)) - g * (double)J[r * m + c];
-    w_lu(F, m);
+    w_lu(F, m, S->pivlu);
     w_dual(F, S, -g, J);
 }
 
@@ -624,7 +625,7 @@ static void w_factor_rodas(olu *F, const osys *S, float invg, const float *J, co
     for (int r = 0; r < m; ++r)
         for (int c = 0; c < m; ++c)
             F->lu[r][c] = (double)J[r * m + c] - (M ? (double)(M[r * m + c] * invg) : (r == c ? (double)invg : 0.0));
-    w_lu(F, m);
+    w_lu(F, m, S->pivlu);
     w_dual(F, S, 1.0, J);
 }
 
@@ -841,7 +842,7 @@ static void kv_w(olu *F, const osys *S, const float *p, float gdt, const float *
     for (int r = 0; r < m; ++r)
         for (int c = 0; c < m; ++c)
             F->lu[r][c] = (S->M ? -(double)S->M[r * m + c] : (r == c ? -1.0 : 0.0)) + (double)gdt * (double)Jm[r * m + c];
-    w_lu(F, m);
+    w_lu(F, m, S->pivlu);
     w_dual(F, S, (double)gdt, Jm);
 }
 
@@ -1564,6 +1565,7 @@ int deg_oracle_solve_cb(int system, int method, int adaptive, int64_t N,
     init_tabs();
     osys Sys = OSYSTEMS[system];
     Sys.fd = (flags & DEG_O_FDJAC) != 0;
+    Sys.pivlu = (flags & DEG_O_PIVLU) != 0;
     if (flags & DEG_O_AUGMENTED) {
         Sys.dualnv = 0;
         Sys.dualp = 0;

@@ -21,7 +21,7 @@ SYSTEM_DIMS = {"lorenz": (3, 3), "decay": (1, 1), "growth": (1, 1), "quad": (1, 
                "robertson_eye": (3, 3), "dae2": (2, 0), "lorenz_dual": (21, 21)}
 METHOD_IDS = {"tsit5": 0, "vern7": 1, "rb23": 2, "vern9": 3, "rodas4": 4, "rodas5p": 5, "kvaerno3": 6,
               "kvaerno5": 7}
-KAHAN, DETPOW, FDJAC, AUGMENTED = 1, 2, 4, 8
+KAHAN, DETPOW, FDJAC, AUGMENTED, PIVLU = 1, 2, 4, 8, 16
 
 _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -155,14 +155,16 @@ def jac(system: str, u, p, t, fd=False):
 
 def solve(system: str, method: str, adaptive: bool, u0, p, tspan, dt, *, abstol=1e-6, reltol=1e-3, saveat=None,
           nsave=None, save_everystep=False, t0s=None, tfs=None, kahan=False, detpow=False, max_attempts=0,
-          nthreads=None, tstops=None, callbacks=(), fdjac=False, augmented=False):
+          nthreads=None, tstops=None, callbacks=(), fdjac=False, augmented=False, pivlu=False):
     """Float32 restatement over an ensemble. u0 (n, N), p (np, N) float32 SoA. Returns dict with
     ts (nsave, N), us (nsave, n, N), retcode, naccept, nreject. Rows the reference leaves
     uninitialised are NaN here. tstops: stopping times; callbacks: descriptors from discrete() /
     continuous(). augmented: a dual-number system is solved as a plain system of all its entries
     (full augmented Jacobian and W, plain norm) -- a check of the dual W factor, not a reference
     behaviour. fdjac: the implicit methods take J and the time gradient by forward differences
-    of f (autodiff = false) instead of the system's analytic ones."""
+    of f (autodiff = false) instead of the system's analytic ones. pivlu: W is solved by LU with
+    partial pivoting also for n <= 3, where the reference (and the default) use Cramer's rule -- a
+    check of the Cramer solve, not a reference behaviour."""
     n, np_ = SYSTEM_DIMS[system]
     u0 = _f32(u0)
     p = _f32(p) if np_ > 0 else np.zeros((0, u0.shape[1]), np.float32)
@@ -176,6 +178,7 @@ def solve(system: str, method: str, adaptive: bool, u0, p, tspan, dt, *, abstol=
     na = np.zeros(N, np.int32)
     nr = np.zeros(N, np.int32)
     flags = (KAHAN if kahan else 0) | (DETPOW if detpow else 0) | (FDJAC if fdjac else 0) | (AUGMENTED if augmented else 0)
+    flags |= PIVLU if pivlu else 0
     t0s, tfs = _f32(t0s), _f32(tfs)
     tst = None if tstops is None else np.ascontiguousarray(np.asarray(tstops, np.float64).astype(np.float32).reshape(-1))
     cbs = list(callbacks or ())

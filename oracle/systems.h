@@ -38,6 +38,7 @@ typedef struct {
     const float *M;
     int fd; /* 1: jac and tgrad by forward differences of f (autodiff = false) */
     int dualnv, dualp; /* dual-number state: dualnv values, each with dualp partials (0: plain) */
+    int pivlu;         /* 1: W by LU with partial pivoting for every n (a check of Cramer's rule) */
 } osys;
 
 static void lorenz_f(const float *u, const float *p, float t, float *du) {

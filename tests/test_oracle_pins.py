@@ -559,6 +559,36 @@ def test_robertson_fixture_vs_oracle(method):
         assert err.max() < 1e-4, err.max()
 
 
+@pytest.mark.parametrize("method", ["rb23", "rodas4", "rodas5p", "kvaerno3", "kvaerno5"])
+def test_cramer_solve_matches_pivoted_lu(method):
+    """For n <= 3 the W solve is Cramer's rule (the reference's formulas, linalg/linsolve.jl:8-57) in
+    FP64; the oracle's pivoted LU (flag pivlu), which the convergence orders and the Radau fixture pin
+    for every n, solves the same FP64 systems. The two agree to FP64 rounding, which almost never moves
+    a Float32 result: the solutions must be bit-identical in at least 99 % of the values and within
+    1e-5 relative everywhere, fixed and adaptive, for n = 3 (Robertson), 2 (Van der Pol) and 1 (decay).
+    A transposed cofactor (W[0][1] given W[1][0]'s formula) fails the check for every method: on
+    Robertson 17-99 % of the values stay equal and the rest differ by up to 5e-4 (checked by planting it)."""
+    r = np.random.default_rng(0)
+    N = 64
+    cases = [
+        ("robertson", np.tile(np.array([[1.0], [0.0], [0.0]], F32), (1, N)),
+         (np.array([[0.04], [3e7], [1e4]]) * np.exp(0.3 * r.normal(size=(3, N)))).astype(F32), 1e-8, 1e-4),
+        ("vdp", r.uniform(-2, 2, (2, N)).astype(F32), np.exp(r.uniform(np.log(0.1), np.log(30), (1, N))).astype(F32),
+         1e-6, 1e-6),
+        ("decay", r.uniform(0.5, 2, (1, N)).astype(F32), r.uniform(0.1, 5, (1, N)).astype(F32), 1e-6, 1e-6),
+    ]
+    for system, u0, p, atol, rtol in cases:
+        for adaptive, tspan, dt in ((False, (0.0, 1.0), 1e-3), (True, (0.0, 5.0), 1e-3)):
+            kw = dict(abstol=atol, reltol=rtol, saveat=np.linspace(0, tspan[1], 6).astype(F32))
+            a = pyoracle.solve(system, method, adaptive, u0, p, tspan, dt, **kw)
+            b = pyoracle.solve(system, method, adaptive, u0, p, tspan, dt, pivlu=True, **kw)
+            assert np.all(a["retcode"] == 0) and np.all(b["retcode"] == 0)
+            ua, ub = a["us"].astype(np.float64), b["us"].astype(np.float64)
+            assert np.mean(ua == ub) >= 0.99, (system, adaptive, np.mean(ua == ub))
+            rel = np.abs(ua - ub) / np.maximum(np.abs(ub), 1e-6)
+            assert rel.max() <= 1e-5, (system, adaptive, rel.max())
+
+
 def test_vdp_fixture_vs_oracle():
     cfg, gold = _workload("vdp"), _golden("vdp_vern9.npz")
     r = pyoracle.solve("vdp", "tsit5", True, cfg["u0"], cfg["p"], cfg["tspan"], cfg["dt"], abstol=1e-6, reltol=1e-6)
