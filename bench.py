@@ -57,6 +57,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (includes dual-issue/packed FP32)
+# the issue ceiling of this build: packed FP32 is compiled out, so one FMA per lane per cycle (78.6 TFLOP/s)
+ISSUE_CEILING_TFLOPS = FP32_PEAK_TFLOPS / 2
 METRIC = "trajectories/sec at fixed FP64-oracle error; learned vs PI controller"
 
 WORKLOADS = {
@@ -405,10 +407,11 @@ def learned_leg(deg, torch, name, f, cfg, prob, alg, dev, kw, every, steps=10, t
     return out
 
 
-def learned_vs_pi(deg, torch, name, every, steps=8):
+def learned_vs_pi(deg, torch, name, every, steps=8, use_cost_model=True):
     """A second learned-vs-PI comparison on its own workload (the 2M Van der Pol ensemble): the PI
     kernel in its tuned configuration and the learned kernel on the same inputs and tolerances,
-    with attempts, rejections and the FP64 error of both on the fixture's 1,000 trajectories."""
+    with attempts, rejections and the FP64 error of both on the fixture's 1,000 trajectories. With
+    use_cost_model both tuners may also choose the start order of a fitted cost model."""
     from diffeqgpu_amd import systems, workloads
 
     cfg = workloads.config(name)
@@ -417,8 +420,16 @@ def learned_vs_pi(deg, torch, name, every, steps=8):
     prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
     kw = solve_kw(cfg)
     dev = deg.HIPDevice(0, contract="fast", powmode="fast", persistent=True)
+    if use_cost_model:
+        # both controllers may start in the order of a cost model fitted on a held-out draw
+        pilot = workloads.config(name, N=200_000, seed=9001)
+        pprobs = deg.HIPProbVector.from_arrays(f, pilot["u0"], pilot["p"], pilot["tspan"], dev)
+        dev = dataclasses.replace(dev, cost_model=deg.costmodel.fit(pprobs, prob, alg, **kw))
+        del pprobs
     probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev)
     dev = deg.autotune.tune(probs, prob, alg, **kw)
+    if use_cost_model and dev.start_order != "model":
+        dev = faster_start(deg, torch, f, cfg, prob, alg, dev, kw)
     probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev)
     res = deg.vectorized_asolve(probs, prob, alg, **kw)
     stream = torch.cuda.current_stream()
@@ -532,6 +543,7 @@ def main_sde(args, name):
                    "block_size": dev.block_size or 256},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel_ms": float(np.mean(kernel_ms)),
+                     "issue_ceiling": {"peak": ISSUE_CEILING_TFLOPS, "frac": achieved / ISSUE_CEILING_TFLOPS},
                      "sde_steps_per_launch": nsteps, "flops_per_step": fstep, "philox_int_ops_per_step": istep,
                      "int_tops": nsteps * istep / k_avg_s / 1e12},
         "retcodes": retcodes,
@@ -720,6 +732,9 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                     "issue_ceiling": {"peak": ISSUE_CEILING_TFLOPS, "frac": achieved / ISSUE_CEILING_TFLOPS,
+                                       "what": "FP32 without packed instructions (compiled out): one FMA per lane "
+                                               "per cycle"},
                      "kernel_ms": float(np.mean(kernel_ms)), "flops_per_step": flops,
                      "attempts_per_step": attempts, "saves_per_step": nsave_interp,
                      "flops_per_attempt": {"fp32": fa32, "fp64": fa64}, "rhs_flops": fc},
@@ -742,7 +757,8 @@ def main():
         lrn["attempts_vs_pi"] = lrn["attempts"] / attempts
         out["learned"] = lrn
     if not args.no_learned and name == "lorenz_adaptive":
-        out["learned_c5"] = learned_vs_pi(deg, torch, "vdp", args.mlp_every)
+        out["learned_c5"] = learned_vs_pi(deg, torch, "vdp", args.mlp_every,
+                                          use_cost_model=not args.no_cost_model and bool(args.persistent))
     if not args.no_cpu:
         threads = args.cpu_threads or host_cpu()[0]
         out["cpu_baseline"] = cpu_baseline(name, cfg, threads, spec["cpu_sample"])
