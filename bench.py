@@ -56,9 +56,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (includes dual-issue/packed FP32)
-# the issue ceiling of this build: packed FP32 is compiled out, so one FMA per lane per cycle (78.6 TFLOP/s)
-ISSUE_CEILING_TFLOPS = FP32_PEAK_TFLOPS / 2
+FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 METRIC = "trajectories/sec at fixed FP64-oracle error; learned vs PI controller"
 
 WORKLOADS = {
@@ -184,10 +182,12 @@ def _tile_order(tile_key, N, T=16, Q=8):
     return order
 
 
-def order_study(deg, torch, f, cfg, prob, alg, dev, kw, reps=7):
-    """Median kernel time (HIP events, reps relaunches, each including its ordering kernels) of the same
-    solve under different start orders, with the tuned launch, on the workload's first N - N % 128
-    trajectories (whole tiles of 16 ids for each of the 8 queues):
+def order_study(deg, torch, f, cfg, prob, alg, dev, kw, reps=5, rounds=3):
+    """Median kernel time (HIP events, each launch including its ordering kernels) of the same solve
+    under different start orders, with the tuned launch, on the workload's first N - N % 128
+    trajectories (whole tiles of 16 ids for each of the 8 queues). Every leg is set up first and then
+    timed in `rounds` interleaved rounds of `reps` relaunches (median over all), so that clock drift
+    over the run does not favour a leg.
       index        ids in order;
       cost_key     descending span / h0 (deg_start_order);
       cost_model   descending predicted attempts (deg_start_order_model), when a model is fitted;
@@ -195,43 +195,63 @@ def order_study(deg, torch, f, cfg, prob, alg, dev, kw, reps=7):
                    descending order of the tile's largest span / h0 key;
       tiles_by_attempts, attempts
                    the same tiles, or single trajectories, in descending order of their true attempt
-                   counts from an earlier solve: bounds, since the counts are known only after solving.
+                   counts from an earlier solve: bounds, since the counts are known only after solving;
+      attempts_bucketed, cost_model_exact
+                   the true counts through the device's bucketed sort (2047 buckets over the keys'
+                   range, no order inside a bucket), and the cost model's keys in exact descending
+                   order (both computed once: no ordering kernels in their launches).
     Also the Spearman rank correlation of each key with the attempt counts."""
+    from diffeqgpu_amd import _lib
+
     N = cfg["N"] - cfg["N"] % 128
     u0, p = cfg["u0"][:, :N], cfg["p"][:, :N]
     base = dataclasses.replace(dev, start_order="none")
     st = torch.cuda.current_stream()
 
-    def timed(d, order=None):
+    def setup(d, order=None):
         probs = deg.HIPProbVector.from_arrays(f, u0, p, cfg["tspan"], d)
         o = None if order is None else torch.as_tensor(order, device="cuda")
         res = deg.vectorized_asolve(probs, prob, alg, order=o, **kw)
-        ms = []
-        for _ in range(reps):
-            a_, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a_.record(st)
-            res.relaunch()
-            b_.record(st)
-            b_.synchronize()
-            ms.append(a_.elapsed_time(b_))
-        return res, float(np.median(ms))
+        torch.cuda.synchronize()
+        return res
 
     def spearman(x, y):
         rx, ry = np.argsort(np.argsort(x)), np.argsort(np.argsort(y))
         return float(np.corrcoef(rx, ry)[0, 1])
 
-    out = {"trajectories": N}
-    res, out["index"] = timed(base)
-    att = (res.naccept.to(torch.int64) + res.nreject.to(torch.int64)).cpu().numpy()
-    res, out["cost_key"] = timed(dataclasses.replace(base, start_order="cost"))
-    key = res.order_key.cpu().numpy().astype(np.float64)
+    out = {"trajectories": N, "rounds": rounds, "relaunches_per_round": reps}
+    legs = {}
+    legs["index"] = setup(base)
+    att = (legs["index"].naccept.to(torch.int64) + legs["index"].nreject.to(torch.int64)).cpu().numpy()
+    legs["cost_key"] = setup(dataclasses.replace(base, start_order="cost"))
+    key = legs["cost_key"].order_key.cpu().numpy().astype(np.float64)
     out["spearman_cost_key"] = spearman(key, att)
     if dev.cost_model:
-        res, out["cost_model"] = timed(dataclasses.replace(base, start_order="model"))
-        out["spearman_cost_model"] = spearman(res.order_key.cpu().numpy().astype(np.float64), att)
-    _, out["tiles_by_key"] = timed(base, _tile_order(key.reshape(-1, 16).max(axis=1), N))
-    _, out["tiles_by_attempts"] = timed(base, _tile_order(att.reshape(-1, 16).max(axis=1), N))
-    _, out["attempts"] = timed(base, np.argsort(-att, kind="stable").astype(np.int32))
+        legs["cost_model"] = setup(dataclasses.replace(base, start_order="model"))
+        mkey = legs["cost_model"].order_key.cpu().numpy().astype(np.float64)
+        out["spearman_cost_model"] = spearman(mkey, att)
+        legs["cost_model_exact"] = setup(base, np.argsort(-mkey, kind="stable").astype(np.int32))
+    legs["tiles_by_key"] = setup(base, _tile_order(key.reshape(-1, 16).max(axis=1), N))
+    legs["tiles_by_attempts"] = setup(base, _tile_order(att.reshape(-1, 16).max(axis=1), N))
+    legs["attempts"] = setup(base, np.argsort(-att, kind="stable").astype(np.int32))
+    akey = torch.as_tensor(att.astype(np.float32), device="cuda")
+    border = torch.empty(N, dtype=torch.int32, device="cuda")
+    work = torch.empty(_lib.ORDER_WORK_INTS, dtype=torch.int32, device="cuda")
+    _lib.order_by_span(0, akey.data_ptr(), 0.0, 0.0, N, border.data_ptr(), work.data_ptr(), st.cuda_stream)
+    torch.cuda.synchronize()
+    legs["attempts_bucketed"] = setup(base, border.cpu().numpy())
+    ms = {k: [] for k in legs}
+    for _ in range(rounds):
+        for k, res in legs.items():
+            for _ in range(reps):
+                a_, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a_.record(st)
+                res.relaunch()
+                b_.record(st)
+                b_.synchronize()
+                ms[k].append(a_.elapsed_time(b_))
+    for k in legs:
+        out[k] = float(np.median(ms[k]))
     out["unit"] = "ms (median kernel time)"
     return out
 
@@ -543,7 +563,6 @@ def main_sde(args, name):
                    "block_size": dev.block_size or 256},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel_ms": float(np.mean(kernel_ms)),
-                     "issue_ceiling": {"peak": ISSUE_CEILING_TFLOPS, "frac": achieved / ISSUE_CEILING_TFLOPS},
                      "sde_steps_per_launch": nsteps, "flops_per_step": fstep, "philox_int_ops_per_step": istep,
                      "int_tops": nsteps * istep / k_avg_s / 1e12},
         "retcodes": retcodes,
@@ -732,9 +751,6 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "issue_ceiling": {"peak": ISSUE_CEILING_TFLOPS, "frac": achieved / ISSUE_CEILING_TFLOPS,
-                                       "what": "FP32 without packed instructions (compiled out): one FMA per lane "
-                                               "per cycle"},
                      "kernel_ms": float(np.mean(kernel_ms)), "flops_per_step": flops,
                      "attempts_per_step": attempts, "saves_per_step": nsave_interp,
                      "flops_per_attempt": {"fp32": fa32, "fp64": fa64}, "rhs_flops": fc},

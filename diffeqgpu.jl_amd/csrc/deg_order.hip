@@ -1,12 +1,20 @@
 // deg_order.hip -- longest-first start order for the persistent queue.
 //
-// A counting sort of the trajectories by their span tf_i - t0_i, descending, into 2048
-// buckets given by the span's leading float bits (sign 0, 8 exponent bits, 3 mantissa bits:
-// non-negative floats order like their bit patterns). Four kernels on one stream:
-// clear the counters, histogram, exclusive scan in descending bucket order (one block),
-// scatter. Histogram and scatter count in LDS per block of 4096 trajectories, so the global
-// atomics are one per non-empty bucket per block (spans often fall into a few dozen buckets). Order within a bucket follows the atomics and is not fixed; the solver's results do
-// not depend on it (each lane's arithmetic is independent of where and when it runs).
+// A counting sort of the trajectories by their span tf_i - t0_i, descending. Non-negative floats order
+// like their bit patterns, so the buckets are ranges of bit patterns: bucket 0 holds zero, negative and
+// NaN spans (started last); buckets 1..2047 split [bmin, bmax], the bit patterns of the smallest and
+// largest positive span, into ranges of 2^shift patterns, with shift the smallest that fits the range
+// into 2047 buckets. Spans within a factor of 2 of each other thus get a resolution of about 1/2000
+// octave whatever their magnitude (a fixed split by the leading bits would give 1/8 octave, and a
+// narrow key range would fall into a handful of buckets). Five kernels on one stream: clear the
+// counters, the range (one atomic min/max pair per block), histogram, exclusive scan in descending
+// bucket order (one block), scatter. Histogram and scatter count in LDS per block of 4096 trajectories,
+// so the global atomics are one per non-empty bucket per block. Order within a bucket follows the
+// atomics and is not fixed; the solver's results do not depend on it (each lane's arithmetic is
+// independent of where and when it runs).
+//
+// work (4096 int32): count[2048], then cursor[2048]. Until the scan, cursor[0] and cursor[1] hold the
+// range (bmin, bmax); after it, count[0] and count[1] hold (bmin, shift) for the scatter.
 #include "deghip.h"
 
 #include <hip/hip_runtime.h>
@@ -19,41 +27,104 @@ int deg_rng_fail(int code, const char* msg);  // thread-local last error (deghip
 namespace {
 
 constexpr int kBuckets = 2048;
+constexpr int kPerBlock = 4096;  // trajectories per block in the range, histogram and scatter passes
 
-__device__ __forceinline__ int span_bucket(const float* t0s, const float* tfs, float t0, float tf, long long i) {
-    const float s = (tfs ? tfs[i] : tf) - (t0s ? t0s[i] : t0);
-    if (!(s > 0.0f)) return 0;                       // zero, negative or NaN spans go last
-    const unsigned int b = __builtin_bit_cast(unsigned int, s) >> 20;  // < 2048 for finite s > 0
-    return b < (unsigned int)kBuckets ? (int)b : kBuckets - 1;
+__device__ __forceinline__ float span_at(const float* t0s, const float* tfs, float t0, float tf, long long i) {
+    return (tfs ? tfs[i] : tf) - (t0s ? t0s[i] : t0);
 }
 
-constexpr int kPerBlock = 4096;  // trajectories per block in the histogram and scatter passes
+// the smallest shift with (bmax - bmin) >> shift <= 2046; 0 when there is at most one positive value
+__device__ __forceinline__ unsigned bucket_shift(unsigned bmin, unsigned bmax) {
+    if (bmax <= bmin) return 0u;
+    const unsigned r = bmax - bmin;
+    unsigned sh = 0u;
+    while ((r >> sh) > (unsigned)(kBuckets - 2)) ++sh;
+    return sh;
+}
+
+__device__ __forceinline__ int span_bucket(float s, unsigned bmin, unsigned shift) {
+    if (!(s > 0.0f)) return 0;  // zero, negative or NaN spans go last
+    const unsigned d = (__builtin_bit_cast(unsigned, s) - bmin) >> shift;
+    return 1 + (int)(d < (unsigned)(kBuckets - 2) ? d : (unsigned)(kBuckets - 2));
+}
 
 __global__ void __launch_bounds__(256) clear_kernel(int* __restrict__ work) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < 2 * kBuckets) work[j] = 0;
+    if (j < kBuckets) work[j] = 0;                          // counters
+    if (j == 0) {
+        work[kBuckets] = (int)0xFFFFFFFFu;                  // range minimum (as unsigned)
+        work[kBuckets + 1] = 0;                             // range maximum
+    }
+}
+
+// bit patterns of the smallest and largest positive span: a block reduction, then one atomic pair
+__global__ void __launch_bounds__(256) range_kernel(const float* t0s, const float* tfs, float t0, float tf,
+                                                    long long n, unsigned* __restrict__ mm) {
+    __shared__ unsigned wlo[4], whi[4];
+    unsigned lo = 0xFFFFFFFFu, hi = 0u;
+    const long long base = (long long)blockIdx.x * kPerBlock;
+    for (int j = threadIdx.x; j < kPerBlock; j += blockDim.x) {
+        const long long i = base + j;
+        if (i < n) {
+            const float s = span_at(t0s, tfs, t0, tf, i);
+            if (s > 0.0f) {
+                const unsigned b = __builtin_bit_cast(unsigned, s);
+                lo = b < lo ? b : lo;
+                hi = b > hi ? b : hi;
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned l2 = __shfl_xor(lo, off), h2 = __shfl_xor(hi, off);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        wlo[w] = lo;
+        whi[w] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            lo = wlo[k] < lo ? wlo[k] : lo;
+            hi = whi[k] > hi ? whi[k] : hi;
+        }
+        if (lo <= hi) {
+            atomicMin(&mm[0], lo);
+            atomicMax(&mm[1], hi);
+        }
+    }
 }
 
 // per-block histogram in LDS, then one global atomic per non-empty bucket of the block
 __global__ void __launch_bounds__(256) histogram_kernel(const float* t0s, const float* tfs, float t0, float tf,
-                                                        long long n, int* __restrict__ count) {
+                                                        long long n, int* __restrict__ count,
+                                                        const unsigned* __restrict__ mm) {
     __shared__ int h[kBuckets];
+    const unsigned bmin = mm[0], shift = bucket_shift(mm[0], mm[1]);
     for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) h[b] = 0;
     __syncthreads();
     const long long base = (long long)blockIdx.x * kPerBlock;
     for (int j = threadIdx.x; j < kPerBlock; j += blockDim.x) {
         const long long i = base + j;
-        if (i < n) atomicAdd(&h[span_bucket(t0s, tfs, t0, tf, i)], 1);
+        if (i < n) atomicAdd(&h[span_bucket(span_at(t0s, tfs, t0, tf, i), bmin, shift)], 1);
     }
     __syncthreads();
     for (int b = threadIdx.x; b < kBuckets; b += blockDim.x)
         if (h[b] != 0) atomicAdd(&count[b], h[b]);
 }
 
-// cursor[b] = number of trajectories in buckets above b (descending order); one block of 1024
-__global__ void __launch_bounds__(1024) scan_kernel(const int* __restrict__ count, int* __restrict__ cursor) {
+// cursor[b] = number of trajectories in buckets above b (descending order); one block of 1024. It
+// reads the range from cursor[0..1] before overwriting them and leaves (bmin, shift) in count[0..1].
+__global__ void __launch_bounds__(1024) scan_kernel(int* __restrict__ count, int* __restrict__ cursor) {
     __shared__ int part[1024];
+    __shared__ unsigned range[2];
     const int t = threadIdx.x;
+    if (t == 0) {
+        range[0] = (unsigned)cursor[0];
+        range[1] = (unsigned)cursor[1];
+    }
     // each thread sums two buckets, taken from the top: thread t covers b = 2047-2t, 2046-2t
     const int b0 = kBuckets - 1 - 2 * t, b1 = b0 - 1;
     const int c0 = count[b0], c1 = count[b1];
@@ -68,20 +139,26 @@ __global__ void __launch_bounds__(1024) scan_kernel(const int* __restrict__ coun
     const int before = part[t] - (c0 + c1);
     cursor[b0] = before;
     cursor[b1] = before + c0;
+    if (t == 0) {  // every thread read its counts before the first barrier
+        count[0] = (int)range[0];
+        count[1] = (int)bucket_shift(range[0], range[1]);
+    }
 }
 
 // each block reserves one contiguous range per non-empty bucket with a single global atomic,
 // then places its trajectories inside the ranges through LDS atomics
 __global__ void __launch_bounds__(256) scatter_kernel(const float* t0s, const float* tfs, float t0, float tf,
-                                                      long long n, int* __restrict__ cursor, int* __restrict__ order) {
+                                                      long long n, int* __restrict__ cursor,
+                                                      const int* __restrict__ count, int* __restrict__ order) {
     __shared__ int h[kBuckets];
     __shared__ int start[kBuckets];
+    const unsigned bmin = (unsigned)count[0], shift = (unsigned)count[1];
     for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) h[b] = 0;
     __syncthreads();
     const long long base = (long long)blockIdx.x * kPerBlock;
     for (int j = threadIdx.x; j < kPerBlock; j += blockDim.x) {
         const long long i = base + j;
-        if (i < n) atomicAdd(&h[span_bucket(t0s, tfs, t0, tf, i)], 1);
+        if (i < n) atomicAdd(&h[span_bucket(span_at(t0s, tfs, t0, tf, i), bmin, shift)], 1);
     }
     __syncthreads();
     for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) {
@@ -92,7 +169,7 @@ __global__ void __launch_bounds__(256) scatter_kernel(const float* t0s, const fl
     for (int j = threadIdx.x; j < kPerBlock; j += blockDim.x) {
         const long long i = base + j;
         if (i < n) {
-            const int b = span_bucket(t0s, tfs, t0, tf, i);
+            const int b = span_bucket(span_at(t0s, tfs, t0, tf, i), bmin, shift);
             order[start[b] + atomicAdd(&h[b], 1)] = (int)i;
         }
     }
@@ -134,11 +211,14 @@ extern "C" int deg_order_by_span(const float* t0s, const float* tfs, float t0, f
     int* count = work;
     int* cursor = work + kBuckets;
     const unsigned blocks = (unsigned)((n + kPerBlock - 1) / kPerBlock);
-    hipLaunchKernelGGL(clear_kernel, dim3((2 * kBuckets + 255) / 256), dim3(256), 0, st, work);
-    hipLaunchKernelGGL(histogram_kernel, dim3(blocks), dim3(256), 0, st, t0s, tfs, t0, tf, (long long)n, count);
+    unsigned* range = (unsigned*)cursor;  // (bmin, bmax) until the scan
+    hipLaunchKernelGGL(clear_kernel, dim3(kBuckets / 256), dim3(256), 0, st, work);
+    hipLaunchKernelGGL(range_kernel, dim3(blocks), dim3(256), 0, st, t0s, tfs, t0, tf, (long long)n, range);
+    hipLaunchKernelGGL(histogram_kernel, dim3(blocks), dim3(256), 0, st, t0s, tfs, t0, tf, (long long)n, count,
+                       (const unsigned*)range);
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, count, cursor);
     hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(256), 0, st, t0s, tfs, t0, tf, (long long)n, cursor,
-                       order);
+                       (const int*)count, order);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return deg_rng_fail(DEG_E_HIP, (std::string("deg_order_by_span: ") + hipGetErrorString(e)).c_str());
     return DEG_OK;

@@ -243,8 +243,9 @@ int deg_draw_normal(float *out, int64_t n, int64_t id_offset, uint64_t seed, uin
 
 /* Longest-first order of trajectories for the persistent queue: writes order[0..n) with a
  * permutation of 0..n-1 sorted by descending span tfs[i] - t0s[i] (NULL arrays: the shared t0/tf),
- * bucketed by the span's leading float bits (2048 buckets, 1/8-octave resolution; no order
- * within a bucket). work: 4096 int32 of device memory. Asynchronous on `stream`, graph-capturable.
+ * in 2047 buckets that split the range of the positive spans' float bit patterns evenly (about
+ * 1/2000 octave for spans within a factor 2; no order within a bucket); zero, negative and NaN spans
+ * come last. work: 4096 int32 of device memory. Asynchronous on `stream`, graph-capturable.
  * Passed as deg_solve_args.order it shortens the tail in which lanes of finished trajectories
  * idle (no counterpart in the reference, which launches one thread per trajectory). */
 int deg_order_by_span(const float *t0s, const float *tfs, float t0, float tf, int64_t n, int32_t *order,

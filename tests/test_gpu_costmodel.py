@@ -3,12 +3,13 @@
 Bars:
   * the device key is 2^(w . x) with x the host features of the same trajectory (span / h0 from the
     kernel's own key) to 1e-4 relative (hardware log2 / exp2);
-  * the order is a permutation in descending key buckets, and a persistent solve started in it equals
+  * the order is a permutation in descending key buckets (the range-adaptive buckets of deg_order_by_span), and a persistent solve started in it equals
     the index-order solve bit for bit;
   * fitted on a held-out draw of the Lorenz law of the 1M adaptive workload, the model ranks the attempt
     counts of another draw better than span / h0 does (Spearman >= 0.9, against about 0.8);
   * a model of the wrong length is refused.
 """
+import _orderbuckets as OB
 import numpy as np
 import pytest
 
@@ -63,8 +64,7 @@ def test_cost_model_key_order_and_results(deg, torch_cuda):
     X = costmodel.features(h0key, test["u0"], test["p"], np.full(test["N"], 10.0))
     np.testing.assert_allclose(mkey, 2.0 ** (X @ np.asarray(w, np.float64)), rtol=1e-4)
     o = outs["model"].order_auto.cpu().numpy()
-    assert np.array_equal(np.sort(o), np.arange(test["N"]))
-    assert np.all(np.diff((mkey.astype(F32).view(np.uint32) >> 20)[o].astype(np.int64)) <= 0)
+    OB.assert_descending(outs["model"].order_key.cpu().numpy(), o)
     # results do not depend on the start order
     for x, y in ((outs["none"].ts, outs["model"].ts), (outs["none"].us_soa, outs["model"].us_soa),
                  (outs["none"].naccept, outs["model"].naccept), (outs["none"].nreject, outs["model"].nreject)):

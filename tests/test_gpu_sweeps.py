@@ -9,6 +9,7 @@ Bars:
   * with the deterministic controller pow and no contraction, each sweep's results equal the
     CPU oracle on the host-drawn parameters of that sweep, bit for bit.
 """
+import _orderbuckets as OB
 import numpy as np
 import pytest
 
@@ -136,9 +137,7 @@ def test_order_by_span_and_order_independence(deg, torch_cuda):
     order = deg.order_by_span(probs)
     torch.cuda.synchronize()
     o = order.cpu().numpy()
-    assert np.array_equal(np.sort(o), np.arange(N))
-    bucket = tfs.view(np.uint32) >> 20
-    assert np.all(np.diff(bucket[o].astype(np.int64)) <= 0)
+    OB.assert_descending(tfs, o)
     kw = dict(dt=0.01, abstol=1e-8, reltol=1e-8)
     outs = []
     for ordr in (None, order, torch.as_tensor(np.arange(N - 1, -1, -1, dtype=np.int32), device="cuda")):
@@ -218,8 +217,7 @@ def test_cost_start_order(deg, torch_cuda):
                 assert r.order_auto is not None
                 key = r.order_key.cpu().numpy()
                 o = r.order_auto.cpu().numpy()
-                assert np.array_equal(np.sort(o), np.arange(N))
-                assert np.all(np.diff((key.view(np.uint32) >> 20)[o].astype(np.int64)) <= 0)
+                OB.assert_descending(key, o)
                 # the key restated in float64 from the inputs
                 x = u0.astype(np.float64)
                 s_, r_, b_ = p0.astype(np.float64)
