@@ -88,6 +88,7 @@ class SolveArgs(ctypes.Structure):
         ("seeds", ctypes.c_void_p),
         ("seed", ctypes.c_uint64),
         ("traj_base", ctypes.c_int64),
+        ("inputs_ordered", ctypes.c_void_p),
     ]
 
 

@@ -170,6 +170,12 @@ class HIPDevice:
                   measures them). An explicit ``order=`` overrides it. Results do not depend on the order.
       cost_model  the 3 + n + np weights of a start-order cost model (costmodel.fit), for
                   start_order="model".
+      ordered_inputs with a cost or model start order: the ordering pass also copies u0 and p into
+                  the order in which the queues hand the trajectories out, so lanes read their
+                  inputs contiguously instead of gathering them (deg_solve_args.inputs_ordered;
+                  same results, (n + np) floats per trajectory of device memory). It cuts the
+                  solver's input reads to about their algorithmic size but costs the scatter about
+                  as much time as the gathers it removes, so it is off by default.
     """
 
     device: int = 0
@@ -187,6 +193,7 @@ class HIPDevice:
     batch_saves: bool = False
     start_order: str | None = None
     cost_model: tuple | None = None
+    ordered_inputs: bool = False
 
     def cost_order(self, adaptive: bool) -> bool:
         """Whether solves on this device start their trajectories in cost order (by the span / initial

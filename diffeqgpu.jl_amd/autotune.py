@@ -14,7 +14,9 @@ lowest measured kernel time:
                    where it points the wrong way).
 The LDS depth of the saveat cache needs no tuning: it is sized to the save grid at every launch.
 
-The search is coordinate descent in that order, starting from the defaults. Each candidate is
+The search is coordinate descent, starting from the defaults: the start order first (it changes
+which launch shape is best), then the other axes in the order above, then the start order again.
+Each candidate is
 timed with HIP events around full relaunches of the solve (start-order kernels included; the best
 of ``repeats``) on the caller's problems, so the result reflects the real trajectory mix. Results are cached in
 ``_jitcache/autotune.json``, keyed by the traced source, n, np, method, flags and the ensemble
@@ -55,7 +57,9 @@ def _key(probs, alg, adaptive, nsave_hint, learned: bool = False) -> str:
                                start_order=None)
     nb = int(round(math.log2(max(1, probs.N))))
     flags = base.flags(adaptive) | (_lib.F_LEARNED if learned else 0)
-    return f"{h}:{probs.n}:{probs.np}:{alg.method}:{flags}:{nb}:{nsave_hint}"
+    # a device that carries a cost model can also be tuned to start in its order: a separate entry
+    model = ":model" if probs.dev.cost_model and adaptive and bool(base.flags(adaptive) & _lib.F_PERSISTENT) else ""
+    return f"{h}:{probs.n}:{probs.np}:{alg.method}:{flags}:{nb}:{nsave_hint}{model}"
 
 
 def _read(path) -> dict:
@@ -137,11 +141,16 @@ def tune(probs, prob, alg, *, adaptive: bool = True, repeats: int = 2, use_cache
     if persistent and adaptive:
         default["start_order"] = "none"
     cfg = dict(default)
-    axes = [("block_size", BLOCKS), ("waves", WAVES)]
+    orders = ("none", "cost", "model") if probs.dev.cost_model else ("none", "cost")
+    axes = []
+    if persistent and adaptive:
+        # the start order changes which launch shape is best: chosen first, and checked again last
+        axes += [("start_order", orders)]
+    axes += [("block_size", BLOCKS), ("waves", WAVES)]
     if persistent:
         axes += [("refill", REFILL), ("chunk", CHUNK), ("blocks_per_cu", None)]
     if persistent and adaptive:
-        axes += [("start_order", ("none", "cost", "model") if probs.dev.cost_model else ("none", "cost"))]
+        axes += [("start_order", orders)]
     for name, values in axes:
         if values is None:
             values = blocks_per_cu_candidates(cfg["block_size"])

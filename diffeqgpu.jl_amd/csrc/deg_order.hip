@@ -145,11 +145,25 @@ __global__ void __launch_bounds__(1024) scan_kernel(int* __restrict__ count, int
     }
 }
 
+// Optional payload of the scatter: columns of per-trajectory values (two source arrays of ncol0 and
+// ncol1 columns, leading dimension src_stride, entry i of column c at src[c * src_stride + i]) copied
+// into dst in queue-major order position: order position k goes to column index (k mod 8) * slab + k / 8
+// with slab = ceil(n / 8), columns 8 * slab apart. The persistent solver's queue q serves positions
+// q, q + 8, q + 16, ...; in this layout each queue reads its trajectories' inputs contiguously.
+struct order_payload {
+    const float* src0;
+    const float* src1;
+    long long src_stride;
+    int ncol0, ncol1;
+    float* dst;
+};
+
 // each block reserves one contiguous range per non-empty bucket with a single global atomic,
 // then places its trajectories inside the ranges through LDS atomics
 __global__ void __launch_bounds__(256) scatter_kernel(const float* t0s, const float* tfs, float t0, float tf,
                                                       long long n, int* __restrict__ cursor,
-                                                      const int* __restrict__ count, int* __restrict__ order) {
+                                                      const int* __restrict__ count, int* __restrict__ order,
+                                                      const order_payload pl) {
     __shared__ int h[kBuckets];
     __shared__ int start[kBuckets];
     const unsigned bmin = (unsigned)count[0], shift = (unsigned)count[1];
@@ -170,7 +184,16 @@ __global__ void __launch_bounds__(256) scatter_kernel(const float* t0s, const fl
         const long long i = base + j;
         if (i < n) {
             const int b = span_bucket(span_at(t0s, tfs, t0, tf, i), bmin, shift);
-            order[start[b] + atomicAdd(&h[b], 1)] = (int)i;
+            const int pos = start[b] + atomicAdd(&h[b], 1);
+            order[pos] = (int)i;
+            if (pl.dst != nullptr) {
+                // coalesced reads of this block's trajectories, one store per column at their positions
+                const long long slab = (n + 7) / 8;
+                float* d = pl.dst + (long long)(pos & 7) * slab + (pos >> 3);
+                for (int c = 0; c < pl.ncol0; ++c) d[(long long)c * 8 * slab] = pl.src0[(long long)c * pl.src_stride + i];
+                for (int c = 0; c < pl.ncol1; ++c)
+                    d[(long long)(pl.ncol0 + c) * 8 * slab] = pl.src1[(long long)c * pl.src_stride + i];
+            }
         }
     }
 }
@@ -202,8 +225,13 @@ int deg_noalloc_scratch(int device, void** out) {
     return DEG_OK;
 }
 
-extern "C" int deg_order_by_span(const float* t0s, const float* tfs, float t0, float tf, int64_t n, int32_t* order,
-                                 int32_t* work, void* stream) {
+// deg_order_by_span, and with payload columns (src0: ncol0 columns, src1: ncol1 columns, leading
+// dimension src_stride) also copied to dst ((ncol0 + ncol1) columns of 8 * ceil(n / 8) floats) in
+// queue-major order position (see order_payload). Used by the start orders of deghip.cpp to lay out
+// each trajectory's inputs in the order the persistent solver takes them.
+int deg_order_payload(const float* t0s, const float* tfs, float t0, float tf, int64_t n, int32_t* order,
+                      int32_t* work, void* stream, const float* src0, int ncol0, const float* src1, int ncol1,
+                      long long src_stride, float* dst) {
     if (n < 0 || n > 0x7fffffffll) return deg_rng_fail(DEG_E_INVALID, "n must be in 0..2^31-1");
     if (n == 0) return DEG_OK;
     if (!order || !work) return deg_rng_fail(DEG_E_INVALID, "order and work are required");
@@ -217,9 +245,21 @@ extern "C" int deg_order_by_span(const float* t0s, const float* tfs, float t0, f
     hipLaunchKernelGGL(histogram_kernel, dim3(blocks), dim3(256), 0, st, t0s, tfs, t0, tf, (long long)n, count,
                        (const unsigned*)range);
     hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, count, cursor);
+    order_payload pl;
+    pl.src0 = src0;
+    pl.src1 = src1;
+    pl.src_stride = src_stride;
+    pl.ncol0 = src0 ? ncol0 : 0;
+    pl.ncol1 = src1 ? ncol1 : 0;
+    pl.dst = dst;
     hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(256), 0, st, t0s, tfs, t0, tf, (long long)n, cursor,
-                       (const int*)count, order);
+                       (const int*)count, order, pl);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return deg_rng_fail(DEG_E_HIP, (std::string("deg_order_by_span: ") + hipGetErrorString(e)).c_str());
     return DEG_OK;
+}
+
+extern "C" int deg_order_by_span(const float* t0s, const float* tfs, float t0, float tf, int64_t n, int32_t* order,
+                                 int32_t* work, void* stream) {
+    return deg_order_payload(t0s, tfs, t0, tf, n, order, work, stream, nullptr, 0, nullptr, 0, 0, nullptr);
 }

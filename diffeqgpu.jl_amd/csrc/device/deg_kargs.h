@@ -37,4 +37,5 @@ struct deg_kargs {
     const unsigned long long* seeds;  // SDE: [stride] per-trajectory seeds, or null -> seed
     unsigned long long seed;          // SDE: shared seed
     long long traj_base;              // SDE: ensemble index of column 0 (the noise stream id is traj_base + column)
+    const float* inputs_ordered;      // optional with order: (n + np) columns of 8 ceil(n_traj / 8), queue-major
 };

@@ -1343,12 +1343,13 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
     // saveat times staged in LDS: reading them then needs no vector-memory wait (which on
     // CDNA would also wait for every store still in flight)
     extern __shared__ float deg_dyn[];  // sized at launch: saveat times, then the save stash
+    const float finf_lds = __builtin_inff();
 #if DEG_SAVEAT_GLOBAL
     const float* __restrict__ svp = a.saveat;  // more save points than the LDS cache holds
 #else
-    float* svp = deg_dyn;  // nsaveat floats
+    float* svp = deg_dyn;  // nsaveat floats and a +inf sentinel
     if (a.saveat != nullptr) {
-        for (int j = threadIdx.x; j < a.nsaveat; j += blockDim.x) svp[j] = a.saveat[j];
+        for (int j = threadIdx.x; j <= a.nsaveat; j += blockDim.x) svp[j] = j < a.nsaveat ? a.saveat[j] : finf_lds;
     }
     __syncthreads();
 #endif
@@ -1357,7 +1358,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
 #if DEG_SAVEAT_GLOBAL
     float* const stash = deg_dyn + (long long)threadIdx.x * DEG_STASH;
 #else
-    float* const stash = deg_dyn + ((a.nsaveat + 3) & ~3) + threadIdx.x * DEG_STASH;
+    float* const stash = deg_dyn + ((a.nsaveat + 4) & ~3) + threadIdx.x * DEG_STASH;
 #endif
     bool stashed = false;  // the stash holds saves [plo, phi) not yet written
     int plo = 0, phi = 0;
@@ -1604,8 +1605,20 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
         if (oi >= 0) {
             // ---- init (nonstiff/types.jl:305-402, stiff/types.jl:32-177; kernels.jl:35-45, :116-126)
             i = a.id_offset + oi;
-            DEG_FOR(c) u[c] = a.u0[(long long)c * S + i];
-            _Pragma("unroll") for (int j = 0; j < DEG_NP; ++j) p[j] = a.p[(long long)j * S + i];
+#if DEG_PERSISTENT
+            if (a.inputs_ordered != nullptr) {
+                // the inputs of start-order position gid, laid out queue-major by the ordering pass: the
+                // lanes that take consecutive entries of one queue read consecutive words
+                const long long slab8 = (a.n_traj + DEG_NQUEUE - 1) / DEG_NQUEUE;
+                const float* q = a.inputs_ordered + (gid % DEG_NQUEUE) * slab8 + gid / DEG_NQUEUE;
+                DEG_FOR(c) u[c] = q[(long long)c * DEG_NQUEUE * slab8];
+                _Pragma("unroll") for (int j = 0; j < DEG_NP; ++j) p[j] = q[(long long)(DEG_N + j) * DEG_NQUEUE * slab8];
+            } else
+#endif
+            {
+                DEG_FOR(c) u[c] = a.u0[(long long)c * S + i];
+                _Pragma("unroll") for (int j = 0; j < DEG_NP; ++j) p[j] = a.p[(long long)j * S + i];
+            }
             t = a.t0s ? a.t0s[i] : a.t0;
             tf = a.tfs ? a.tfs[i] : a.tf;
             tprev = t;
@@ -2047,6 +2060,13 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                 if (use_saveat) {
 #if DEG_HAS_INTERP && !DEG_BATCH_SAVES
                     while (tnext <= t) {
+                        // the following save time is read first, so that its load latency overlaps
+                        // the interpolation (the LDS copy ends with a +inf sentinel at nsaveat)
+#if DEG_SAVEAT_GLOBAL
+                        const float tafter = (cur + 1 < a.nsaveat) ? svp[cur + 1] : finf;
+#else
+                        const float tafter = svp[min(cur + 1, a.nsaveat)];
+#endif
                         float out[DEG_N];
 #if DEG_PARITY
                         const float th = (tnext - tprev) / dtacc;
@@ -2061,7 +2081,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
                         ++cur;
                         ts_w += S;
                         us_w += (long long)DEG_N * S;
-                        tnext = (cur < a.nsaveat) ? svp[cur] : finf;
+                        tnext = tafter;
                     }
 #endif
                 } else if (a.save_everystep) {

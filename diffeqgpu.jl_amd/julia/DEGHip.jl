@@ -44,7 +44,7 @@ const SAVEAT_LDS_CAP = 4096
 
 # mirror of deg_solve_args; its size is checked against deg_solve_args_size() at first use
 Base.@kwdef mutable struct SolveArgs
-    struct_size::UInt32 = 272
+    struct_size::UInt32 = 280
     n::Int32 = 0
     n_traj::Int64 = 0
     stride::Int64 = 0
@@ -86,6 +86,7 @@ Base.@kwdef mutable struct SolveArgs
     seeds::Ptr{UInt64} = C_NULL
     seed::UInt64 = 0
     traj_base::Int64 = 0
+    inputs_ordered::Ptr{Float32} = C_NULL
 end
 
 last_error() = unsafe_string(ccall((:deg_last_error, libdeghip), Cstring, ()))

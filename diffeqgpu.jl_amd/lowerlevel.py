@@ -250,7 +250,7 @@ def _launch(probs: HIPProbVector, alg, adaptive: bool, *, dt, abstol, reltol, sa
     if mlp is not None:
         a.mlp = mlp.data_ptr()
         a.mlp_every = int(mlp_every)
-    okey = owork = omodel = None
+    okey = owork = omodel = oinp = None
     if order is None and not sde and dev.cost_order(adaptive) and N > 0:
         # cost-ordered start, recomputed on the device before every launch of this solve
         okey, order, owork = (outputs.get(k) for k in ("order_key", "order_auto", "order_work"))
@@ -268,6 +268,15 @@ def _launch(probs: HIPProbVector, alg, adaptive: bool, *, dt, abstol, reltol, sa
             owork = torch.empty(_lib.ORDER_WORK_INTS, dtype=torch.int32, device=device)
         if okey.numel() != N or order.numel() != N or owork.numel() != _lib.ORDER_WORK_INTS:
             raise ValueError("preallocated start-order buffers have the wrong size")
+        if dev.ordered_inputs:
+            # the ordering pass also lays out u0 and p in the order the queues hand the trajectories out
+            ninp = (n + probs.np) * 8 * ((N + 7) // 8)
+            oinp = outputs.get("inputs_ordered")
+            if oinp is None:
+                oinp = torch.empty(ninp, dtype=torch.float32, device=device)
+            if oinp.numel() != ninp:
+                raise ValueError("preallocated inputs_ordered has the wrong size")
+            a.inputs_ordered = oinp.data_ptr()
     if order is not None:
         if order.dtype != torch.int32 or order.numel() != N or order.device != device:
             raise ValueError("order must be an int32 tensor of N trajectory indices on the problems' device")
@@ -313,7 +322,7 @@ def _launch(probs: HIPProbVector, alg, adaptive: bool, *, dt, abstol, reltol, sa
     return SolveResult(ts, us_soa.permute(0, 2, 1), retcode=retcode, naccept=nacc, nreject=nrej, us_soa=us_soa,
                        kernel_ms=ms, saveat=sv, kernel=kern, flags=flags, relaunch=relaunch, args=a,
                        probs=probs, tstops=tst, order_key=okey, order_auto=order if okey is not None else None,
-                       order_work=owork, cost_model=omodel)
+                       order_work=owork, cost_model=omodel, inputs_ordered=oinp)
 
 
 def vectorized_solve(probs, prob, alg, *, dt, saveat=None, save_everystep=True, debug=False, callback=None,

@@ -81,7 +81,7 @@ class ParameterSweep:
         self.stream.synchronize()
         self.outputs = [dict(ts=r.ts, us_soa=r.us_soa, retcode=r.retcode, naccept=r.naccept, nreject=r.nreject,
                              saveat=r.saveat, order_key=r.order_key, order_auto=r.order_auto,
-                             order_work=r.order_work, cost_model=r.cost_model) for r in self.results]
+                             order_work=r.order_work, cost_model=r.cost_model, inputs_ordered=r.inputs_ordered) for r in self.results]
         # capture
         sp = self.stream.cuda_stream
         _lib.Graph.begin(sp)

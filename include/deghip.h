@@ -137,6 +137,14 @@ typedef struct deg_solve_args {
     uint64_t seed;           /* DEG_EM / DEG_SIEA: the seed of every trajectory when seeds is NULL */
     int64_t traj_base;       /* DEG_EM / DEG_SIEA: ensemble index of column 0; the noise of column i is the
                                 stream traj_base + i, so a chunked ensemble draws what one launch would */
+    float *inputs_ordered;   /* optional, persistent kernels with order: device array of (n + np) columns of
+                                8 * ceil(n_traj / 8) floats, the trajectories' u0 then p values in queue-major
+                                start-order position (order entry k at column index (k mod 8) * ceil(n_traj / 8)
+                                + k / 8: the entries each of the 8 queues serves, contiguously).
+                                deg_start_order / deg_start_order_model fill it when set, and deg_solve then
+                                reads each trajectory's inputs from it in the order the queues hand them out
+                                instead of gathering them from u0 and p (no counterpart in the reference).
+                                NULL: inputs are read from u0 and p */
 } deg_solve_args;
 
 /* Context: one device (launches go to the stream given per call). */

@@ -79,3 +79,45 @@ def test_cost_model_key_order_and_results(deg, torch_cuda):
     with pytest.raises(_lib.DegError, match="3 \\+ n \\+ np"):
         r.kernel.start_order_model(r.args, r.cost_model.data_ptr(), 4, r.order_key.data_ptr(),
                                    r.order_auto.data_ptr(), r.order_work.data_ptr())
+
+
+def test_ordered_inputs_layout_and_results(deg, torch_cuda):
+    """With a cost order the ordering pass also writes u0 and p in queue-major order position
+    (deg_solve_args.inputs_ordered): entry k of the order at column index (k mod 8) * ceil(N / 8) + k / 8.
+    The solve that reads them equals, bit for bit, the one that gathers from u0 and p; N not a
+    multiple of 8 exercises the padded columns. An inputs_ordered without an order is refused."""
+    torch = torch_cuda
+    from diffeqgpu_amd import _lib, systems, workloads
+
+    f, n, np_, _ = systems.SYSTEMS["lorenz"]
+    kw = dict(dt=0.1, abstol=1e-6, reltol=1e-6, saveat=1.0)
+    for N in (20_003, 64):
+        cfg = workloads.config("lorenz_adaptive", N=N, seed=31)
+        prob = deg.ODEProblem(f, cfg["u0"][:, 0], cfg["tspan"], cfg["p"][:, 0])
+        outs = {}
+        for ordered in (True, False):
+            print(f"[gpu] ordered inputs N={N} {ordered}", flush=True)
+            dev = deg.HIPDevice(persistent=True, start_order="cost", ordered_inputs=ordered)
+            probs = deg.HIPProbVector.from_arrays(f, cfg["u0"], cfg["p"], cfg["tspan"], dev)
+            r = deg.vectorized_asolve(probs, prob, deg.GPUTsit5(), **kw)
+            torch.cuda.synchronize()
+            outs[ordered] = r
+        r = outs[True]
+        assert outs[False].inputs_ordered is None and r.inputs_ordered is not None
+        slab = (N + 7) // 8
+        inp = r.inputs_ordered.cpu().numpy().reshape(n + np_, 8 * slab)
+        o = r.order_auto.cpu().numpy()
+        k = np.arange(N)
+        col = (k % 8) * slab + k // 8
+        np.testing.assert_array_equal(inp[:n, col], cfg["u0"][:, o])
+        np.testing.assert_array_equal(inp[n:, col], cfg["p"][:, o])
+        for name in ("ts", "us_soa", "retcode", "naccept", "nreject"):
+            np.testing.assert_array_equal(getattr(outs[True], name).cpu().numpy(),
+                                          getattr(outs[False], name).cpu().numpy())
+    # refused without an order
+    a = r.args
+    saved = a.order
+    a.order = None
+    with pytest.raises(_lib.DegError, match="inputs_ordered needs an order"):
+        r.kernel.solve(a)
+    a.order = saved

@@ -44,7 +44,8 @@ def test_solve_args_offsets_match_header(deg):
     assert off["ts"] == 112 and off["stream"] == 160 and off["steplog_cap"] == 208 and off["chunk"] == 216
     assert off["order"] == 224 and off["tstops"] == 232 and off["ntstops"] == 240
     assert off["seeds"] == 248 and off["seed"] == 256 and off["traj_base"] == 264
-    assert ctypes.sizeof(_lib.SolveArgs) == 272
+    assert off["inputs_ordered"] == 272
+    assert ctypes.sizeof(_lib.SolveArgs) == 280
 
 
 @pytest.mark.parametrize("method", [0, 1, 2, 3, 4, 5, 6, 7])
