@@ -40,6 +40,10 @@ Persistent adaptive workloads (not the sweeps) fit the start-order cost model (c
 200,000-trajectory held-out draw before the timed region; the autotuner then also measures its order
 (the shipped table holds the MI355X result for the benchmark workloads).
 
+Before the W warmup steps the step is relaunched, untimed, for --settle-ms (300) of GPU time: the
+MI355X's clocks rise over the first ~0.1 s of sustained load, and without this the K timed steps of a
+short run would partly measure the ramp (reported as config.clock_settle; --settle-ms 0 turns it off).
+
 Options: --gpus N (1 only: one GPU), --steps K, --warmup W, --workload, --n-traj, --no-autotune,
 --copy-probe (one 256 MiB device copy first, to calibrate HBM counters under rocprofv3),
 --no-cpu (skip the CPU baseline), --no-extras (skip the end-to-end and compile timings).
@@ -122,6 +126,9 @@ def parse():
                          "16 consecutive ids, and the true attempt counts as a bound); adds 'start_orders'")
     ap.add_argument("--mlp-every", type=int, default=2, help="learned leg: evaluate the proposer every k iterations")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--settle-ms", type=float, default=300.0,
+                    help="untimed relaunches of the step, before the warmup steps, until this much GPU time has "
+                         "passed (MI355X clocks rise over the first ~0.1 s of sustained load); 0: none")
     return ap.parse_args()
 
 
@@ -144,6 +151,33 @@ def flop_model(method, n, fc):
     if method == "GPUVern7":
         return 116 * n + 24 + 10 * F, 0, 0
     return 25 * n + 8 + 3 * F + FJ + FT, n * n + n + (2 * n ** 3) // 3 + 6 * n * n, 4 * n + 8
+
+
+def settle_clocks(torch, run, stream, settle_ms, max_relaunches=2000):
+    """Relaunch the step, untimed, until settle_ms of GPU time has passed (at least once when
+    settle_ms > 0): the GPU's clocks rise over the first ~0.1 s of sustained load, and the timed steps
+    should see the steady state. Returns what was done, for the bench line."""
+    if settle_ms <= 0:
+        return {"relaunches": 0, "ms": 0.0}
+    a_, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a_.record(stream)
+    run()
+    b_.record(stream)
+    b_.synchronize()
+    n = 1
+    elapsed = per = a_.elapsed_time(b_)
+    while elapsed < settle_ms and n < max_relaunches:
+        # batches of back-to-back relaunches sized to the time left (one synchronisation per batch)
+        k = max(1, min(64, int((settle_ms - elapsed) / max(per, 1e-3))))
+        for _ in range(k):
+            run()
+        n += k
+        b_.record(stream)
+        b_.synchronize()
+        elapsed = a_.elapsed_time(b_)
+    return {"relaunches": n, "ms": elapsed,
+            "what": "untimed relaunches of the same step before the warmup steps, so that the timed steps run at "
+                    "settled clocks"}
 
 
 def _tile_order(tile_key, N, T=16, Q=8):
@@ -502,6 +536,7 @@ def main_sde(args, name):
     stream = torch.cuda.current_stream(torch.device("cuda", 0))
     res = deg.vectorized_solve(probs, prob, alg, dt=dt, save_everystep=False)
     run = res.relaunch
+    settle = settle_clocks(torch, run, stream, args.settle_ms)
     for _ in range(warmup):
         run()
     torch.cuda.synchronize()
@@ -535,7 +570,7 @@ def main_sde(args, name):
         "dtype": "f32",
         "data": "synthetic (the reference's SDE test problem; Philox4x32-10 Wiener increments, seed 123)",
         "config": {"workload": spec["desc"], "trajectories_per_step": N, "kernel_flags": res.flags,
-                   "block_size": dev.block_size or 256},
+                   "block_size": dev.block_size or 256, "clock_settle": settle},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel_ms": float(np.mean(kernel_ms)),
                      "sde_steps_per_launch": nsteps, "flops_per_step": fstep, "philox_int_ops_per_step": istep,
@@ -668,6 +703,7 @@ def main():
         per_step = N
         results = [res]
 
+    settle = settle_clocks(torch, run, stream, args.settle_ms)
     for _ in range(warmup):
         run()
     torch.cuda.synchronize()
@@ -719,11 +755,14 @@ def main():
                    "waves": dev.waves, "refill": dev.refill or 4, "chunk": dev.chunk or 16, "autotuned": tuned,
                    "start_order": ("span" if name == "lorenz_uneven" and args.start_order == "span" and not args.no_order
                                    else dev.start_order),
-                   "cost_model": cost_model},
+                   "cost_model": cost_model, "clock_settle": settle},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "kernel_ms": float(np.mean(kernel_ms)), "flops_per_step": flops,
+                     "kernel_ms": float(np.mean(kernel_ms)),
+                     "kernel_ms_min_median_max": [float(np.min(kernel_ms)), float(np.median(kernel_ms)),
+                                                  float(np.max(kernel_ms))],
+                     "flops_per_step": flops,
                      "attempts_per_step": attempts, "saves_per_step": nsave_interp,
                      "flops_per_attempt": {"fp32": fa32, "fp64": fa64}, "rhs_flops": fc},
         "fp64_oracle_error": err,

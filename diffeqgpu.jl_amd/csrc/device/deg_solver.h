@@ -594,6 +594,14 @@ __device__ __forceinline__ void w_dual(deg_wf& F, double beta, const float (&J)[
 // so W stays in VGPRs. With DEG_PARITY every division is an IEEE double division, as the oracle does;
 // otherwise the factor keeps reciprocals (1 / det, or 1 / U_kk on the diagonal), so a solve divides
 // nowhere.
+// 1 / d in FP64 for the contraction-allowed mode: the hardware reciprocal estimate refined by two
+// Newton steps (r += r (1 - d r), each doubling the correct bits), in place of the IEEE division
+// sequence; within a few units in the last place of the division (parity mode divides).
+__device__ __forceinline__ double deg_recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
+    return __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
+}
 __device__ __forceinline__ void w_lu(deg_wf& F) {
     auto& W = F.W;
 #if DEG_CRAMER
@@ -623,7 +631,7 @@ __device__ __forceinline__ void w_lu(deg_wf& F) {
     W[2][2] = a00 * a11 - a01 * a10;
 #endif
 #if !DEG_PARITY
-    F.det = 1.0 / F.det;
+    F.det = deg_recip(F.det);
 #endif
 #else
     _Pragma("unroll") for (int kk = 0; kk < DEG_JC; ++kk) {
@@ -647,7 +655,7 @@ __device__ __forceinline__ void w_lu(deg_wf& F) {
             }
         }
 #if !DEG_PARITY
-        const double rp = 1.0 / W[kk][kk];
+        const double rp = deg_recip(W[kk][kk]);
 #endif
         _Pragma("unroll") for (int i = kk + 1; i < DEG_JC; ++i) {
 #if DEG_PARITY
