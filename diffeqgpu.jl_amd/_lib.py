@@ -31,6 +31,7 @@ F_COMPACT = 1 << 12
 F_BATCH_SAVES = 1 << 13
 F_TSTOPS = 1 << 14
 F_CALLBACKS = 1 << 15
+F_ORDERED_INPUTS = 1 << 16
 SAVEAT_LDS_CAP = 4096
 
 RET_SUCCESS, RET_DTMIN, RET_MAXITERS, RET_NONFINITE, RET_TERMINATED, RET_FAILURE = 0, 1, 2, 3, 4, 5

@@ -243,6 +243,8 @@ class HIPDevice:
             f |= _lib.F_COMPACT
         if self.batch_saves and adaptive:
             f |= _lib.F_BATCH_SAVES
+        if self.ordered_inputs and adaptive and persistent and (self.start_order or "none") in ("cost", "model"):
+            f |= _lib.F_ORDERED_INPUTS
         if not 0 <= self.waves <= 7:
             raise ValueError("waves must be in 0..7")
         f |= self.waves << _lib.F_WAVES_SHIFT

@@ -1245,6 +1245,10 @@ __device__ __forceinline__ float deg_advance(float t, float dt, float& comp) {
 #ifndef DEG_COMPACT_LANES
 #define DEG_COMPACT_LANES 24
 #endif
+// the start order's inputs laid out in claim order (deg_solve_args.inputs_ordered)
+#ifndef DEG_ORDERED_INPUTS
+#define DEG_ORDERED_INPUTS 0
+#endif
 // Batched dense output (DEG_BATCH_SAVES, adaptive Tsit5): a lane whose accepted step crosses
 // save points copies the step's dense-output data to its LDS stash; the wave interpolates
 // for all stashed lanes together once DEG_SAVE_BATCH lanes wait (or a lane needs its stash
@@ -1613,7 +1617,7 @@ extern "C" __global__ void __launch_bounds__(DEG_BLOCK_MAX) DEG_ATTR_WAVES DEG_K
         if (oi >= 0) {
             // ---- init (nonstiff/types.jl:305-402, stiff/types.jl:32-177; kernels.jl:35-45, :116-126)
             i = a.id_offset + oi;
-#if DEG_PERSISTENT
+#if DEG_PERSISTENT && DEG_ORDERED_INPUTS
             if (a.inputs_ordered != nullptr) {
                 // the inputs of start-order position gid, laid out queue-major by the ordering pass: the
                 // lanes that take consecutive entries of one queue read consecutive words

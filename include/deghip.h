@@ -72,6 +72,9 @@ extern "C" {
 /* the traced source defines callbacks (DEG_NDCB / DEG_NCCB and their functions, see
  * deg_kernel_compile), applied after every accepted step (integrator_utils.jl:274-325) */
 #define DEG_F_CALLBACKS (1u << 15)
+/* persistent adaptive ODE kernels: read each trajectory's inputs from deg_solve_args.inputs_ordered when it
+ * is set (the code path is compiled in only with this flag) */
+#define DEG_F_ORDERED_INPUTS (1u << 16)
 /* register cap: compile for at least w (1..7) resident waves per SIMD (bits 9..11; 0: no cap) */
 #define DEG_F_WAVES_SHIFT 9
 #define DEG_F_WAVES(w) ((uint32_t)(w) << DEG_F_WAVES_SHIFT)
@@ -137,7 +140,7 @@ typedef struct deg_solve_args {
     uint64_t seed;           /* DEG_EM / DEG_SIEA: the seed of every trajectory when seeds is NULL */
     int64_t traj_base;       /* DEG_EM / DEG_SIEA: ensemble index of column 0; the noise of column i is the
                                 stream traj_base + i, so a chunked ensemble draws what one launch would */
-    float *inputs_ordered;   /* optional, persistent kernels with order: device array of (n + np) columns of
+    float *inputs_ordered;   /* optional, kernels compiled with DEG_F_ORDERED_INPUTS, with order: device array of (n + np) columns of
                                 8 * ceil(n_traj / 8) floats, the trajectories' u0 then p values in queue-major
                                 start-order position (order entry k at column index (k mod 8) * ceil(n_traj / 8)
                                 + k / 8: the entries each of the 8 queues serves, contiguously).
