@@ -15,7 +15,9 @@ lowest measured kernel time:
 The LDS depth of the saveat cache needs no tuning: it is sized to the save grid at every launch.
 
 The search is coordinate descent, starting from the defaults: the start order first (it changes
-which launch shape is best), then the other axes in the order above, then the start order again.
+which launch shape is best), then the other axes in the order above, then the start order again; the
+whole sweep runs twice, because the best register cap depends on the refill batch and chunk chosen
+after it (a second sweep re-times only new candidates).
 Each candidate is
 timed with HIP events around full relaunches of the solve (start-order kernels included; the best
 of ``repeats``) on the caller's problems, so the result reflects the real trajectory mix. Results are cached in
@@ -151,16 +153,19 @@ def tune(probs, prob, alg, *, adaptive: bool = True, repeats: int = 2, use_cache
         axes += [("refill", REFILL), ("chunk", CHUNK), ("blocks_per_cu", None)]
     if persistent and adaptive:
         axes += [("start_order", orders)]
-    for name, values in axes:
-        if values is None:
-            values = blocks_per_cu_candidates(cfg["block_size"])
-        best_v, best_t = cfg[name], measure(cfg)
-        for v in values:
-            c = dict(cfg, **{name: v})
-            t = measure(c)
-            if t < best_t:
-                best_v, best_t = v, t
-        cfg[name] = best_v
+    # two sweeps over the axes: the best register cap, for one, depends on the refill batch and chunk
+    # chosen after it (repeated candidates are timed once)
+    for _sweep in range(2):
+        for name, values in axes:
+            if values is None:
+                values = blocks_per_cu_candidates(cfg["block_size"])
+            best_v, best_t = cfg[name], measure(cfg)
+            for v in values:
+                c = dict(cfg, **{name: v})
+                t = measure(c)
+                if t < best_t:
+                    best_v, best_t = v, t
+            cfg[name] = best_v
     table = _load() if use_cache else {}
     table[key] = dict(config=cfg, kernel_ms=trials[tuple(sorted(cfg.items()))],
                       default_ms=trials.get(tuple(sorted(default.items()))), trials=len(trials))
