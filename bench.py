@@ -153,6 +153,19 @@ def flop_model(method, n, fc):
     return 25 * n + 8 + 3 * F + FJ + FT, n * n + n + (2 * n ** 3) // 3 + 6 * n * n, 4 * n + 8
 
 
+def cached_code_objects():
+    """Number of code objects in the package's kernel cache. The cache ships precompiled; a count that
+    grows during a run means a kernel was compiled in this process (reported as
+    config.kernels_compiled_at_run). Such kernels have measured up to 8 % slower than the same code
+    object loaded from the cache, so a timed run should report 0."""
+    from diffeqgpu_amd import _lib
+
+    try:
+        return sum(1 for e in os.listdir(_lib.CACHE_DIR) if e.endswith(".co"))
+    except OSError:
+        return 0
+
+
 def settle_clocks(torch, run, stream, settle_ms, max_relaunches=2000):
     """Relaunch the step, untimed, until settle_ms of GPU time has passed (at least once when
     settle_ms > 0): the GPU's clocks rise over the first ~0.1 s of sustained load, and the timed steps
@@ -517,6 +530,7 @@ def main_sde(args, name):
     deg = entry.load_package()
     from diffeqgpu_amd import systems, tracer
 
+    co0 = cached_code_objects()
     spec = SDE_WORKLOADS[name]
     steps = args.steps or spec["steps"]
     warmup = spec["warmup"] if args.warmup < 0 else args.warmup
@@ -570,7 +584,8 @@ def main_sde(args, name):
         "dtype": "f32",
         "data": "synthetic (the reference's SDE test problem; Philox4x32-10 Wiener increments, seed 123)",
         "config": {"workload": spec["desc"], "trajectories_per_step": N, "kernel_flags": res.flags,
-                   "block_size": dev.block_size or 256, "clock_settle": settle},
+                   "block_size": dev.block_size or 256, "clock_settle": settle,
+                   "kernels_compiled_at_run": cached_code_objects() - co0},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel_ms": float(np.mean(kernel_ms)),
                      "sde_steps_per_launch": nsteps, "flops_per_step": fstep, "philox_int_ops_per_step": istep,
@@ -629,6 +644,7 @@ def main():
     deg = entry.load_package()
     from diffeqgpu_amd import systems, tracer, workloads
 
+    co0 = cached_code_objects()
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
@@ -703,6 +719,7 @@ def main():
         per_step = N
         results = [res]
 
+    co_timed = cached_code_objects()
     settle = settle_clocks(torch, run, stream, args.settle_ms)
     for _ in range(warmup):
         run()
@@ -755,7 +772,8 @@ def main():
                    "waves": dev.waves, "refill": dev.refill or 4, "chunk": dev.chunk or 16, "autotuned": tuned,
                    "start_order": ("span" if name == "lorenz_uneven" and args.start_order == "span" and not args.no_order
                                    else dev.start_order),
-                   "cost_model": cost_model, "clock_settle": settle},
+                   "cost_model": cost_model, "clock_settle": settle,
+                   "kernels_compiled_at_run": {"before_timed_steps": co_timed - co0}},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
@@ -794,6 +812,7 @@ def main():
             out["fp64_oracle_error"]["reference_algorithm"] = {
                 "median": ref_err["median"], "p99": ref_err["p99"], "max": ref_err["max"],
                 "what": "the CPU restatement of the reference's Float32 algorithm on the same 1,000 trajectories"}
+    out["config"]["kernels_compiled_at_run"]["whole_run"] = cached_code_objects() - co0
     print(json.dumps(out))
 
 

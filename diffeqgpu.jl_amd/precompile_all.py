@@ -48,14 +48,16 @@ def variants():
                             HIPDevice(batch_saves=True, persistent=False)):
                     out.append((name, src, n, np_, m, dev.flags(True)))
             if m != _lib.DEG_RB23:
-                # the learned proposer and the step-log writer (default adaptive flags); the learned kernel
-                # also with the register caps its tuner tries on the benchmark systems
+                # the learned proposer and the step-log writer (default adaptive flags); on the benchmark
+                # systems also with the register caps the learned kernel's tuner tries and the tuned PI
+                # launches that write the training logs
                 base = HIPDevice().flags(True)
                 for extra in (_lib.F_LEARNED, _lib.F_STEPLOG, _lib.F_LEARNED | _lib.F_STEPLOG):
                     out.append((name, src, n, np_, m, base | extra))
                 if m == _lib.DEG_TSIT5 and name in ("lorenz", "vdp"):
                     for w in (4, 5, 6, 7):
-                        out.append((name, src, n, np_, m, HIPDevice(waves=w).flags(True) | _lib.F_LEARNED))
+                        for extra in (_lib.F_LEARNED, _lib.F_STEPLOG):
+                            out.append((name, src, n, np_, m, HIPDevice(waves=w).flags(True) | extra))
     # the other methods in the variants the parity tests launch (fixed step, bitwise adaptive)
     more = [(HIPDevice(contract="off"), False),
             (HIPDevice(contract="off", powmode="det", persistent=False), True),
