@@ -27,10 +27,31 @@ int deg_rng_fail(int code, const char* msg);  // thread-local last error (deghip
 namespace {
 
 constexpr int kBuckets = 2048;
-constexpr int kPerBlock = 4096;  // trajectories per block in the range, histogram and scatter passes
+constexpr int kThreads = 256;                    // block size of the range, histogram and scatter passes
+constexpr int kPerThread = 16;                   // trajectories per thread in those passes
+constexpr int kPerBlock = kThreads * kPerThread;  // trajectories per block
 
-__device__ __forceinline__ float span_at(const float* t0s, const float* tfs, float t0, float tf, long long i) {
-    return (tfs ? tfs[i] : tf) - (t0s ? t0s[i] : t0);
+// The spans of this thread's kPerThread trajectories (base + threadIdx.x + m * kThreads), with every
+// load issued before the first is used: a loop that used each value as it arrived waited out one
+// memory latency per trajectory. There is no branch around a load: which arrays exist is a template
+// parameter (kT0: t0s, kTF: tfs), and an index at or past n reads entry n - 1 (n >= 1) and yields NaN,
+// which falls into bucket 0 and must not be counted.
+template <bool kT0, bool kTF>
+__device__ __forceinline__ void load_spans(const float* t0s, const float* tfs, float t0, float tf, long long base,
+                                           long long n, float (&s)[kPerThread]) {
+    float a[kPerThread], b[kPerThread];
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m) {
+        const long long i = base + threadIdx.x + (long long)m * kThreads;
+        const long long ic = i < n ? i : n - 1;
+        a[m] = kTF ? tfs[ic] : tf;
+        b[m] = kT0 ? t0s[ic] : t0;
+    }
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m) {
+        const long long i = base + threadIdx.x + (long long)m * kThreads;
+        s[m] = i < n ? a[m] - b[m] : __builtin_nanf("");
+    }
 }
 
 // the smallest shift with (bmax - bmin) >> shift <= 2046; 0 when there is at most one positive value
@@ -58,20 +79,19 @@ __global__ void __launch_bounds__(256) clear_kernel(int* __restrict__ work) {
 }
 
 // bit patterns of the smallest and largest positive span: a block reduction, then one atomic pair
-__global__ void __launch_bounds__(256) range_kernel(const float* t0s, const float* tfs, float t0, float tf,
-                                                    long long n, unsigned* __restrict__ mm) {
-    __shared__ unsigned wlo[4], whi[4];
+template <bool kT0, bool kTF>
+__global__ void __launch_bounds__(kThreads) range_kernel(const float* t0s, const float* tfs, float t0, float tf,
+                                                         long long n, unsigned* __restrict__ mm) {
+    __shared__ unsigned wlo[kThreads / 64], whi[kThreads / 64];
+    float sp[kPerThread];
+    load_spans<kT0, kTF>(t0s, tfs, t0, tf, (long long)blockIdx.x * kPerBlock, n, sp);
     unsigned lo = 0xFFFFFFFFu, hi = 0u;
-    const long long base = (long long)blockIdx.x * kPerBlock;
-    for (int j = threadIdx.x; j < kPerBlock; j += blockDim.x) {
-        const long long i = base + j;
-        if (i < n) {
-            const float s = span_at(t0s, tfs, t0, tf, i);
-            if (s > 0.0f) {
-                const unsigned b = __builtin_bit_cast(unsigned, s);
-                lo = b < lo ? b : lo;
-                hi = b > hi ? b : hi;
-            }
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m) {
+        if (sp[m] > 0.0f) {  // false for NaN, so entries past n drop out
+            const unsigned b = __builtin_bit_cast(unsigned, sp[m]);
+            lo = b < lo ? b : lo;
+            hi = b > hi ? b : hi;
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -86,7 +106,7 @@ __global__ void __launch_bounds__(256) range_kernel(const float* t0s, const floa
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+        for (int k = 1; k < kThreads / 64; ++k) {
             lo = wlo[k] < lo ? wlo[k] : lo;
             hi = whi[k] > hi ? whi[k] : hi;
         }
@@ -98,20 +118,22 @@ __global__ void __launch_bounds__(256) range_kernel(const float* t0s, const floa
 }
 
 // per-block histogram in LDS, then one global atomic per non-empty bucket of the block
-__global__ void __launch_bounds__(256) histogram_kernel(const float* t0s, const float* tfs, float t0, float tf,
-                                                        long long n, int* __restrict__ count,
-                                                        const unsigned* __restrict__ mm) {
+template <bool kT0, bool kTF>
+__global__ void __launch_bounds__(kThreads) histogram_kernel(const float* t0s, const float* tfs, float t0, float tf,
+                                                             long long n, int* __restrict__ count,
+                                                             const unsigned* __restrict__ mm) {
     __shared__ int h[kBuckets];
-    const unsigned bmin = mm[0], shift = bucket_shift(mm[0], mm[1]);
-    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) h[b] = 0;
-    __syncthreads();
     const long long base = (long long)blockIdx.x * kPerBlock;
-    for (int j = threadIdx.x; j < kPerBlock; j += blockDim.x) {
-        const long long i = base + j;
-        if (i < n) atomicAdd(&h[span_bucket(span_at(t0s, tfs, t0, tf, i), bmin, shift)], 1);
-    }
+    float sp[kPerThread];
+    load_spans<kT0, kTF>(t0s, tfs, t0, tf, base, n, sp);
+    const unsigned bmin = mm[0], shift = bucket_shift(mm[0], mm[1]);
+    for (int b = threadIdx.x; b < kBuckets; b += kThreads) h[b] = 0;
     __syncthreads();
-    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x)
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m)
+        if (base + threadIdx.x + (long long)m * kThreads < n) atomicAdd(&h[span_bucket(sp[m], bmin, shift)], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < kBuckets; b += kThreads)
         if (h[b] != 0) atomicAdd(&count[b], h[b]);
 }
 
@@ -160,31 +182,36 @@ struct order_payload {
 
 // each block reserves one contiguous range per non-empty bucket with a single global atomic,
 // then places its trajectories inside the ranges through LDS atomics
-__global__ void __launch_bounds__(256) scatter_kernel(const float* t0s, const float* tfs, float t0, float tf,
-                                                      long long n, int* __restrict__ cursor,
-                                                      const int* __restrict__ count, int* __restrict__ order,
-                                                      const order_payload pl) {
+template <bool kT0, bool kTF>
+__global__ void __launch_bounds__(kThreads) scatter_kernel(const float* t0s, const float* tfs, float t0, float tf,
+                                                           long long n, int* __restrict__ cursor,
+                                                           const int* __restrict__ count, int* __restrict__ order,
+                                                           const order_payload pl) {
     __shared__ int h[kBuckets];
     __shared__ int start[kBuckets];
-    const unsigned bmin = (unsigned)count[0], shift = (unsigned)count[1];
-    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) h[b] = 0;
-    __syncthreads();
     const long long base = (long long)blockIdx.x * kPerBlock;
-    for (int j = threadIdx.x; j < kPerBlock; j += blockDim.x) {
-        const long long i = base + j;
-        if (i < n) atomicAdd(&h[span_bucket(span_at(t0s, tfs, t0, tf, i), bmin, shift)], 1);
-    }
+    float sp[kPerThread];
+    load_spans<kT0, kTF>(t0s, tfs, t0, tf, base, n, sp);
+    const unsigned bmin = (unsigned)count[0], shift = (unsigned)count[1];
+    int bk[kPerThread];  // bucket of each of this thread's trajectories (both passes use it)
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m) bk[m] = span_bucket(sp[m], bmin, shift);
+    for (int b = threadIdx.x; b < kBuckets; b += kThreads) h[b] = 0;
     __syncthreads();
-    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) {
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m)
+        if (base + threadIdx.x + (long long)m * kThreads < n) atomicAdd(&h[bk[m]], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < kBuckets; b += kThreads) {
         start[b] = (h[b] != 0) ? atomicAdd(&cursor[b], h[b]) : 0;
         h[b] = 0;
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < kPerBlock; j += blockDim.x) {
-        const long long i = base + j;
+#pragma unroll
+    for (int m = 0; m < kPerThread; ++m) {
+        const long long i = base + threadIdx.x + (long long)m * kThreads;
         if (i < n) {
-            const int b = span_bucket(span_at(t0s, tfs, t0, tf, i), bmin, shift);
-            const int pos = start[b] + atomicAdd(&h[b], 1);
+            const int pos = start[bk[m]] + atomicAdd(&h[bk[m]], 1);
             order[pos] = (int)i;
             if (pl.dst != nullptr) {
                 // coalesced reads of this block's trajectories, one store per column at their positions
@@ -196,6 +223,21 @@ __global__ void __launch_bounds__(256) scatter_kernel(const float* t0s, const fl
             }
         }
     }
+}
+
+// the range, histogram, scan and scatter passes for one combination of span arrays
+template <bool kT0, bool kTF>
+void launch_passes(hipStream_t st, unsigned blocks, const float* t0s, const float* tfs, float t0, float tf, long long n,
+                   int* work, int32_t* order, const order_payload& pl) {
+    int* count = work;
+    int* cursor = work + kBuckets;
+    unsigned* range = (unsigned*)cursor;  // (bmin, bmax) until the scan
+    hipLaunchKernelGGL((range_kernel<kT0, kTF>), dim3(blocks), dim3(kThreads), 0, st, t0s, tfs, t0, tf, n, range);
+    hipLaunchKernelGGL((histogram_kernel<kT0, kTF>), dim3(blocks), dim3(kThreads), 0, st, t0s, tfs, t0, tf, n, count,
+                       (const unsigned*)range);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, count, cursor);
+    hipLaunchKernelGGL((scatter_kernel<kT0, kTF>), dim3(blocks), dim3(kThreads), 0, st, t0s, tfs, t0, tf, n, cursor,
+                       (const int*)count, order, pl);
 }
 
 }  // namespace
@@ -236,15 +278,7 @@ int deg_order_payload(const float* t0s, const float* tfs, float t0, float tf, in
     if (n == 0) return DEG_OK;
     if (!order || !work) return deg_rng_fail(DEG_E_INVALID, "order and work are required");
     hipStream_t st = (hipStream_t)stream;
-    int* count = work;
-    int* cursor = work + kBuckets;
     const unsigned blocks = (unsigned)((n + kPerBlock - 1) / kPerBlock);
-    unsigned* range = (unsigned*)cursor;  // (bmin, bmax) until the scan
-    hipLaunchKernelGGL(clear_kernel, dim3(kBuckets / 256), dim3(256), 0, st, work);
-    hipLaunchKernelGGL(range_kernel, dim3(blocks), dim3(256), 0, st, t0s, tfs, t0, tf, (long long)n, range);
-    hipLaunchKernelGGL(histogram_kernel, dim3(blocks), dim3(256), 0, st, t0s, tfs, t0, tf, (long long)n, count,
-                       (const unsigned*)range);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, count, cursor);
     order_payload pl;
     pl.src0 = src0;
     pl.src1 = src1;
@@ -252,8 +286,15 @@ int deg_order_payload(const float* t0s, const float* tfs, float t0, float tf, in
     pl.ncol0 = src0 ? ncol0 : 0;
     pl.ncol1 = src1 ? ncol1 : 0;
     pl.dst = dst;
-    hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(256), 0, st, t0s, tfs, t0, tf, (long long)n, cursor,
-                       (const int*)count, order, pl);
+    hipLaunchKernelGGL(clear_kernel, dim3(kBuckets / 256), dim3(256), 0, st, work);
+    if (t0s && tfs)
+        launch_passes<true, true>(st, blocks, t0s, tfs, t0, tf, n, work, order, pl);
+    else if (t0s)
+        launch_passes<true, false>(st, blocks, t0s, tfs, t0, tf, n, work, order, pl);
+    else if (tfs)
+        launch_passes<false, true>(st, blocks, t0s, tfs, t0, tf, n, work, order, pl);
+    else
+        launch_passes<false, false>(st, blocks, t0s, tfs, t0, tf, n, work, order, pl);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return deg_rng_fail(DEG_E_HIP, (std::string("deg_order_by_span: ") + hipGetErrorString(e)).c_str());
     return DEG_OK;
