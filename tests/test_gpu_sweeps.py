@@ -162,6 +162,36 @@ def test_order_by_span_and_order_independence(deg, torch_cuda):
             np.testing.assert_array_equal(res[0][k][key], res[1][k][key])
 
 
+def test_order_by_span_layouts_and_ragged_sizes(deg, torch_cuda):
+    """deg_order_by_span for each combination of per-trajectory t0s / tfs arrays (present or scalar),
+    at sizes around the 4096-trajectory block of its passes (one trajectory, one short of a block,
+    exactly one, one over, three blocks and one), with zero, negative and NaN spans among the
+    keys: a permutation in non-increasing bucket order each time."""
+    torch = torch_cuda
+    from diffeqgpu_amd import _lib
+
+    rs = np.random.default_rng(5)
+    for N in (1, 4095, 4096, 4097, 12289):
+        t0s = rs.uniform(-1.0, 1.0, N).astype(F32)
+        tfs = (t0s + rs.lognormal(0.0, 2.0, N)).astype(F32)
+        if N > 8:
+            tfs[3] = t0s[3]            # zero span
+            tfs[5] = t0s[5] - 1.0      # negative span
+            tfs[7] = np.float32("nan")  # NaN span
+        for has_t0, has_tf in ((True, True), (True, False), (False, True), (False, False)):
+            t0_d = torch.as_tensor(t0s, device="cuda") if has_t0 else None
+            tf_d = torch.as_tensor(tfs, device="cuda") if has_tf else None
+            t0, tf = 0.25, 10.0
+            keys = (tfs if has_tf else np.full(N, tf, F32)) - (t0s if has_t0 else np.full(N, t0, F32))
+            order = torch.full((N,), -1, dtype=torch.int32, device="cuda")
+            work = torch.empty(_lib.ORDER_WORK_INTS, dtype=torch.int32, device="cuda")
+            print(f"[gpu] order_by_span N={N} t0s={has_t0} tfs={has_tf}", flush=True)
+            _lib.order_by_span(t0_d.data_ptr() if has_t0 else 0, tf_d.data_ptr() if has_tf else 0, t0, tf, N,
+                               order.data_ptr(), work.data_ptr(), torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+            OB.assert_descending(keys, order.cpu().numpy())
+
+
 def test_order_entries_outside_range_are_skipped(deg, torch_cuda):
     """A caller's start order with entries outside [0, N) solves every listed trajectory exactly
     as without an order and leaves the rest alone (the kernel's own bounds check)."""
