@@ -222,7 +222,7 @@ def test_learned_removes_rejections(deg, torch_cuda):
                           "error) is not met: the PI controller rejects about 3 % of attempts here and the proposer's "
                           "evaluation costs more than the attempts it saves", strict=False)
 def test_learned_throughput_target(deg, torch_cuda):
-    """The north star's target for the proposer: at least 1.5x the PI kernel's trajectories/s (kernel
+    """The throughput target for the proposer: at least 1.5x the PI kernel's trajectories/s (kernel
     time) on held-out Van der Pol trajectories at equal or lower median and 99th-percentile FP64 error
     (the bench's setting: target ratio 0.2, evaluated every 2nd iteration)."""
     pi, ml = _train_and_compare(deg, torch_cuda, rho=0.2, every=2)
