@@ -258,7 +258,7 @@ class EnsembleKernelAlgorithm:
 class EnsembleGPUKernel(EnsembleKernelAlgorithm):
     """``EnsembleGPUKernel(dev, cpu_offload = 0.0)`` (src/algorithms.jl:192-211).
 
-    ``cpu_offload`` must be 0.0: this build has no CPU solver path (the reference offloads a
+    ``cpu_offload`` must be 0.0: this library has no CPU solver path (the reference offloads a
     fraction of trajectories to SimpleDiffEq on host threads, solve.jl:37-81)."""
 
     def __init__(self, dev: HIPDevice | None = None, cpu_offload: float = 0.0):
