@@ -8,7 +8,7 @@
 // octave whatever their magnitude (a fixed split by the leading bits would give 1/8 octave, and a
 // narrow key range would fall into a handful of buckets). Five kernels on one stream: clear the
 // counters, the range (one atomic min/max pair per block), histogram, exclusive scan in descending
-// bucket order (one block), scatter. Histogram and scatter count in LDS per block of 4096 trajectories,
+// bucket order (one block), scatter. Histogram and scatter count in LDS per block of 8192 trajectories,
 // so the global atomics are one per non-empty bucket per block. Order within a bucket follows the
 // atomics and is not fixed; the solver's results do not depend on it (each lane's arithmetic is
 // independent of where and when it runs).
@@ -28,7 +28,7 @@ namespace {
 
 constexpr int kBuckets = 2048;
 constexpr int kThreads = 256;                    // block size of the range, histogram and scatter passes
-constexpr int kPerThread = 16;                   // trajectories per thread in those passes
+constexpr int kPerThread = 32;                   // trajectories per thread in those passes
 constexpr int kPerBlock = kThreads * kPerThread;  // trajectories per block
 
 // The spans of this thread's kPerThread trajectories (base + threadIdx.x + m * kThreads), with every

@@ -164,14 +164,14 @@ def test_order_by_span_and_order_independence(deg, torch_cuda):
 
 def test_order_by_span_layouts_and_ragged_sizes(deg, torch_cuda):
     """deg_order_by_span for each combination of per-trajectory t0s / tfs arrays (present or scalar),
-    at sizes around the 4096-trajectory block of its passes (one trajectory, one short of a block,
-    exactly one, one over, three blocks and one), with zero, negative and NaN spans among the
-    keys: a permutation in non-increasing bucket order each time."""
+    at sizes around the 8192-trajectory block of its passes (one trajectory, half a block, one short
+    of a block, exactly one, one over, three blocks and one), with zero, negative and NaN spans among
+    the keys: a permutation in non-increasing bucket order each time."""
     torch = torch_cuda
     from diffeqgpu_amd import _lib
 
     rs = np.random.default_rng(5)
-    for N in (1, 4095, 4096, 4097, 12289):
+    for N in (1, 4096, 8191, 8192, 8193, 24577):
         t0s = rs.uniform(-1.0, 1.0, N).astype(F32)
         tfs = (t0s + rs.lognormal(0.0, 2.0, N)).astype(F32)
         if N > 8:
